@@ -1,0 +1,289 @@
+"""gpugraph500_amd -- MI355X (gfx950) hot path of gpugraph500's 2D BFS.
+
+Python bindings (ctypes) over the C ABI declared in include/g5hip.h and
+implemented by libg5hip.so (HIP, built in-tree by gpugraph500_amd.build).
+There is no fallback: if the library is missing or fails to load, every
+entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libg5hip.so")
+
+c_i64 = ctypes.c_int64
+c_i32 = ctypes.c_int32
+c_sz = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+c_u64 = ctypes.c_uint64
+
+_lib = None
+
+
+class G5Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"g5 error {code}: {msg}")
+        self.code = code
+
+
+def _sig(L, name, res, args):
+    f = getattr(L, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def lib():
+    """Load libg5hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise G5Error(-100, f"{LIB_PATH} is missing; run python -m gpugraph500_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    I = ctypes.c_int
+    P = c_vp
+    _sig(L, "g5_last_error", ctypes.c_char_p, [])
+    _sig(L, "g5_init", I, [I])
+    _sig(L, "g5_device_sync", I, [])
+    _sig(L, "g5_dmalloc", I, [ctypes.POINTER(c_vp), c_sz])
+    _sig(L, "g5_dfree", I, [P])
+    _sig(L, "g5_memcpy_h2d", I, [P, P, c_sz])
+    _sig(L, "g5_memcpy_d2h", I, [P, P, c_sz])
+    _sig(L, "g5_graph_upload", I, [P, P, c_i64, c_i64, P, c_i64, c_i64, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_graph_kronecker", I, [I, I, c_u64, c_u64, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_graph_info", I, [P, P, P, P, P, P])
+    _sig(L, "g5_graph_download", I, [P, P, P, P])
+    _sig(L, "g5_graph_free", I, [P])
+    _sig(L, "g5_kronecker_edges", I, [I, c_u64, c_u64, c_i64, c_i64, P])
+    _sig(L, "g5_bfs_create", I, [P, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_bfs_free", I, [P])
+    _sig(L, "g5_bfs_start", I, [P, c_i64])
+    _sig(L, "g5_bfs_expand", I, [P, ctypes.POINTER(I)])
+    _sig(L, "g5_bfs_outqueue", I, [P, P, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_bfs_set_outqueue", I, [P, P, c_i64])
+    _sig(L, "g5_bfs_set_inqueue", I, [P, P, c_i64])
+    _sig(L, "g5_bfs_result", I, [P, P, P])
+    _sig(L, "g5_bfs_run", I, [P, c_i64, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_bfs_nedge", I, [P, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_s4_max_words", c_sz, [c_sz])
+    _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
+    _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
+    _sig(L, "g5_vreduce_or", I, [P, I, c_i64, P])
+    _sig(L, "g5_vreduce_min_pred", I, [P, I, c_i64, P])
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().g5_last_error()
+        raise G5Error(rc, msg.decode() if msg else "")
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_vp)
+
+
+def init(device=0):
+    check(lib().g5_init(device))
+
+
+class DeviceArray:
+    """A device (HBM) buffer owned by the library's allocator."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.ptr = c_vp()
+        check(lib().g5_dmalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)))
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            check(lib().g5_memcpy_h2d(d.ptr, _p(a), a.nbytes))
+        return d
+
+    def to_host(self, n=None):
+        n = self.n if n is None else int(n)
+        out = np.empty(n, self.dtype)
+        if n:
+            check(lib().g5_memcpy_d2h(_p(out), self.ptr, n * self.dtype.itemsize))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().g5_dfree(self.ptr)
+            self.ptr = c_vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Graph:
+    """One rank's DistMatrix2d block resident in HBM."""
+
+    def __init__(self, handle):
+        self.h = handle
+        info = [c_i64() for _ in range(5)]
+        check(lib().g5_graph_info(self.h, *[ctypes.byref(x) for x in info]))
+        self.n_rows, self.n_cols, self.nnz, self.nglobal, self.n_gen_edges = [x.value for x in info]
+
+    @classmethod
+    def kronecker(cls, scale, edgefactor=16, seed1=1, seed2=2):
+        h = c_vp()
+        check(lib().g5_graph_kronecker(scale, edgefactor, seed1, seed2, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def upload(cls, rowptr, cols, n_rows, n_cols=None, gdeg=None, row_start=0, col_start=0):
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        gd = None if gdeg is None else np.ascontiguousarray(gdeg, dtype=np.int64)
+        h = c_vp()
+        check(lib().g5_graph_upload(_p(rowptr), _p(cols) if cols.size else None, n_rows,
+                                    n_rows if n_cols is None else n_cols, _p(gd), row_start,
+                                    col_start, ctypes.byref(h)))
+        return cls(h)
+
+    def download(self):
+        rp = np.empty(self.n_rows + 1, np.int64)
+        cols = np.empty(max(self.nnz, 1), np.uint32)
+        gdeg = np.empty(max(self.n_rows, 1), np.int64)
+        check(lib().g5_graph_download(self.h, _p(rp), _p(cols), _p(gdeg)))
+        return rp, cols[:self.nnz], gdeg[:self.n_rows]
+
+    def free(self):
+        if self.h:
+            lib().g5_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def kronecker_edges(scale, start, end, seed1=1, seed2=2):
+    out = np.empty(2 * (end - start), np.int64)
+    check(lib().g5_kronecker_edges(scale, seed1, seed2, start, end, _p(out)))
+    return out.reshape(-1, 2)
+
+
+class BFS:
+    """Single-root local expansion (the CUDA_BFS plug-in's hooks)."""
+
+    def __init__(self, graph):
+        self.graph = graph
+        self.h = c_vp()
+        check(lib().g5_bfs_create(graph.h, ctypes.byref(self.h)))
+
+    def start(self, root):
+        check(lib().g5_bfs_start(self.h, root))
+
+    def expand(self):
+        a = ctypes.c_int()
+        check(lib().g5_bfs_expand(self.h, ctypes.byref(a)))
+        return bool(a.value)
+
+    def outqueue(self):
+        q = np.empty(max(self.graph.n_cols, 1), np.uint32)
+        n = c_i64(q.size)
+        check(lib().g5_bfs_outqueue(self.h, _p(q), ctypes.byref(n)))
+        return q[:n.value].copy()
+
+    def set_outqueue(self, q=None):
+        if q is None:
+            check(lib().g5_bfs_set_outqueue(self.h, None, 0))
+        else:
+            q = np.ascontiguousarray(q, dtype=np.uint32)
+            check(lib().g5_bfs_set_outqueue(self.h, _p(q) if q.size else None, q.size))
+
+    def set_inqueue(self, q=None):
+        if q is None:
+            check(lib().g5_bfs_set_inqueue(self.h, None, 0))
+        else:
+            q = np.ascontiguousarray(q, dtype=np.uint32)
+            check(lib().g5_bfs_set_inqueue(self.h, _p(q) if q.size else np.zeros(1, np.uint32).ctypes.data_as(c_vp), q.size))
+
+    def result(self):
+        pred = np.empty(self.graph.n_cols, np.int64)
+        lev = np.empty(self.graph.n_cols, np.int32)
+        check(lib().g5_bfs_result(self.h, _p(pred), _p(lev)))
+        return pred, lev
+
+    def run(self, root, codec=False):
+        nl = c_i32()
+        ms = ctypes.c_float()
+        check(lib().g5_bfs_run(self.h, root, 1 if codec else 0, ctypes.byref(nl), ctypes.byref(ms)))
+        return nl.value, ms.value
+
+    def nedge(self):
+        n = c_i64()
+        check(lib().g5_bfs_nedge(self.h, ctypes.byref(n)))
+        return n.value
+
+    def free(self):
+        if self.h:
+            lib().g5_bfs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def s4_max_words(n):
+    return lib().g5_s4_max_words(n)
+
+
+def s4_encode(x):
+    """Host uint32 array -> reference-identical word stream, encoded on the GPU."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    din = DeviceArray.from_host(x) if x.size else DeviceArray(1, np.uint32)
+    cap = s4_max_words(x.size)
+    dout = DeviceArray(cap, np.uint32)
+    nw = c_sz()
+    check(lib().g5_s4_encode(din.ptr, x.size, dout.ptr, cap, ctypes.byref(nw)))
+    return dout.to_host(nw.value)
+
+
+def s4_decode(words, cap):
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    din = DeviceArray.from_host(w)
+    dout = DeviceArray(max(cap, 1), np.uint32)
+    n = c_sz()
+    check(lib().g5_s4_decode(din.ptr, w.size, dout.ptr, cap, ctypes.byref(n)))
+    return dout.to_host(n.value)
+
+
+def vreduce_or(maps):
+    maps = np.ascontiguousarray(maps, dtype=np.uint64)
+    k, n = maps.shape
+    d = DeviceArray.from_host(maps.reshape(-1))
+    o = DeviceArray(n, np.uint64)
+    check(lib().g5_vreduce_or(d.ptr, k, n, o.ptr))
+    return o.to_host()
+
+
+def vreduce_min_pred(preds):
+    preds = np.ascontiguousarray(preds, dtype=np.int64)
+    k, n = preds.shape
+    d = DeviceArray.from_host(preds.reshape(-1))
+    o = DeviceArray(n, np.int64)
+    check(lib().g5_vreduce_min_pred(d.ptr, k, n, o.ptr))
+    return o.to_host()

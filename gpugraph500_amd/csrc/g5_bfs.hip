@@ -1,0 +1,458 @@
+// g5_bfs.hip -- single-root local expansion: the GPU "Derived" plug-in of
+// GlobalBFS (cpu_2d/cuda/cuda_bfs.h:39-84) re-designed for gfx950.
+//
+// State per search lives in HBM as bitmaps (visited, out) plus a sorted
+// uint32 in-queue.  A level is:
+//   1. degree prefix over the in-queue (exclusive scan of row lengths),
+//   2. k_expand: edge-balanced top-down expansion -- every block takes a
+//      2048-edge tile of the level's edge range, finds each edge's source by
+//      binary search in the degree prefix, and for an unvisited target sets
+//      its out bit (atomicOr, skipped when already set) and lowers its parent
+//      with atomicMin.  The surviving parent is the minimum-id frontier
+//      neighbour: the same vertex CPUBFS_bin's ascending scan keeps
+//      (cpu_2d/cpubfs_bin.cpp:150-173), independent of thread timing.
+//   3. k_commit: visited |= out, level := depth + 1, count of new vertices.
+//   4. out bitmap -> sorted queue (word popcount scan), the next in-queue.
+// Queues are born sorted and unique, so the reference's thrust::sort and host
+// unique_copy (cuda_bfs.cu:380-475) have no counterpart.
+#include <algorithm>
+
+#include "g5_codec.hpp"
+#include "g5_internal.hpp"
+
+namespace {
+
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;
+constexpr uint32_t kRootMark = 0xFFFFFFFEu;  // parent slot of the root column
+constexpr int kExpandBlock = 256;
+constexpr int kExpandTile = 2048;            // edges per block tile
+constexpr int kExpandGrid = 4096;            // persistent grid, grid-stride over tiles
+
+enum Cnt { kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNCnt = 8 };
+
+}  // namespace
+
+struct g5_bfs {
+    g5_graph *g = nullptr;
+    hipStream_t s = nullptr;
+    int64_t wr = 0, wc = 0;  // row / column bitmap words
+    uint64_t *d_visited = nullptr, *d_out = nullptr;
+    uint32_t *d_parent = nullptr;
+    int32_t *d_level = nullptr;
+    uint32_t *d_queue = nullptr;  // in-queue (local rows), sorted
+    uint32_t *d_outq = nullptr;   // out-queue (local columns), sorted
+    int64_t *d_qoff = nullptr;    // degree prefix over the in-queue, qlen + 1
+    int64_t *d_scratch = nullptr;
+    int64_t *d_cnt = nullptr;
+    int64_t *h_cnt = nullptr;     // pinned mirror of d_cnt
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    g5::S4Codec codec;
+    uint32_t *d_words = nullptr;  // codec stream for G5_RUN_CODEC
+    size_t words_cap = 0;
+    int64_t root = -1;
+    int depth = -1;
+    int64_t qlen = 0;
+};
+
+namespace {
+
+__device__ __forceinline__ int64_t src_of(const int64_t *__restrict__ qoff, int64_t lo, int64_t hi,
+                                          int64_t e)
+{
+    // last s in [lo, hi] with qoff[s] <= e
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (qoff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_queue_degrees(const uint32_t *__restrict__ q, const int64_t *__restrict__ d_qlen,
+                                const int64_t *__restrict__ rowptr, int64_t *__restrict__ qoff)
+{
+    const int64_t n = *d_qlen;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = q[i];
+        qoff[i] = rowptr[u + 1] - rowptr[u];
+    }
+}
+
+__global__ __launch_bounds__(kExpandBlock) void k_expand(
+    const uint32_t *__restrict__ q, const int64_t *__restrict__ qoff, const int64_t *__restrict__ cnt,
+    const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+    const uint64_t *__restrict__ visited, uint64_t *__restrict__ out, uint32_t *__restrict__ parent)
+{
+    __shared__ int64_t s_lo, s_hi;
+    const int64_t qn = cnt[kQLen];
+    const int64_t E = cnt[kEdges];
+    for (int64_t t0 = blockIdx.x * (int64_t)kExpandTile; t0 < E; t0 += (int64_t)gridDim.x * kExpandTile) {
+        const int64_t t1 = min(E, t0 + (int64_t)kExpandTile);
+        if (threadIdx.x == 0) {
+            s_lo = src_of(qoff, 0, qn - 1, t0);
+            s_hi = src_of(qoff, s_lo, qn - 1, t1 - 1);
+        }
+        __syncthreads();
+        const int64_t lo = s_lo, hi = s_hi;
+        for (int64_t e = t0 + threadIdx.x; e < t1; e += kExpandBlock) {
+            const int64_t si = src_of(qoff, lo, hi, e);
+            const uint32_t u = q[si];
+            const uint32_t v = cols[rowptr[u] + (e - qoff[si])];
+            const uint64_t bit = 1ull << (v & 63);
+            if (visited[v >> 6] & bit) continue;
+            if (!(out[v >> 6] & bit)) atomicOr((unsigned long long *)&out[v >> 6], (unsigned long long)bit);
+            if (parent[v] > u) atomicMin(&parent[v], u);
+        }
+        __syncthreads();
+    }
+}
+
+// visited |= out (minus already visited), level := depth for new bits, count.
+__global__ void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visited,
+                         int32_t *__restrict__ level, int64_t wc, int32_t depth, int64_t *__restrict__ cnt)
+{
+    int64_t c = 0;
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < wc;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t o = out[w];
+        if (!o) continue;
+        const uint64_t vis = visited[w];
+        const uint64_t nw = o & ~vis;
+        if (nw != o) out[w] = nw;
+        visited[w] = vis | nw;
+        c += __popcll(nw);
+        uint64_t b = nw;
+        while (b) {
+            const int i = __ffsll((long long)b) - 1;
+            level[w * 64 + i] = depth;
+            b &= b - 1;
+        }
+    }
+    c = g5_wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)&cnt[kNew], (unsigned long long)c);
+}
+
+__global__ void k_count_bits(const uint64_t *__restrict__ bits, int64_t n, int64_t *__restrict__ dst)
+{
+    int64_t c = 0;
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n;
+         w += (int64_t)gridDim.x * blockDim.x)
+        c += __popcll(bits[w]);
+    c = g5_wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
+}
+
+__global__ void k_start(int64_t root_row, int64_t root_col, uint32_t *queue, uint64_t *visited,
+                        int32_t *level, uint32_t *parent, int64_t *cnt)
+{
+    cnt[kQLen] = root_row >= 0 ? 1 : 0;
+    if (root_row >= 0) queue[0] = (uint32_t)root_row;
+    if (root_col >= 0) {
+        visited[root_col >> 6] = 1ull << (root_col & 63);
+        level[root_col] = 0;
+        parent[root_col] = kRootMark;
+    }
+}
+
+__global__ void k_result(const uint32_t *__restrict__ parent, int64_t n, int64_t row_start,
+                         int64_t col_start, int64_t *__restrict__ pred)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = parent[i];
+        pred[i] = p == kNoParent ? -1 : (p == kRootMark ? i + col_start : (int64_t)p + row_start);
+    }
+}
+
+__global__ void k_nedge(const uint32_t *__restrict__ gdeg, const int32_t *__restrict__ level, int64_t n,
+                        int64_t *__restrict__ dst)
+{
+    int64_t c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (level[i] >= 0) c += gdeg[i];
+    c = g5_wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
+}
+
+__global__ void k_set_cnt(int64_t *cnt, int idx, int64_t v) { cnt[idx] = v; }
+__global__ void k_copy_cnt(int64_t *cnt, int dst, int src) { cnt[dst] = cnt[src]; }
+
+// One level's expansion from the current in-queue (d_queue, cnt[kQLen]).
+int expand_level(g5_bfs *b)
+{
+    g5_graph *g = b->g;
+    hipStream_t s = b->s;
+    G5_HIP(g5::fill_u64(b->d_out, b->wc, 0, s));
+    hipLaunchKernelGGL(k_queue_degrees, g5::grid_for(g->n_rows, 256, 4096), 256, 0, s,
+                       (const uint32_t *)b->d_queue, (const int64_t *)b->d_cnt + kQLen,
+                       (const int64_t *)g->d_rowptr, b->d_qoff);
+    G5_HIP(hipGetLastError());
+    // the scan length must be known on the host: it is the in-queue length
+    G5_HIP(g5::exclusive_scan_i64(b->d_qoff, b->qlen, b->d_cnt + kEdges, b->d_scratch, s));
+    if (b->qlen > 0) {
+        // qoff[qlen] = total, so src_of never reads past the prefix
+        hipLaunchKernelGGL(k_copy_cnt, 1, 1, 0, s, b->d_cnt, (int)kEdges, (int)kEdges);
+        G5_HIP(hipMemcpyAsync(b->d_qoff + b->qlen, b->d_cnt + kEdges, 8, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_expand, kExpandGrid, kExpandBlock, 0, s, (const uint32_t *)b->d_queue,
+                           (const int64_t *)b->d_qoff, (const int64_t *)b->d_cnt,
+                           (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_cols,
+                           (const uint64_t *)b->d_visited, b->d_out, b->d_parent);
+        G5_HIP(hipGetLastError());
+    }
+    return G5_OK;
+}
+
+int sync_counters(g5_bfs *b)
+{
+    G5_HIP(hipMemcpyAsync(b->h_cnt, b->d_cnt, kNCnt * 8, hipMemcpyDeviceToHost, b->s));
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
+int reset_search(g5_bfs *b, int64_t root)
+{
+    g5_graph *g = b->g;
+    hipStream_t s = b->s;
+    const int64_t rr = root - g->row_start, rc = root - g->col_start;
+    const int64_t root_row = (root >= 0 && rr >= 0 && rr < g->n_rows) ? rr : -1;
+    const int64_t root_col = (root >= 0 && rc >= 0 && rc < g->n_cols) ? rc : -1;
+    G5_HIP(g5::fill_u64(b->d_visited, b->wc, 0, s));
+    G5_HIP(g5::fill_u64(b->d_out, b->wc, 0, s));
+    G5_HIP(g5::fill_u32(b->d_parent, g->n_cols, kNoParent, s));
+    G5_HIP(g5::fill_i32(b->d_level, g->n_cols, -1, s));
+    G5_HIP(g5::fill_i64(b->d_cnt, kNCnt, 0, s));
+    hipLaunchKernelGGL(k_start, 1, 1, 0, s, root_row, root_col, b->d_queue, b->d_visited, b->d_level,
+                       b->d_parent, b->d_cnt);
+    G5_HIP(hipGetLastError());
+    b->root = root;
+    b->depth = 0;
+    b->qlen = root_row >= 0 ? 1 : 0;
+    return G5_OK;
+}
+
+}  // namespace
+
+extern "C" int g5_bfs_create(g5_graph *g, g5_bfs **out)
+{
+    G5_REQUIRE(g && out, G5_E_ARG, "g5_bfs_create: null argument");
+    g5_bfs *b = new g5_bfs();
+    b->g = g;
+    b->wr = g->n_rows / 64;
+    b->wc = g->n_cols / 64;
+    auto fail = [&](int rc) {
+        g5_bfs_free(b);
+        return rc;
+    };
+    hipError_t e = hipStreamCreate(&b->s);
+    const int64_t nmax = std::max(g->n_rows, g->n_cols);
+    const size_t scr = g5::scan_scratch_words(nmax + 1) + g5::scan_scratch_words(b->wc + 1) + 64;
+    if (e == hipSuccess) e = hipMalloc(&b->d_visited, (size_t)std::max<int64_t>(b->wc, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_out, (size_t)std::max<int64_t>(b->wc, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_parent, (size_t)std::max<int64_t>(g->n_cols, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&b->d_level, (size_t)std::max<int64_t>(g->n_cols, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&b->d_queue, (size_t)std::max<int64_t>(nmax, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&b->d_outq, (size_t)std::max<int64_t>(nmax, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&b->d_qoff, (size_t)(g->n_rows + 2) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_scratch, scr * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_cnt, kNCnt * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_cnt, kNCnt * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_bfs_create allocation", __FILE__, __LINE__));
+    *out = b;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_free(g5_bfs *b)
+{
+    if (!b) return G5_OK;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    void *dev[] = {b->d_visited, b->d_out, b->d_parent, b->d_level, b->d_queue,
+                   b->d_outq,    b->d_qoff, b->d_scratch, b->d_cnt, b->d_words};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (b->h_cnt) (void)hipHostFree(b->h_cnt);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    b->codec.release();
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_start(g5_bfs *b, int64_t root)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_start: null handle");
+    int rc = reset_search(b, root);
+    if (rc) return rc;
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_expand(g5_bfs *b, int *any_new)
+{
+    G5_REQUIRE(b && any_new, G5_E_ARG, "g5_bfs_expand: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_expand: g5_bfs_start not called");
+    int rc = expand_level(b);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kOutLen, (int64_t)0);
+    hipLaunchKernelGGL(k_count_bits, g5::grid_for(b->wc, 256, 2048), 256, 0, b->s,
+                       (const uint64_t *)b->d_out, b->wc, b->d_cnt + kOutLen);
+    G5_HIP(hipGetLastError());
+    rc = sync_counters(b);
+    if (rc) return rc;
+    *any_new = b->h_cnt[kOutLen] > 0;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_outqueue(g5_bfs *b, uint32_t *q, int64_t *n)
+{
+    G5_REQUIRE(b && n, G5_E_ARG, "g5_bfs_outqueue: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_outqueue: g5_bfs_start not called");
+    G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_outq, b->d_cnt + kOutLen, b->d_scratch, b->s));
+    int rc = sync_counters(b);
+    if (rc) return rc;
+    const int64_t len = b->h_cnt[kOutLen];
+    G5_REQUIRE(len <= *n, G5_E_OVERFLOW, "g5_bfs_outqueue: queue of %lld entries exceeds capacity %lld",
+               (long long)len, (long long)*n);
+    *n = len;
+    if (len) {
+        G5_REQUIRE(q, G5_E_ARG, "g5_bfs_outqueue: null queue buffer");
+        G5_HIP(hipMemcpy(q, b->d_outq, (size_t)len * 4, hipMemcpyDeviceToHost));
+    }
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_set_outqueue(g5_bfs *b, const uint32_t *q, int64_t n)
+{
+    G5_REQUIRE(b && n >= 0 && (q || n == 0 || !q), G5_E_ARG, "g5_bfs_set_outqueue: bad arguments");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_outqueue: g5_bfs_start not called");
+    g5_graph *g = b->g;
+    if (q) {
+        G5_REQUIRE(n <= g->n_cols, G5_E_OVERFLOW, "g5_bfs_set_outqueue: queue longer than the column slice");
+        for (int64_t i = 0; i < n; ++i)
+            G5_REQUIRE(q[i] < (uint64_t)g->n_cols && (i == 0 || q[i] > q[i - 1]), G5_E_ARG,
+                       "g5_bfs_set_outqueue: queue not sorted/unique/in range at %lld", (long long)i);
+        G5_HIP(g5::fill_u64(b->d_out, b->wc, 0, b->s));
+        if (n) {
+            G5_HIP(hipMemcpyAsync(b->d_outq, q, (size_t)n * 4, hipMemcpyHostToDevice, b->s));
+            G5_HIP(g5::queue_to_bitmap(b->d_outq, n, b->d_out, b->s));
+        }
+    }
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
+    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
+                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_set_inqueue(g5_bfs *b, const uint32_t *q, int64_t n)
+{
+    G5_REQUIRE(b && n >= 0, G5_E_ARG, "g5_bfs_set_inqueue: bad arguments");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_inqueue: g5_bfs_start not called");
+    g5_graph *g = b->g;
+    if (q) {
+        G5_REQUIRE(n <= g->n_rows, G5_E_OVERFLOW, "g5_bfs_set_inqueue: queue longer than the row slice");
+        for (int64_t i = 0; i < n; ++i)
+            G5_REQUIRE(q[i] < (uint64_t)g->n_rows && (i == 0 || q[i] > q[i - 1]), G5_E_ARG,
+                       "g5_bfs_set_inqueue: queue not sorted/unique/in range at %lld", (long long)i);
+        if (n) G5_HIP(hipMemcpyAsync(b->d_queue, q, (size_t)n * 4, hipMemcpyHostToDevice, b->s));
+        hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kQLen, n);
+        b->qlen = n;
+    } else {
+        G5_REQUIRE(g->n_rows == g->n_cols && g->row_start == g->col_start, G5_E_STATE,
+                   "g5_bfs_set_inqueue: own out-queue as in-queue needs a diagonal block");
+        G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
+        int rc = sync_counters(b);
+        if (rc) return rc;
+        b->qlen = b->h_cnt[kQLen];
+    }
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(b->s));
+    b->depth++;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_result(g5_bfs *b, int64_t *pred, int32_t *level)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_result: null handle");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_result: no search");
+    g5_graph *g = b->g;
+    if (pred) {
+        int64_t *d = nullptr;
+        G5_HIP(hipMallocAsync((void **)&d, (size_t)std::max<int64_t>(g->n_cols, 1) * 8, b->s));
+        hipLaunchKernelGGL(k_result, g5::grid_for(g->n_cols, 256, 8192), 256, 0, b->s,
+                           (const uint32_t *)b->d_parent, g->n_cols, g->row_start, g->col_start, d);
+        G5_HIP(hipGetLastError());
+        G5_HIP(hipMemcpyAsync(pred, d, (size_t)g->n_cols * 8, hipMemcpyDeviceToHost, b->s));
+        G5_HIP(hipFreeAsync(d, b->s));
+    }
+    if (level) G5_HIP(hipMemcpyAsync(level, b->d_level, (size_t)g->n_cols * 4, hipMemcpyDeviceToHost, b->s));
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_nedge(g5_bfs *b, int64_t *nedge)
+{
+    G5_REQUIRE(b && nedge, G5_E_ARG, "g5_bfs_nedge: null argument");
+    G5_REQUIRE(b->g->n_rows == b->g->n_cols, G5_E_STATE, "g5_bfs_nedge: needs a 1x1 block");
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kOutLen, (int64_t)0);
+    hipLaunchKernelGGL(k_nedge, g5::grid_for(b->g->n_rows, 256, 4096), 256, 0, b->s,
+                       (const uint32_t *)b->g->d_gdeg, (const int32_t *)b->d_level, b->g->n_rows,
+                       b->d_cnt + kOutLen);
+    G5_HIP(hipGetLastError());
+    int rc = sync_counters(b);
+    if (rc) return rc;
+    *nedge = b->h_cnt[kOutLen] / 2;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_run: null handle");
+    g5_graph *g = b->g;
+    G5_REQUIRE(g->n_rows == g->n_cols && g->row_start == g->col_start, G5_E_STATE,
+               "g5_bfs_run: the resident loop needs a diagonal (1x1) block");
+    G5_REQUIRE(root >= 0 && root < g->n_rows, G5_E_ARG, "g5_bfs_run: root %lld out of range", (long long)root);
+    const bool codec = (flags & G5_RUN_CODEC) != 0;
+    if (codec && !b->d_words) {
+        b->words_cap = g5_s4_max_words((size_t)g->n_cols);
+        G5_HIP(hipMalloc(&b->d_words, b->words_cap * 4));
+        int rc = b->codec.reserve((int64_t)g->n_cols);
+        if (rc) return rc;
+    }
+    G5_HIP(hipStreamSynchronize(b->s));
+    G5_HIP(hipEventRecord(b->ev0, b->s));
+    int rc = reset_search(b, root);
+    if (rc) return rc;
+    for (;;) {
+        rc = expand_level(b);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
+        hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out,
+                           b->d_visited, b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+        G5_HIP(hipGetLastError());
+        G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
+        rc = sync_counters(b);
+        if (rc) return rc;
+        if (b->h_cnt[kNew] == 0) break;
+        b->qlen = b->h_cnt[kQLen];
+        if (codec) {
+            size_t nw = 0, nd = 0;
+            rc = b->codec.encode(b->d_queue, b->qlen, b->d_words, b->words_cap, &nw, b->s);
+            if (rc) return rc;
+            rc = b->codec.decode(b->d_words, nw, b->d_queue, (size_t)g->n_rows, &nd, b->s);
+            if (rc) return rc;
+            G5_REQUIRE((int64_t)nd == b->qlen, G5_E_STREAM, "g5_bfs_run: codec round trip lost values");
+        }
+        b->depth++;
+    }
+    G5_HIP(hipEventRecord(b->ev1, b->s));
+    G5_HIP(hipEventSynchronize(b->ev1));
+    if (ms) G5_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    if (nlevels) *nlevels = b->depth + 1;
+    return G5_OK;
+}

@@ -1,0 +1,550 @@
+// g5_codec.hip -- s4-bp128-d4 on the device, word-for-word the stream that
+// CompositeCodec<SIMDBinaryPacking<SIMDIntegratedBlockPacker<CoarseDelta4SIMD,
+// true>>, VariableByte<true>> writes (cpu_2d/compression/cpusimd/include/
+// codecfactory.h:103-105) into a 16-byte aligned buffer.
+//
+// Stream (uint32 words):
+//   [L = n/128*128, 123456, 123456, 123456]            simdbinarypacking.h:182-183
+//   per group of <= 16 miniblocks of 128 values:
+//     4 header words, 8-bit widths MSB first            simdbinarypacking.h:195-202
+//     per miniblock 4*b words: 4-strided deltas d[k] = x[k] - x[k-4]
+//     (x[-4..-1] = previous miniblock's last values, 0 at the start),
+//     value k in lane k&3 at bit (k>>2)*b of that lane's word stream;
+//     b = 32 stores the raw values, b = 0 stores nothing
+//   varint tail of n - L values, deltas restarting at 0, 7-bit groups LSB
+//   first, stop bit on the last byte, zero padded to a word (variablebyte.h).
+//
+// Encode: one wavefront per miniblock computes b with a wave OR-reduce; an
+// exclusive scan of 4*b gives every miniblock's position; a second pass packs
+// (each lane assembles whole output words from the deltas staged in LDS, so
+// stores are coalesced and need no atomics); one wave writes the varint tail.
+// Decode: one workgroup walks the group headers through LDS-staged chunks
+// (their positions depend on all earlier widths), then one wave per
+// miniblock unpacks; the 4-lane running sums across miniblocks come from an
+// exclusive scan of per-miniblock (reset, sum) records, where b = 32
+// miniblocks reset the running value to their raw last four values.
+#include "g5_codec.hpp"
+
+namespace {
+
+constexpr uint32_t kCookie = 123456u;
+
+struct Carry {
+    uint32_t r, v[4];
+};
+
+__device__ __forceinline__ Carry comb(const Carry &a, const Carry &b)
+{
+    if (b.r) return b;
+    Carry c = a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c.v[j] += b.v[j];
+    return c;
+}
+
+__device__ __forceinline__ Carry load_carry(const uint32_t *p, int64_t i)
+{
+    Carry c;
+    c.r = p[5 * i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c.v[j] = p[5 * i + 1 + j];
+    return c;
+}
+
+__device__ __forceinline__ void store_carry(uint32_t *p, int64_t i, const Carry &c)
+{
+    p[5 * i] = c.r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p[5 * i + 1 + j] = c.v[j];
+}
+
+__device__ __forceinline__ uint32_t width_of(uint32_t acc) { return acc ? 32u - __clz(acc) : 0u; }
+
+// ------------------------------------------------------------------ encode
+
+__global__ __launch_bounds__(256) void k_enc_widths(const uint32_t *__restrict__ in, int64_t nblk,
+                                                    int64_t *__restrict__ words)
+{
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nblk) return;
+    const int l = threadIdx.x & 63;
+    const int64_t base = i * 128;
+    const uint32_t x0 = in[base + l], x1 = in[base + l + 64];
+    const uint32_t r0 = (base + l >= 4) ? in[base + l - 4] : 0u;
+    const uint32_t r1 = in[base + l + 60];
+    const uint32_t acc = g5_wave_or32((x0 - r0) | (x1 - r1));
+    if (l == 0) words[i] = 4 * (int64_t)width_of(acc);
+}
+
+__global__ __launch_bounds__(256) void k_enc_pack(const uint32_t *__restrict__ in, int64_t nblk,
+                                                  const int64_t *__restrict__ P, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t lds[4][128];
+    const int wv = threadIdx.x >> 6;
+    const int64_t i = blockIdx.x * 4 + wv;
+    if (i >= nblk) return;
+    const int l = threadIdx.x & 63;
+    const int64_t base = i * 128;
+    const int64_t g = i >> 4;
+    const uint32_t b = (uint32_t)((P[i + 1] - P[i]) >> 2);
+    if ((i & 15) == 0 && l < 4) {  // group header: widths of miniblocks 16g .. 16g+15
+        uint32_t w = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t j = i + 4 * l + t;
+            const uint32_t bj = (j < nblk) ? (uint32_t)((P[j + 1] - P[j]) >> 2) : 0u;
+            w |= bj << (24 - 8 * t);
+        }
+        out[4 + 4 * g + P[i] + l] = w;
+    }
+    uint32_t *dst = out + 8 + 4 * g + P[i];
+    const uint32_t x0 = in[base + l], x1 = in[base + l + 64];
+    if (b == 32) {
+        dst[l] = x0;
+        dst[l + 64] = x1;
+        return;
+    }
+    if (b == 0) return;
+    const uint32_t r0 = (base + l >= 4) ? in[base + l - 4] : 0u;
+    const uint32_t r1 = in[base + l + 60];
+    lds[wv][l] = x0 - r0;
+    lds[wv][l + 64] = x1 - r1;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t t = l; t < 4 * b; t += 64) {
+        const uint32_t j = t & 3, w = t >> 2, lo = 32 * w;
+        uint32_t word = 0;
+        const uint32_t i0 = lo / b;
+        uint32_t i1 = (lo + 31) / b;
+        if (i1 > 31) i1 = 31;
+        for (uint32_t idx = i0; idx <= i1; ++idx) {
+            const int sh = (int)(idx * b) - (int)lo;
+            const uint32_t v = lds[wv][4 * idx + j];
+            word |= sh >= 0 ? (v << sh) : (v >> (-sh));
+        }
+        dst[t] = word;
+    }
+}
+
+__device__ __forceinline__ uint32_t varint_len(uint32_t v)
+{
+    return v < (1u << 7) ? 1u : v < (1u << 14) ? 2u : v < (1u << 21) ? 3u : v < (1u << 28) ? 4u : 5u;
+}
+
+// stream header + varint tail; one wave.  meta[0] = total words.
+__global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in, int64_t n, int64_t L,
+                                                 int64_t nblk, const int64_t *__restrict__ P,
+                                                 uint32_t *__restrict__ out, int64_t *__restrict__ meta)
+{
+    __shared__ uint8_t bytes[128 * 5 + 8];
+    const int l = threadIdx.x;
+    if (l == 0) {
+        out[0] = (uint32_t)L;
+        out[1] = out[2] = out[3] = kCookie;
+    }
+    const int64_t tailpos = 4 + 4 * ((nblk + 15) / 16) + P[nblk];
+    const int64_t nt = n - L;
+    for (int k = l; k < 128 * 5 + 8; k += 64) bytes[k] = 0;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int r = 0; r < 2; ++r) {
+        const int k = l + 64 * r;
+        uint32_t v = 0, len = 0;
+        if (k < nt) {
+            v = in[L + k] - (k ? in[L + k - 1] : 0u);
+            len = varint_len(v);
+        }
+        const int64_t inc = g5_wave_incl_scan((int64_t)len);
+        uint32_t pos = run + (uint32_t)(inc - len);
+        for (uint32_t q = 0; q < len; ++q) {
+            const uint32_t last = (q + 1 == len);
+            bytes[pos + q] = (uint8_t)(((v >> (7 * q)) & 127u) | (last ? 128u : 0u));
+        }
+        run += (uint32_t)__shfl(inc, 63, 64);
+    }
+    __syncthreads();
+    const uint32_t nw = (run + 3) / 4;
+    for (uint32_t w = l; w < nw; w += 64)
+        out[tailpos + w] = (uint32_t)bytes[4 * w] | ((uint32_t)bytes[4 * w + 1] << 8) |
+                           ((uint32_t)bytes[4 * w + 2] << 16) | ((uint32_t)bytes[4 * w + 3] << 24);
+    if (l == 0) meta[0] = tailpos + nw;
+}
+
+// ------------------------------------------------------------------ decode
+
+enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 4 };
+
+constexpr int kWalkChunk = 8192;
+
+// One workgroup: stream header check and the group-header chain.
+__global__ __launch_bounds__(256) void k_dec_walk(const uint32_t *__restrict__ in, int64_t nwords,
+                                                  int64_t cap, int64_t *__restrict__ grp,
+                                                  int64_t *__restrict__ meta)
+{
+    __shared__ uint32_t lds[kWalkChunk];
+    __shared__ int64_t s_pos, s_g, s_err;
+    const int64_t L = in[0];
+    const int64_t nblk = L / 128, ngroups = (nblk + 15) / 16;
+    if (threadIdx.x == 0) {
+        int64_t err = 0;
+        if (in[1] != kCookie || in[2] != kCookie || in[3] != kCookie) err = 1;
+        if (L % 128 != 0 || L > cap) err = 2;
+        s_err = err;
+        s_pos = 4;
+        s_g = 0;
+        meta[kMetaL] = L;
+        meta[kMetaGroups] = ngroups;
+    }
+    __syncthreads();
+    while (s_err == 0 && s_g < ngroups) {
+        const int64_t pos = s_pos;
+        const int64_t len = min((int64_t)kWalkChunk, nwords - pos);
+        for (int64_t t = threadIdx.x; t < len; t += blockDim.x) lds[t] = in[pos + t];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t local = 0, g = s_g, err = 0;
+            const int64_t g_at_entry = g;
+            while (g < ngroups && local + 4 <= len) {
+                const int64_t cnt = min((int64_t)16, nblk - 16 * g);
+                int64_t size = 4;
+                for (int64_t j = 0; j < cnt; ++j) {
+                    const uint32_t bj = (lds[local + (j >> 2)] >> (24 - 8 * (j & 3))) & 0xFF;
+                    if (bj > 32) err = 3;
+                    size += 4 * bj;
+                }
+                if (err) break;
+                grp[g] = pos + local;
+                local += size;
+                ++g;
+            }
+            if (!err && g == g_at_entry) err = 4;  // truncated: no header fits
+            if (!err && pos + local > nwords) err = 4;
+            s_pos = pos + local;
+            s_g = g;
+            s_err = err;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        meta[kMetaTailPos] = s_pos;
+        meta[kMetaErr] = s_err;
+    }
+}
+
+// One wave per miniblock: data offset and the (reset, lane sums) record.
+__global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ in, int64_t nblk,
+                                                   const int64_t *__restrict__ grp,
+                                                   int64_t *__restrict__ off, uint32_t *__restrict__ carry)
+{
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nblk) return;
+    const int l = threadIdx.x & 63;
+    const int64_t g = i >> 4;
+    const int64_t hp = grp[g];
+    int64_t o = hp + 4;
+    for (int64_t j = 16 * g; j < i; ++j) {
+        const int64_t jj = j - 16 * g;
+        o += 4 * ((in[hp + (jj >> 2)] >> (24 - 8 * (jj & 3))) & 0xFF);
+    }
+    const int64_t ii = i - 16 * g;
+    const uint32_t b = (in[hp + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
+    if (l == 0) off[i] = o;
+    Carry c;
+    if (b == 32) {
+        c.r = 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c.v[j] = in[o + 124 + j];
+    } else {
+        c.r = 0;
+        uint32_t s = 0;
+        if (b > 0) {
+            // lane l holds values k = l and k = l + 64, both of class l & 3
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t k = l + 64 * h, j = k & 3, idx = k >> 2;
+                const uint32_t bit = idx * b, w = bit >> 5, sh = bit & 31;
+                uint64_t v = in[o + 4 * w + j] >> sh;
+                if (sh + b > 32) v |= (uint64_t)in[o + 4 * (w + 1) + j] << (32 - sh);
+                s += (uint32_t)(v & ((1ull << b) - 1));
+            }
+        }
+        // sum over lanes of the same class (lane distance multiples of 4)
+        s += __shfl_xor(s, 4, 64);
+        s += __shfl_xor(s, 8, 64);
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c.v[j] = __shfl(s, j, 64);
+    }
+    if (l == 0) store_carry(carry, i, c);
+}
+
+// --- exclusive scan of Carry records (tile of 256, recursive over tiles)
+__device__ __forceinline__ Carry block_scan_carry(Carry v, Carry *lds, Carry *total)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        Carry prev;
+        const bool has = t >= off;
+        if (has) prev = lds[t - off];
+        __syncthreads();
+        if (has) v = comb(prev, v);
+        lds[t] = v;
+        __syncthreads();
+    }
+    *total = lds[255];
+    Carry ex;
+    if (t > 0) ex = lds[t - 1];
+    else ex = Carry{0, {0, 0, 0, 0}};
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(256) void k_carry_tiles(const uint32_t *__restrict__ c, int64_t n,
+                                                     uint32_t *__restrict__ agg)
+{
+    __shared__ Carry lds[256];
+    const int64_t i = blockIdx.x * 256 + threadIdx.x;
+    Carry v = (i < n) ? load_carry(c, i) : Carry{0, {0, 0, 0, 0}};
+    Carry tot;
+    block_scan_carry(v, lds, &tot);
+    if (threadIdx.x == 0) store_carry(agg, blockIdx.x, tot);
+}
+
+__global__ __launch_bounds__(256) void k_carry_apply(uint32_t *__restrict__ c, int64_t n,
+                                                     const uint32_t *__restrict__ tile_prefix)
+{
+    __shared__ Carry lds[256];
+    const int64_t i = blockIdx.x * 256 + threadIdx.x;
+    Carry v = (i < n) ? load_carry(c, i) : Carry{0, {0, 0, 0, 0}};
+    Carry tot;
+    Carry ex = block_scan_carry(v, lds, &tot);
+    if (tile_prefix) ex = comb(load_carry(tile_prefix, blockIdx.x), ex);
+    if (i < n) store_carry(c, i, ex);
+}
+
+int64_t carry_scratch_records(int64_t n)
+{
+    int64_t w = 0;
+    while (n > 256) {
+        n = (n + 255) / 256;
+        w += n;
+    }
+    return w + 1;
+}
+
+hipError_t carry_scan(uint32_t *c, int64_t n, uint32_t *scratch, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t tiles = (n + 255) / 256;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_carry_apply, 1, 256, 0, s, c, n, (const uint32_t *)nullptr);
+        return hipGetLastError();
+    }
+    uint32_t *agg = scratch;
+    hipLaunchKernelGGL(k_carry_tiles, (unsigned)tiles, 256, 0, s, (const uint32_t *)c, n, agg);
+    hipError_t e = carry_scan(agg, tiles, scratch + 5 * tiles, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_carry_apply, (unsigned)tiles, 256, 0, s, c, n, (const uint32_t *)agg);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ in, int64_t nblk,
+                                                   const int64_t *__restrict__ grp,
+                                                   const int64_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ carry,
+                                                   uint32_t *__restrict__ out)
+{
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nblk) return;
+    const int l = threadIdx.x & 63;
+    const int64_t g = i >> 4, ii = i - 16 * g;
+    const uint32_t b = (in[grp[g] + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
+    const int64_t o = off[i];
+    if (b == 32) {
+        out[128 * i + l] = in[o + l];
+        out[128 * i + l + 64] = in[o + l + 64];
+        return;
+    }
+    uint32_t d[2] = {0u, 0u};
+    if (b > 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t k = l + 64 * h, j = k & 3, idx = k >> 2;
+            const uint32_t bit = idx * b, w = bit >> 5, sh = bit & 31;
+            uint64_t v = in[o + 4 * w + j] >> sh;
+            if (sh + b > 32) v |= (uint64_t)in[o + 4 * (w + 1) + j] << (32 - sh);
+            d[h] = (uint32_t)(v & ((1ull << b) - 1));
+        }
+    }
+    // inclusive prefix over lanes of the same class (stride 4)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int dist = 4; dist < 64; dist <<= 1) {
+            const uint32_t t = __shfl_up(d[h], dist, 64);
+            if (l >= dist) d[h] += t;
+        }
+    }
+    const uint32_t j = l & 3;
+    const uint32_t first_half = __shfl(d[0], 60 + j, 64);
+    const Carry c = load_carry(carry, i);
+    out[128 * i + l] = c.v[j] + d[0];
+    out[128 * i + l + 64] = c.v[j] + first_half + d[1];
+}
+
+// varint tail, one thread; meta[kMetaTail] = values, or kMetaErr on overflow
+__global__ void k_dec_tail(const uint32_t *__restrict__ in, int64_t nwords, int64_t L, int64_t cap,
+                           uint32_t *__restrict__ out, int64_t *__restrict__ meta)
+{
+    const int64_t tp = meta[kMetaTailPos];
+    const uint8_t *bi = (const uint8_t *)(in + tp);
+    const int64_t nb = (nwords - tp) * 4;
+    uint32_t prev = 0, v = 0;
+    uint32_t sh = 0;
+    int64_t o = L;
+    for (int64_t q = 0; q < nb; ++q) {
+        const uint32_t c = bi[q];
+        v += (c & 127u) << sh;
+        sh += 7;
+        if (c & 128u) {
+            prev += v;
+            if (o >= cap) {
+                meta[kMetaErr] = 5;
+                return;
+            }
+            out[o++] = prev;
+            v = 0;
+            sh = 0;
+        }
+    }
+    meta[kMetaTail] = o - L;
+}
+
+}  // namespace
+
+namespace g5 {
+
+int S4Codec::reserve(int64_t n)
+{
+    if (n <= cap && d_meta) return G5_OK;
+    release();
+    const int64_t nblk = n / 128 + 2;
+    const int64_t ngrp = nblk / 16 + 2;
+    hipError_t e = hipMalloc(&d_words, (size_t)(nblk + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_scan, (scan_scratch_words(nblk + 1) + 8) * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_off, (size_t)nblk * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_carry, (size_t)nblk * 20);
+    if (e == hipSuccess) e = hipMalloc(&d_carry_scan, (size_t)(carry_scratch_records(nblk) + 2) * 20);
+    if (e == hipSuccess) e = hipMalloc(&d_grp, (size_t)ngrp * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_meta, 16 * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&h_meta, 16 * 8, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        release();
+        return fail_hip(e, "S4Codec::reserve", __FILE__, __LINE__);
+    }
+    cap = n;
+    return G5_OK;
+}
+
+void S4Codec::release()
+{
+    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (h_meta) (void)hipHostFree(h_meta);
+    d_words = d_scan = d_off = d_grp = d_meta = nullptr;
+    d_carry = d_carry_scan = nullptr;
+    h_meta = nullptr;
+    cap = 0;
+}
+
+int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap_words, size_t *nwords,
+                    hipStream_t s)
+{
+    G5_REQUIRE(n >= 0 && cap_words >= g5_s4_max_words((size_t)n), G5_E_OVERFLOW,
+               "s4 encode: output capacity %zu below the bound for %lld values", cap_words, (long long)n);
+    int rc = reserve(n);
+    if (rc) return rc;
+    const int64_t L = n / 128 * 128, nblk = L / 128;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_enc_widths, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk, d_words);
+        G5_HIP(hipGetLastError());
+    }
+    G5_HIP(exclusive_scan_i64(d_words, nblk, d_words + nblk, d_scan, s));
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_enc_pack, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk,
+                           (const int64_t *)d_words, d_out);
+        G5_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_enc_tail, 1, 64, 0, s, d_in, n, L, nblk, (const int64_t *)d_words, d_out, d_meta);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    *nwords = (size_t)h_meta[0];
+    return G5_OK;
+}
+
+int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t cap_values, size_t *n,
+                    hipStream_t s)
+{
+    G5_REQUIRE(nwords >= 4, G5_E_STREAM, "s4 decode: stream of %zu words has no header", nwords);
+    int rc = reserve((int64_t)cap_values);
+    if (rc) return rc;
+    G5_HIP(fill_i64(d_meta, 16, 0, s));
+    hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, (int64_t)nwords, (int64_t)cap_values, d_grp, d_meta);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_STREAM, "s4 decode: malformed stream (code %lld)",
+               (long long)h_meta[kMetaErr]);
+    const int64_t L = h_meta[kMetaL], nblk = L / 128;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_dec_local, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk,
+                           (const int64_t *)d_grp, d_off, d_carry);
+        G5_HIP(hipGetLastError());
+        G5_HIP(carry_scan(d_carry, nblk, d_carry_scan, s));
+        hipLaunchKernelGGL(k_dec_write, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk,
+                           (const int64_t *)d_grp, (const int64_t *)d_off, (const uint32_t *)d_carry, d_out);
+        G5_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_dec_tail, 1, 1, 0, s, d_in, (int64_t)nwords, L, (int64_t)cap_values, d_out, d_meta);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_OVERFLOW, "s4 decode: output capacity %zu exceeded", cap_values);
+    *n = (size_t)(L + h_meta[kMetaTail]);
+    return G5_OK;
+}
+
+}  // namespace g5
+
+namespace {
+g5::S4Codec &abi_codec()
+{
+    static thread_local g5::S4Codec c;
+    return c;
+}
+}  // namespace
+
+extern "C" size_t g5_s4_max_words(size_t n)
+{
+    const size_t L = n / 128 * 128;
+    return 4 + 4 * ((L + 2047) / 2048) + L + (5 * (n - L) + 3) / 4;
+}
+
+extern "C" int g5_s4_encode(const uint32_t *d_in, size_t n, uint32_t *d_out, size_t out_cap, size_t *nwords)
+{
+    G5_REQUIRE((d_in || n == 0) && d_out && nwords, G5_E_ARG, "g5_s4_encode: null argument");
+    G5_REQUIRE(((uintptr_t)d_out & 15) == 0, G5_E_ARG, "g5_s4_encode: output must be 16-byte aligned");
+    return abi_codec().encode(d_in, (int64_t)n, d_out, out_cap, nwords, nullptr);
+}
+
+extern "C" int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t out_cap, size_t *n)
+{
+    G5_REQUIRE(d_in && (d_out || out_cap == 0) && n, G5_E_ARG, "g5_s4_decode: null argument");
+    return abi_codec().decode(d_in, nwords, d_out, out_cap, n, nullptr);
+}

@@ -1,0 +1,30 @@
+// g5_codec.hpp -- device s4-bp128-d4 codec state (see g5_codec.hip).
+#pragma once
+
+#include "g5_internal.hpp"
+
+namespace g5 {
+
+// Scratch for one stream at a time on one stream; sized for up to cap values.
+struct S4Codec {
+    int64_t cap = 0;               // values
+    int64_t *d_words = nullptr;    // per-miniblock word counts -> prefix, nblk + 1
+    int64_t *d_scan = nullptr;     // int64 scan scratch
+    int64_t *d_off = nullptr;      // decode: per-miniblock data offset
+    uint32_t *d_carry = nullptr;   // decode: per-miniblock carry records (5 words)
+    uint32_t *d_carry_scan = nullptr;
+    int64_t *d_grp = nullptr;      // decode: group header positions
+    int64_t *d_meta = nullptr;     // small device record
+    int64_t *h_meta = nullptr;     // pinned mirror
+
+    int reserve(int64_t n_values);
+    void release();
+    // d_in[n] -> d_out; requires cap_words >= g5_s4_max_words(n).
+    int encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap_words, size_t *nwords,
+               hipStream_t s);
+    // d_in[nwords] -> d_out (capacity cap_values); *n = decoded count.
+    int decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t cap_values, size_t *n,
+               hipStream_t s);
+};
+
+}  // namespace g5

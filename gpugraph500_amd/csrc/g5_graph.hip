@@ -1,0 +1,499 @@
+// g5_graph.hip -- the rank's DistMatrix2d block in HBM: upload of a host CSR,
+// and device-side Graph500 Kronecker generation + CSR construction.
+//
+// Generation follows generate_kronecker_range (cpu_2d/generator/
+// graph_generator.c:157-187): every edge index ei owns an MRG stream skipped
+// ahead by ei * 2^64 steps (splittable_mrg.c:105-125), so edges are generated
+// independently, one thread per edge.  The skip matrices A^(v * 2^(8b)) are
+// computed on the host by squaring the one-step matrix modulo 2^31-1.
+//
+// CSR construction follows setupMatrix2 (cpu_2d/distmatrix2d.hh:745-1166) at a
+// 1x1 grid: both directions of every edge, sorted by (row, col), duplicates
+// merged, self-loops dropped, rows padded to a multiple of 64.  The sort and
+// the unique/select passes are rocPRIM device primitives (construction is not
+// on the BFS hot path).
+#include <cstring>
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "g5_internal.hpp"
+
+namespace {
+
+constexpr uint32_t kP = 0x7FFFFFFFu;
+constexpr uint32_t kX = 107374182u;
+constexpr uint32_t kY = 104480u;
+
+struct Mat5 {
+    uint32_t m[25];
+};
+
+uint32_t h_mulmod(uint32_t a, uint32_t b) { return (uint32_t)((uint64_t)a * b % kP); }
+
+Mat5 h_matmul(const Mat5 &a, const Mat5 &b)
+{
+    Mat5 r;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            uint64_t s = 0;
+            for (int k = 0; k < 5; ++k) s += h_mulmod(a.m[i * 5 + k], b.m[k * 5 + j]);
+            r.m[i * 5 + j] = (uint32_t)(s % kP);
+        }
+    return r;
+}
+
+void h_apply(const Mat5 &a, uint32_t z[5])
+{
+    uint32_t o[5];
+    for (int i = 0; i < 5; ++i) {
+        uint64_t s = 0;
+        for (int k = 0; k < 5; ++k) s += h_mulmod(a.m[i * 5 + k], z[k]);
+        o[i] = (uint32_t)(s % kP);
+    }
+    std::memcpy(z, o, sizeof(o));
+}
+
+uint32_t h_step(uint32_t z[5])
+{
+    const uint32_t n = (uint32_t)(((uint64_t)h_mulmod(z[0], kX) + h_mulmod(z[4], kY)) % kP);
+    z[4] = z[3];
+    z[3] = z[2];
+    z[2] = z[1];
+    z[1] = z[0];
+    z[0] = n;
+    return n;
+}
+
+// Host tables: pow2[b] = A^(2^(8b)), b = 0..23.
+struct MrgTables {
+    Mat5 pow2[24];
+    MrgTables()
+    {
+        Mat5 a{};
+        a.m[0] = kX;
+        a.m[4] = kY;
+        for (int i = 1; i < 5; ++i) a.m[i * 5 + i - 1] = 1;
+        for (int b = 0; b < 24; ++b) {
+            pow2[b] = a;
+            for (int s = 0; s < 8; ++s) a = h_matmul(a, a);
+        }
+    }
+    Mat5 power(int b, unsigned v) const  // A^(v * 2^(8b))
+    {
+        Mat5 r{};
+        for (int i = 0; i < 5; ++i) r.m[i * 6] = 1;
+        Mat5 p = pow2[b];
+        while (v) {
+            if (v & 1) r = h_matmul(r, p);
+            p = h_matmul(p, p);
+            v >>= 1;
+        }
+        return r;
+    }
+};
+
+const MrgTables &tables()
+{
+    static MrgTables t;
+    return t;
+}
+
+struct KronSetup {
+    uint32_t base[5];
+    uint64_t val0, val1;
+};
+
+KronSetup kron_setup(uint64_t s1, uint64_t s2)
+{
+    KronSetup k;
+    // make_mrg_seed, cpu_2d/generator/utils.c:61-68
+    k.base[0] = (uint32_t)((s1 & 0x3FFFFFFF) + 1);
+    k.base[1] = (uint32_t)(((s1 >> 30) & 0x3FFFFFFF) + 1);
+    k.base[2] = (uint32_t)((s2 & 0x3FFFFFFF) + 1);
+    k.base[3] = (uint32_t)(((s2 >> 30) & 0x3FFFFFFF) + 1);
+    k.base[4] = (uint32_t)(((s2 >> 60) << 4) + (s1 >> 60) + 1);
+    // scramble constants: skip by 50 * 2^128 + 7 * 2^64 (graph_generator.c:169-179)
+    uint32_t z[5];
+    std::memcpy(z, k.base, sizeof(z));
+    h_apply(tables().power(8, 7), z);
+    h_apply(tables().power(16, 50), z);
+    k.val0 = h_step(z);
+    k.val0 *= 0xFFFFFFFFull;
+    k.val0 += h_step(z);
+    k.val1 = h_step(z);
+    k.val1 *= 0xFFFFFFFFull;
+    k.val1 += h_step(z);
+    return k;
+}
+
+// Device table: skip[k][v] = A^(v * 2^(64 + 8k)), k = 0..7 (the bytes of ei).
+struct DevSkip {
+    uint32_t *d = nullptr;
+    int init(int device)
+    {
+        if (d) return G5_OK;
+        std::vector<uint32_t> h(8 * 256 * 25);
+        const MrgTables &t = tables();
+        for (int k = 0; k < 8; ++k) {
+            Mat5 p = t.pow2[8 + k], acc{};
+            for (int i = 0; i < 5; ++i) acc.m[i * 6] = 1;
+            for (int v = 0; v < 256; ++v) {
+                std::memcpy(&h[(k * 256 + v) * 25], acc.m, sizeof(acc.m));
+                acc = h_matmul(acc, p);
+            }
+        }
+        (void)device;
+        G5_HIP(hipMalloc(&d, h.size() * 4));
+        G5_HIP(hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        return G5_OK;
+    }
+};
+
+DevSkip g_skip;
+
+__device__ __forceinline__ uint32_t d_red(uint64_t x)  // x < 2^62 -> x mod (2^31-1)
+{
+    x = (x & kP) + (x >> 31);
+    x = (x & kP) + (x >> 31);
+    return (uint32_t)(x >= kP ? x - kP : x);
+}
+
+__device__ __forceinline__ uint32_t d_step(uint32_t z[5])
+{
+    const uint32_t n = d_red((uint64_t)z[0] * kX + (uint64_t)z[4] * kY);
+    z[4] = z[3];
+    z[3] = z[2];
+    z[2] = z[1];
+    z[1] = z[0];
+    z[0] = n;
+    return n;
+}
+
+__device__ __forceinline__ void d_apply(const uint32_t *__restrict__ m, uint32_t z[5])
+{
+    uint32_t o[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s += d_red((uint64_t)m[i * 5 + k] * z[k]);
+        o[i] = d_red(s);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) z[i] = o[i];
+}
+
+__device__ __forceinline__ uint64_t d_scramble(uint64_t v, int lgN, uint64_t val0, uint64_t val1)
+{
+    v += val0 + val1;
+    v *= (val0 | 0x4519840211493211ull);
+    v = __brevll(v) >> (64 - lgN);
+    v *= (val1 | 0x3050852102C843A5ull);
+    v = __brevll(v) >> (64 - lgN);
+    return v;
+}
+
+// One Kronecker edge (make_one_edge, graph_generator.c:122-151 with the
+// initiator draw of :45-80).
+__device__ __forceinline__ void d_edge(int64_t ei, const uint32_t *__restrict__ skip,
+                                       const KronSetup &ks, int lgN, uint64_t &src, uint64_t &dst)
+{
+    uint32_t z[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) z[i] = ks.base[i];
+    uint64_t e = (uint64_t)ei;
+    for (int k = 0; e; ++k, e >>= 8)
+        if (e & 0xFF) d_apply(skip + (k * 256 + (e & 0xFF)) * 25, z);
+    uint64_t bs = 0, bt = 0;
+    const uint32_t limit = 0xFFFFFFFFu % 10000u;
+    for (int lev = lgN - 1; lev >= 0; --lev) {
+        uint32_t val = d_step(z);
+        while (val < limit) val = d_step(z);
+        val %= 10000u;
+        int sq;
+        if (val < 1900u) sq = 1;
+        else if (val < 3800u) sq = 2;
+        else if (val < 9500u) sq = 0;
+        else sq = 3;
+        uint64_t so = (uint64_t)(sq >> 1), to = (uint64_t)(sq & 1);
+        if (bs == bt && so > to) {  // clip-and-flip
+            so = 0;
+            to = 1;
+        }
+        bs += so << lev;
+        bt += to << lev;
+    }
+    src = d_scramble(bs, lgN, ks.val0, ks.val1);
+    dst = d_scramble(bt, lgN, ks.val0, ks.val1);
+}
+
+__global__ void k_gen_edges(int64_t start, int64_t end, const uint32_t *skip, KronSetup ks, int lgN,
+                            int64_t *out)
+{
+    for (int64_t i = start + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < end;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t a, b;
+        d_edge(i, skip, ks, lgN, a, b);
+        out[2 * (i - start)] = (int64_t)a;
+        out[2 * (i - start) + 1] = (int64_t)b;
+    }
+}
+
+// Both directed copies of every edge as (row << sh | col) keys; self-loops
+// are kept here (they count in the generated degree) and dropped later.
+__global__ void k_gen_keys(int64_t m, const uint32_t *skip, KronSetup ks, int lgN, int sh,
+                           uint64_t *keys, unsigned long long *d_maxid)
+{
+    uint64_t mx = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t a, b;
+        d_edge(i, skip, ks, lgN, a, b);
+        keys[2 * i] = (a << sh) | b;
+        keys[2 * i + 1] = (b << sh) | a;
+        mx = max(mx, max(a, b));
+    }
+    mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, 32, 64));
+#pragma unroll
+    for (int d = 16; d >= 1; d >>= 1) mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(d_maxid, (unsigned long long)mx);
+}
+
+// pos[r] = first index i with row(keys[i]) >= r, for r in [0, n_rows].
+__global__ void k_row_bounds(const uint64_t *keys, int64_t n, int sh, int64_t n_rows, int64_t *pos)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = (i < n) ? (int64_t)(keys[i] >> sh) : n_rows;
+        const int64_t rp = (i == 0) ? -1 : (int64_t)(keys[i - 1] >> sh);
+        for (int64_t rr = rp + 1; rr <= r; ++rr) pos[rr] = i;
+    }
+}
+
+__global__ void k_gdeg(const int64_t *pos, int64_t n_rows, uint32_t *gdeg)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x)
+        gdeg[r] = (uint32_t)(pos[r + 1] - pos[r]);
+}
+
+__global__ void k_key_cols(const uint64_t *keys, int64_t n, uint64_t mask, uint32_t *cols)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        cols[i] = (uint32_t)(keys[i] & mask);
+}
+
+struct NotSelfLoop {
+    int sh;
+    uint64_t mask;
+    __host__ __device__ bool operator()(uint64_t k) const { return (k >> sh) != (k & mask); }
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int build_kronecker(int scale, int edgefactor, uint64_t s1, uint64_t s2, g5_graph *g)
+{
+    hipStream_t s = nullptr;  // construction runs on the null stream
+    int dev = 0;
+    G5_HIP(hipGetDevice(&dev));
+    int rc = g_skip.init(dev);
+    if (rc) return rc;
+    const KronSetup ks = kron_setup(s1, s2);
+    const int64_t m = (int64_t)edgefactor << scale;
+    const int64_t n2 = 2 * m;
+    const int sh = scale < 1 ? 1 : scale;
+    DevBuf ka, kb, tmp, maxid, cnt;
+    G5_HIP(hipMalloc(&ka.p, (size_t)(n2 ? n2 : 1) * 8));
+    G5_HIP(hipMalloc(&kb.p, (size_t)(n2 ? n2 : 1) * 8));
+    G5_HIP(hipMalloc(&maxid.p, 16));
+    G5_HIP(hipMemset(maxid.p, 0, 16));
+    uint64_t *A = (uint64_t *)ka.p, *B = (uint64_t *)kb.p;
+    if (m > 0)
+        hipLaunchKernelGGL(k_gen_keys, g5::grid_for(m, 256, 65536), 256, 0, s, m,
+                           (const uint32_t *)g_skip.d, ks, scale, sh, A,
+                           (unsigned long long *)maxid.p);
+    G5_HIP(hipGetLastError());
+    unsigned long long h_max = 0;
+    G5_HIP(hipMemcpy(&h_max, maxid.p, 8, hipMemcpyDeviceToHost));
+    g->nglobal = m > 0 ? (int64_t)h_max + 1 : 0;
+    g->n_rows = g->n_cols = (g->nglobal + 63) / 64 * 64;
+    g->n_gen_edges = m;
+    g->scale = scale;
+
+    // sort all directed keys by (row, col)
+    rocprim::double_buffer<uint64_t> db(A, B);
+    size_t tb = 0;
+    G5_HIP(rocprim::radix_sort_keys(nullptr, tb, db, (size_t)n2, 0u, (unsigned)(2 * sh), s));
+    G5_HIP(hipMalloc(&tmp.p, tb ? tb : 1));
+    G5_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, (size_t)n2, 0u, (unsigned)(2 * sh), s));
+    uint64_t *sorted = db.current(), *other = db.alternate();
+
+    // generated directed degree from the row ranges of the full sorted list
+    G5_HIP(hipMalloc(&g->d_rowptr, (size_t)(g->n_rows + 1) * 8));
+    G5_HIP(hipMalloc(&g->d_gdeg, (size_t)(g->n_rows ? g->n_rows : 1) * 4));
+    hipLaunchKernelGGL(k_row_bounds, g5::grid_for(n2 + 1, 256, 65536), 256, 0, s,
+                       (const uint64_t *)sorted, n2, sh, g->n_rows, g->d_rowptr);
+    hipLaunchKernelGGL(k_gdeg, g5::grid_for(g->n_rows, 256, 65536), 256, 0, s,
+                       (const int64_t *)g->d_rowptr, g->n_rows, g->d_gdeg);
+    G5_HIP(hipGetLastError());
+
+    // unique, then drop self-loops (distmatrix2d.hh:1033-1165)
+    G5_HIP(hipMalloc(&cnt.p, 16));
+    size_t tu = 0, tsel = 0;
+    const uint64_t mask = (sh >= 64) ? ~0ull : ((1ull << sh) - 1);
+    G5_HIP(rocprim::unique(nullptr, tu, sorted, other, (size_t *)cnt.p, (size_t)n2,
+                           rocprim::equal_to<uint64_t>(), s));
+    G5_HIP(rocprim::select(nullptr, tsel, other, sorted, (size_t *)cnt.p + 1, (size_t)n2,
+                           NotSelfLoop{sh, mask}, s));
+    if (tu > tb || tsel > tb) {
+        (void)hipFree(tmp.p);
+        tmp.p = nullptr;
+        G5_HIP(hipMalloc(&tmp.p, std::max(tu, tsel)));
+    }
+    G5_HIP(rocprim::unique(tmp.p, tu, sorted, other, (size_t *)cnt.p, (size_t)n2,
+                           rocprim::equal_to<uint64_t>(), s));
+    size_t h_cnt[2] = {0, 0};
+    G5_HIP(hipMemcpy(h_cnt, cnt.p, 8, hipMemcpyDeviceToHost));
+    G5_HIP(rocprim::select(tmp.p, tsel, other, sorted, (size_t *)cnt.p + 1, h_cnt[0],
+                           NotSelfLoop{sh, mask}, s));
+    G5_HIP(hipMemcpy(h_cnt, cnt.p, 16, hipMemcpyDeviceToHost));
+    g->nnz = (int64_t)h_cnt[1];
+    G5_HIP(hipMalloc(&g->d_cols, (size_t)(g->nnz ? g->nnz : 1) * 4));
+    hipLaunchKernelGGL(k_key_cols, g5::grid_for(g->nnz, 256, 65536), 256, 0, s,
+                       (const uint64_t *)sorted, g->nnz, mask, g->d_cols);
+    hipLaunchKernelGGL(k_row_bounds, g5::grid_for(g->nnz + 1, 256, 65536), 256, 0, s,
+                       (const uint64_t *)sorted, g->nnz, sh, g->n_rows, g->d_rowptr);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipDeviceSynchronize());
+    return G5_OK;
+}
+
+}  // namespace
+
+extern "C" int g5_graph_upload(const int64_t *rowptr, const uint32_t *cols, int64_t n_rows,
+                               int64_t n_cols, const int64_t *gdeg, int64_t row_start,
+                               int64_t col_start, g5_graph **out)
+{
+    G5_REQUIRE(rowptr && out && n_rows >= 0 && n_cols >= 0, G5_E_ARG, "g5_graph_upload: bad arguments");
+    G5_REQUIRE(n_rows % 64 == 0 && n_cols % 64 == 0, G5_E_ARG,
+               "g5_graph_upload: n_rows and n_cols must be multiples of 64");
+    const int64_t nnz = rowptr[n_rows];
+    G5_REQUIRE(nnz >= 0 && (nnz == 0 || cols), G5_E_ARG, "g5_graph_upload: bad nnz/cols");
+    for (int64_t i = 0; i < n_rows; ++i)
+        G5_REQUIRE(rowptr[i] <= rowptr[i + 1], G5_E_ARG, "g5_graph_upload: rowptr not monotone at %lld", (long long)i);
+    for (int64_t j = 0; j < nnz; ++j)
+        G5_REQUIRE(cols[j] < (uint64_t)n_cols, G5_E_ARG, "g5_graph_upload: column id out of range at %lld", (long long)j);
+    g5_graph *g = new g5_graph();
+    G5_HIP(hipGetDevice(&g->device));
+    g->n_rows = n_rows;
+    g->n_cols = n_cols;
+    g->nnz = nnz;
+    g->nglobal = std::max(n_rows + row_start, n_cols + col_start);
+    g->row_start = row_start;
+    g->col_start = col_start;
+    int rc = G5_OK;
+    auto fail = [&](int r) {
+        g5_graph_free(g);
+        return r;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&g->d_rowptr, (size_t)(n_rows + 1) * 8)) != hipSuccess ||
+        (e = hipMalloc(&g->d_cols, (size_t)(nnz ? nnz : 1) * 4)) != hipSuccess ||
+        (e = hipMalloc(&g->d_gdeg, (size_t)(n_rows ? n_rows : 1) * 4)) != hipSuccess)
+        return fail(g5::fail_hip(e, "hipMalloc", __FILE__, __LINE__));
+    if ((e = hipMemcpy(g->d_rowptr, rowptr, (size_t)(n_rows + 1) * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (nnz && (e = hipMemcpy(g->d_cols, cols, (size_t)nnz * 4, hipMemcpyHostToDevice)) != hipSuccess))
+        return fail(g5::fail_hip(e, "hipMemcpy", __FILE__, __LINE__));
+    std::vector<uint32_t> gd(n_rows);
+    int64_t gen2 = 0;
+    for (int64_t i = 0; i < n_rows; ++i) {
+        const int64_t v = gdeg ? gdeg[i] : rowptr[i + 1] - rowptr[i];
+        gd[i] = (uint32_t)v;
+        gen2 += v;
+    }
+    g->n_gen_edges = gen2 / 2;
+    if (n_rows && (e = hipMemcpy(g->d_gdeg, gd.data(), (size_t)n_rows * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(g5::fail_hip(e, "hipMemcpy", __FILE__, __LINE__));
+    *out = g;
+    return rc;
+}
+
+extern "C" int g5_graph_kronecker(int scale, int edgefactor, uint64_t seed1, uint64_t seed2,
+                                  g5_graph **out)
+{
+    G5_REQUIRE(out && scale >= 1 && scale <= 30 && edgefactor >= 1 && edgefactor <= 1024, G5_E_ARG,
+               "g5_graph_kronecker: scale must be in [1,30] and edgefactor in [1,1024]");
+    g5_graph *g = new g5_graph();
+    G5_HIP(hipGetDevice(&g->device));
+    int rc = build_kronecker(scale, edgefactor, seed1, seed2, g);
+    if (rc) {
+        g5_graph_free(g);
+        return rc;
+    }
+    *out = g;
+    return G5_OK;
+}
+
+extern "C" int g5_graph_info(const g5_graph *g, int64_t *n_rows, int64_t *n_cols, int64_t *nnz,
+                             int64_t *nglobal, int64_t *n_gen_edges)
+{
+    G5_REQUIRE(g, G5_E_ARG, "g5_graph_info: null graph");
+    if (n_rows) *n_rows = g->n_rows;
+    if (n_cols) *n_cols = g->n_cols;
+    if (nnz) *nnz = g->nnz;
+    if (nglobal) *nglobal = g->nglobal;
+    if (n_gen_edges) *n_gen_edges = g->n_gen_edges;
+    return G5_OK;
+}
+
+extern "C" int g5_graph_download(const g5_graph *g, int64_t *rowptr, uint32_t *cols, int64_t *gdeg)
+{
+    G5_REQUIRE(g, G5_E_ARG, "g5_graph_download: null graph");
+    if (rowptr) G5_HIP(hipMemcpy(rowptr, g->d_rowptr, (size_t)(g->n_rows + 1) * 8, hipMemcpyDeviceToHost));
+    if (cols && g->nnz) G5_HIP(hipMemcpy(cols, g->d_cols, (size_t)g->nnz * 4, hipMemcpyDeviceToHost));
+    if (gdeg && g->n_rows) {
+        std::vector<uint32_t> gd(g->n_rows);
+        G5_HIP(hipMemcpy(gd.data(), g->d_gdeg, (size_t)g->n_rows * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < g->n_rows; ++i) gdeg[i] = gd[i];
+    }
+    return G5_OK;
+}
+
+extern "C" int g5_graph_free(g5_graph *g)
+{
+    if (!g) return G5_OK;
+    if (g->d_rowptr) (void)hipFree(g->d_rowptr);
+    if (g->d_cols) (void)hipFree(g->d_cols);
+    if (g->d_gdeg) (void)hipFree(g->d_gdeg);
+    delete g;
+    return G5_OK;
+}
+
+extern "C" int g5_kronecker_edges(int scale, uint64_t seed1, uint64_t seed2, int64_t start,
+                                  int64_t end, int64_t *edges)
+{
+    G5_REQUIRE(scale >= 1 && scale <= 32 && start >= 0 && end >= start && (edges || end == start),
+               G5_E_ARG, "g5_kronecker_edges: bad arguments");
+    if (end == start) return G5_OK;
+    int dev = 0;
+    G5_HIP(hipGetDevice(&dev));
+    int rc = g_skip.init(dev);
+    if (rc) return rc;
+    const KronSetup ks = kron_setup(seed1, seed2);
+    DevBuf buf;
+    const int64_t n = end - start;
+    G5_HIP(hipMalloc(&buf.p, (size_t)n * 16));
+    hipLaunchKernelGGL(k_gen_edges, g5::grid_for(n, 256, 65536), 256, 0, nullptr, start, end,
+                       (const uint32_t *)g_skip.d, ks, scale, (int64_t *)buf.p);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpy(edges, buf.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+    return G5_OK;
+}

@@ -1,0 +1,112 @@
+// g5_internal.hpp -- shared host/device definitions of the gfx950 library.
+//
+// Layout in HBM (one rank's DistMatrix2d block, cpu_2d/distmatrix2d.hh:30-130):
+//   rowptr  int64[n_rows + 1]   CSR row offsets
+//   cols    uint32[nnz]         local column ids, ascending and unique per row
+//   gdeg    uint32[n_rows]      generated directed degree (duplicates and
+//                               self-loops counted) -- the TEPS numerator
+// Bitmaps are uint64 words, bit (v & 63) of word (v >> 6); n_rows and n_cols
+// are padded to multiples of 64 as the reference's ALG = 64 does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/g5hip.h"
+
+namespace g5 {
+
+void set_error(const char *fmt, ...);
+int fail_hip(hipError_t e, const char *what, const char *file, int line);
+
+}  // namespace g5
+
+#define G5_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) return g5::fail_hip(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define G5_REQUIRE(cond, code, ...)         \
+    do {                                            \
+        if (!(cond)) {                              \
+            g5::set_error(__VA_ARGS__);             \
+            return (code);                          \
+        }                                           \
+    } while (0)
+
+struct g5_graph {
+    int device = 0;
+    int scale = 0;          // 0 when uploaded from a host CSR
+    int64_t n_rows = 0;     // local rows, multiple of 64
+    int64_t n_cols = 0;     // local columns, multiple of 64
+    int64_t nnz = 0;
+    int64_t nglobal = 0;    // global vertex count (max id + 1)
+    int64_t n_gen_edges = 0;
+    int64_t row_start = 0;  // global id of local row 0
+    int64_t col_start = 0;  // global id of local column 0
+    int64_t *d_rowptr = nullptr;
+    uint32_t *d_cols = nullptr;
+    uint32_t *d_gdeg = nullptr;
+};
+
+namespace g5 {
+
+constexpr int kWave = 64;
+
+// ---- device-wide primitives (g5_util.hip) ----
+// Exclusive scan of n int64 values in place; returns the total through *d_total
+// (device pointer, may be null).  scratch must hold scan_scratch_words(n) int64.
+size_t scan_scratch_words(int64_t n);
+hipError_t exclusive_scan_i64(int64_t *d_data, int64_t n, int64_t *d_total, int64_t *scratch,
+                              hipStream_t s);
+// Bitmap -> sorted queue of set-bit indices (+ base).  Writes the count to
+// *d_count.  scratch as for exclusive_scan_i64 over n_words.
+hipError_t bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint32_t *d_queue,
+                           int64_t *d_count, int64_t *scratch, hipStream_t s);
+// Queue -> bitmap (bitmap must be cleared first).
+hipError_t queue_to_bitmap(const uint32_t *d_queue, int64_t n, uint64_t *d_bits, hipStream_t s);
+hipError_t fill_u64(uint64_t *d, int64_t n, uint64_t v, hipStream_t s);
+hipError_t fill_u32(uint32_t *d, int64_t n, uint32_t v, hipStream_t s);
+hipError_t fill_i32(int32_t *d, int64_t n, int32_t v, hipStream_t s);
+hipError_t fill_i64(int64_t *d, int64_t n, int64_t v, hipStream_t s);
+int grid_for(int64_t n, int block, int max_grid = 1 << 20);
+
+}  // namespace g5
+
+// ---- device helpers ----
+__device__ __forceinline__ uint32_t g5_lane() { return __lane_id(); }
+
+__device__ __forceinline__ int64_t g5_wave_incl_scan(int64_t v)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int64_t t = __shfl_up(v, d, 64);
+        if ((int)__lane_id() >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t g5_wave_or(uint64_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t g5_wave_or32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ int64_t g5_wave_sum(int64_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}

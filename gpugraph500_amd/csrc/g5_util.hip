@@ -1,0 +1,269 @@
+// g5_util.hip -- library plumbing and device-wide primitives: error state,
+// device memory helpers, exclusive scan, bitmap <-> sorted queue.
+#include <cstdarg>
+#include <cstring>
+
+#include "g5_internal.hpp"
+
+namespace g5 {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int fail_hip(hipError_t e, const char *what, const char *file, int line)
+{
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? G5_E_NOMEM : G5_E_HIP;
+}
+
+int grid_for(int64_t n, int block, int max_grid)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------- fills
+
+template <typename T>
+__global__ void k_fill(T *d, int64_t n, T v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+
+hipError_t fill_u64(uint64_t *d, int64_t n, uint64_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<uint64_t>, grid_for(n, 256, 8192), 256, 0, s, d, n, v);
+    return hipGetLastError();
+}
+hipError_t fill_u32(uint32_t *d, int64_t n, uint32_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<uint32_t>, grid_for(n, 256, 8192), 256, 0, s, d, n, v);
+    return hipGetLastError();
+}
+hipError_t fill_i64(int64_t *d, int64_t n, int64_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<int64_t>, grid_for(n, 256, 8192), 256, 0, s, d, n, v);
+    return hipGetLastError();
+}
+hipError_t fill_i32(int32_t *d, int64_t n, int32_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill<int32_t>, grid_for(n, 256, 8192), 256, 0, s, d, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- scan
+// Three-phase exclusive scan over 1024-element tiles (256 threads x 4):
+// tile sums -> recursive scan of the sums -> tile scan plus offset.
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *lds_waves, int64_t *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t inc = g5_wave_incl_scan(v);
+    if (lane == 63) lds_waves[wave] = inc;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        const int64_t t = lds_waves[w];
+        off += (w < wave) ? t : 0;
+        tot += t;
+    }
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ void k_tile_sums(const int64_t *d, int64_t n, int64_t *sums)
+{
+    __shared__ int64_t lw[kScanBlock / 64];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += d[base + i];
+    int64_t tot;
+    block_excl_scan(s, lw, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void k_tile_scan(int64_t *d, int64_t n, const int64_t *offs, int64_t *d_total)
+{
+    __shared__ int64_t lw[kScanBlock / 64];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? d[base + i] : 0;
+        s += v[i];
+    }
+    int64_t tot;
+    int64_t run = block_excl_scan(s, lw, &tot) + (offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) d[base + i] = run;
+        run += v[i];
+    }
+    if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) *d_total = run;
+}
+
+size_t scan_scratch_words(int64_t n)
+{
+    size_t w = 0;
+    while (n > kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        w += (size_t)n + 1;
+    }
+    return w + 1;
+}
+
+hipError_t exclusive_scan_i64(int64_t *d, int64_t n, int64_t *d_total, int64_t *scratch,
+                              hipStream_t s)
+{
+    if (n <= 0) return d_total ? fill_i64(d_total, 1, 0, s) : hipSuccess;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_tile_scan, 1, kScanBlock, 0, s, d, n, (const int64_t *)nullptr,
+                           d_total);
+        return hipGetLastError();
+    }
+    int64_t *sums = scratch;
+    hipLaunchKernelGGL(k_tile_sums, (unsigned)tiles, kScanBlock, 0, s, (const int64_t *)d, n, sums);
+    hipError_t e = exclusive_scan_i64(sums, tiles, nullptr, scratch + tiles + 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_scan, (unsigned)tiles, kScanBlock, 0, s, d, n,
+                       (const int64_t *)sums, d_total);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- bitmap <-> queue
+
+// One thread per word.  Word popcounts are scanned per 256-word block; the
+// block offsets come from a scan of the block totals.
+__global__ void k_bm_counts(const uint64_t *bits, int64_t n_words, int64_t *blk)
+{
+    __shared__ int64_t lw[4];
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t c = (w < n_words) ? __popcll(bits[w]) : 0;
+    int64_t tot;
+    block_excl_scan(c, lw, &tot);
+    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void k_bm_write(const uint64_t *bits, int64_t n_words, const int64_t *blk_off,
+                           uint32_t *queue, int64_t *d_count)
+{
+    __shared__ int64_t lw[4];
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint64_t word = (w < n_words) ? bits[w] : 0;
+    int64_t tot;
+    int64_t pos = block_excl_scan(__popcll(word), lw, &tot) + blk_off[blockIdx.x];
+    while (word) {
+        const int b = __ffsll((long long)word) - 1;
+        queue[pos++] = (uint32_t)(w * 64 + b);
+        word &= word - 1;
+    }
+    if (d_count && blockIdx.x == gridDim.x - 1 && threadIdx.x == blockDim.x - 1) *d_count = pos;
+}
+
+hipError_t bitmap_to_queue(const uint64_t *bits, int64_t n_words, uint32_t *queue,
+                           int64_t *d_count, int64_t *scratch, hipStream_t s)
+{
+    if (n_words <= 0) return fill_i64(d_count, 1, 0, s);
+    const int64_t blocks = (n_words + 255) / 256;
+    int64_t *blk = scratch;
+    hipLaunchKernelGGL(k_bm_counts, (unsigned)blocks, 256, 0, s, bits, n_words, blk);
+    hipError_t e = exclusive_scan_i64(blk, blocks, nullptr, scratch + blocks + 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bm_write, (unsigned)blocks, 256, 0, s, bits, n_words,
+                       (const int64_t *)blk, queue, d_count);
+    return hipGetLastError();
+}
+
+// Sorted queue: the first entry of each run of equal words ORs the run's bits
+// with one atomic (runs that cross a wave boundary simply take two atomics).
+__global__ void k_q2bm(const uint32_t *q, int64_t n, uint64_t *bits)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = q[i];
+    const uint32_t w = v >> 6;
+    const bool lead = (i == 0) || ((q[i - 1] >> 6) != w) || ((threadIdx.x & 63) == 0);
+    if (!lead) return;
+    uint64_t acc = 0;
+    int64_t j = i;
+    do {
+        acc |= 1ull << (q[j] & 63);
+        ++j;
+    } while (j < n && (q[j] >> 6) == w && (((threadIdx.x + (j - i)) & 63) != 0));
+    atomicOr((unsigned long long *)&bits[w], (unsigned long long)acc);
+}
+
+hipError_t queue_to_bitmap(const uint32_t *q, int64_t n, uint64_t *bits, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_q2bm, (unsigned)((n + 255) / 256), 256, 0, s, q, n, bits);
+    return hipGetLastError();
+}
+
+}  // namespace g5
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" const char *g5_last_error(void) { return g5::g_err; }
+
+extern "C" int g5_init(int device)
+{
+    G5_HIP(hipSetDevice(device));
+    G5_HIP(hipFree(nullptr));
+    return G5_OK;
+}
+
+extern "C" int g5_device_sync(void)
+{
+    G5_HIP(hipDeviceSynchronize());
+    return G5_OK;
+}
+
+extern "C" int g5_dmalloc(void **p, size_t bytes)
+{
+    G5_REQUIRE(p != nullptr, G5_E_ARG, "g5_dmalloc: null output pointer");
+    G5_HIP(hipMalloc(p, bytes ? bytes : 1));
+    return G5_OK;
+}
+
+extern "C" int g5_dfree(void *p)
+{
+    if (p) G5_HIP(hipFree(p));
+    return G5_OK;
+}
+
+extern "C" int g5_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) G5_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return G5_OK;
+}
+
+extern "C" int g5_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) G5_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return G5_OK;
+}

@@ -1,0 +1,126 @@
+/*
+ * g5hip.h -- C ABI of the MI355X (gfx950) gpugraph500 hot path.
+ *
+ * Drop-in boundary for one rank's share of the reference's 2D BFS:
+ *   - the local expansion plug-in (the GPU "Derived" class of GlobalBFS,
+ *     cpu_2d/cuda/cuda_bfs.h:39-84, cpu_2d/globalbfs.hh:51-126),
+ *   - the vreduce combines (cpu_2d/vreduce.hpp:41-544,
+ *     cpu_2d/globalbfs.hh:235-524),
+ *   - the s4-bp128-d4 frontier-queue codec behind Compression<T,T_C>
+ *     (cpu_2d/compression/compression.hh:11-28, cpusimd.hh:20-184).
+ *
+ * Conventions: every function returns 0 on success and a negative G5_E* code
+ * on failure; g5_last_error() describes the last failure of the calling
+ * thread.  No exception crosses this boundary.  Handles are opaque.  Pointers
+ * named d_* are device (HBM) pointers; all others are host pointers.
+ * Vertex ids in queues are uint32 local ids (the reference narrows its int64
+ * queues to uint32 before coding, cpusimd.hh:93-96).
+ */
+#ifndef G5HIP_H
+#define G5HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    G5_OK = 0,
+    G5_E_ARG = -1,      /* invalid argument (sizes, ids, null handles) */
+    G5_E_HIP = -2,      /* a HIP runtime call failed */
+    G5_E_NOMEM = -3,    /* device allocation failed */
+    G5_E_OVERFLOW = -4, /* output buffer too small (reference: "increase
+                           queue-sizing factor", enactor_multi_gpu_2d.cuh:706-729) */
+    G5_E_STREAM = -5,   /* malformed codec stream (simdbinarypacking.h:240-247) */
+    G5_E_STATE = -6     /* call out of order (e.g. level before start) */
+};
+
+typedef struct g5_graph g5_graph;
+typedef struct g5_bfs g5_bfs;
+
+/* ---- library ---- */
+const char *g5_last_error(void);
+int g5_init(int device);
+int g5_device_sync(void);
+int g5_dmalloc(void **d_ptr, size_t bytes);
+int g5_dfree(void *d_ptr);
+int g5_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
+int g5_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
+
+/* ---- the DistMatrix2d block (cpu_2d/distmatrix2d.hh:30-130) ----
+ * Upload a host CSR block: rowptr int64[n_rows+1], cols uint32[nnz] local
+ * column ids (ascending, unique per row), gdeg int64[n_rows] generated
+ * directed degrees (nullable).  n_rows and n_cols must be multiples of 64.
+ * Replaces CsrProblem::FromHostProblem (csr_problem_2d.cuh:314) as called
+ * from CUDA_BFS's constructor (cuda/cuda_bfs.cu:79-87). */
+int g5_graph_upload(const int64_t *rowptr, const uint32_t *cols, int64_t n_rows, int64_t n_cols,
+                    const int64_t *gdeg, int64_t row_start, int64_t col_start, g5_graph **out);
+/* Generate the Graph500 Kronecker graph (cpu_2d/generator/make_graph.c:34-69,
+ * seeds as cpu_2d/main.cpp:221) and build the 1x1 block on the device with
+ * setupMatrix2's rules (cpu_2d/distmatrix2d.hh:745-1166): sorted, deduplicated
+ * columns, self-loops dropped, rows padded to 64. */
+int g5_graph_kronecker(int scale, int edgefactor, uint64_t seed1, uint64_t seed2, g5_graph **out);
+int g5_graph_info(const g5_graph *g, int64_t *n_rows, int64_t *n_cols, int64_t *nnz,
+                  int64_t *nglobal, int64_t *n_gen_edges);
+int g5_graph_download(const g5_graph *g, int64_t *rowptr, uint32_t *cols, int64_t *gdeg);
+int g5_graph_free(g5_graph *g);
+/* Edges [start, end) of the Kronecker list generated on the device and copied
+ * to edges[2*(end-start)] (generate_kronecker_range, graph_generator.c:157-187). */
+int g5_kronecker_edges(int scale, uint64_t seed1, uint64_t seed2, int64_t start, int64_t end,
+                       int64_t *edges);
+
+/* ---- local expansion plug-in: the CUDA_BFS hooks (cuda/cuda_bfs.h:68-83) ---- */
+int g5_bfs_create(g5_graph *g, g5_bfs **out);
+int g5_bfs_free(g5_bfs *b);
+/* setStartVertex (cuda_bfs.cu:547-640); root = -1 on ranks that do not own it */
+int g5_bfs_start(g5_bfs *b, int64_t root);
+/* runLocalBFS (cuda_bfs.cu:642-659) + istheresomethingnew (globalbfs.hh:716):
+ * expands the in-queue; *any_new = 1 if the out-queue is non-empty.  New
+ * vertices get their minimum-id frontier neighbour as parent. */
+int g5_bfs_expand(g5_bfs *b, int *any_new);
+/* getBackOutqueue (cuda_bfs.cu:380-475): the column out-queue, sorted and
+ * unique, into q (capacity *n entries); *n = queue length. */
+int g5_bfs_outqueue(g5_bfs *b, uint32_t *q, int64_t *n);
+/* setModOutgoingFQ (cuda_bfs.cu:230-270): adopt the column-reduced queue
+ * (q = NULL: keep the own out-queue) and mark it visited at depth + 1. */
+int g5_bfs_set_outqueue(g5_bfs *b, const uint32_t *q, int64_t n);
+/* setIncommingFQ + setBackInqueue (cuda_bfs.cu:275-321,477-545): the row
+ * queue for the next level (q = NULL: the adopted out-queue). */
+int g5_bfs_set_inqueue(g5_bfs *b, const uint32_t *q, int64_t n);
+/* getBackPredecessor (cuda_bfs.cu:328-378): pred int64[n_cols] global ids
+ * (-1 unreached, root -> root), level int32[n_cols] (-1 unreached); either
+ * pointer may be NULL. */
+int g5_bfs_result(g5_bfs *b, int64_t *pred, int32_t *level);
+/* The whole 1x1 runBFS (globalbfs.hh:548-1263) resident in HBM: no host
+ * round trip per level.  flags: G5_RUN_CODEC routes every level's queue
+ * through s4-bp128-d4 encode -> decode on the device, as the row broadcast
+ * delivers it to non-root ranks (globalbfs.hh:864-905). *ms = device time. */
+enum { G5_RUN_CODEC = 1 };
+int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
+/* TEPS numerator of the last search: generated undirected edges with a
+ * reached endpoint (validate/validate.h:439-485). */
+int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);
+
+/* ---- s4-bp128-d4 codec (codecfactory.h:103-105) on the device ----
+ * Word stream identical to the reference codec's output for 16-byte aligned
+ * buffers.  Worst-case size: g5_s4_max_words(n). */
+size_t g5_s4_max_words(size_t n);
+/* d_in uint32[n] -> d_out words; *nwords = stream length (host). */
+int g5_s4_encode(const uint32_t *d_in, size_t n, uint32_t *d_out, size_t out_cap,
+                 size_t *nwords);
+/* d_in words[nwords] -> d_out uint32 (capacity out_cap); *n = values decoded. */
+int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t out_cap, size_t *n);
+
+/* ---- vreduce combines (vreduce.hpp:41-544, globalbfs.hh:235-524) ----
+ * d_maps: k contiguous bitmaps of n_words uint64 each -> d_out = OR. */
+int g5_vreduce_or(const uint64_t *d_maps, int k, int64_t n_words, uint64_t *d_out);
+/* d_preds: k contiguous int64[n] predecessor arrays (-1 absent) -> d_out =
+ * minimum present id (the canonical rule replacing rank precedence). */
+int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
