@@ -57,6 +57,9 @@ def lib():
     _sig(L, "g5_graph_download", I, [P, P, P, P])
     _sig(L, "g5_graph_free", I, [P])
     _sig(L, "g5_kronecker_edges", I, [I, c_u64, c_u64, c_i64, c_i64, P])
+    _sig(L, "g5_select_roots", I, [P, c_i64, I, P, ctypes.POINTER(I)])
+    _sig(L, "g5_bfs_profile", I, [P, I])
+    _sig(L, "g5_bfs_stats", I, [P, P, P, P, P])
     _sig(L, "g5_bfs_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_bfs_free", I, [P])
     _sig(L, "g5_bfs_start", I, [P, c_i64])
@@ -157,6 +160,12 @@ class Graph:
                                     col_start, ctypes.byref(h)))
         return cls(h)
 
+    def select_roots(self, vertices, nroots=64):
+        roots = np.empty(max(nroots, 1), np.int64)
+        found = ctypes.c_int()
+        check(lib().g5_select_roots(self.h, vertices, nroots, _p(roots), ctypes.byref(found)))
+        return roots[:found.value].copy()
+
     def download(self):
         rp = np.empty(self.n_rows + 1, np.int64)
         cols = np.empty(max(self.nnz, 1), np.uint32)
@@ -229,6 +238,16 @@ class BFS:
         ms = ctypes.c_float()
         check(lib().g5_bfs_run(self.h, root, 1 if codec else 0, ctypes.byref(nl), ctypes.byref(ms)))
         return nl.value, ms.value
+
+    def profile(self, enable=True):
+        check(lib().g5_bfs_profile(self.h, 1 if enable else 0))
+
+    def stats(self):
+        ms, bytes_ = ctypes.c_double(), ctypes.c_double()
+        launches, edges = c_i64(), c_i64()
+        check(lib().g5_bfs_stats(self.h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(bytes_),
+                                 ctypes.byref(edges)))
+        return {"ms": ms.value, "launches": launches.value, "bytes": bytes_.value, "edges": edges.value}
 
     def nedge(self):
         n = c_i64()

@@ -52,6 +52,11 @@ struct g5_bfs {
     int64_t root = -1;
     int depth = -1;
     int64_t qlen = 0;
+    // per-kernel accounting of k_expand (g5_bfs_profile / g5_bfs_stats)
+    bool prof = false;
+    hipEvent_t ek0 = nullptr, ek1 = nullptr;
+    double exp_ms = 0, exp_bytes = 0;
+    int64_t exp_launches = 0, exp_edges = 0;
 };
 
 namespace {
@@ -177,7 +182,6 @@ __global__ void k_nedge(const uint32_t *__restrict__ gdeg, const int32_t *__rest
 }
 
 __global__ void k_set_cnt(int64_t *cnt, int idx, int64_t v) { cnt[idx] = v; }
-__global__ void k_copy_cnt(int64_t *cnt, int dst, int src) { cnt[dst] = cnt[src]; }
 
 // One level's expansion from the current in-queue (d_queue, cnt[kQLen]).
 int expand_level(g5_bfs *b)
@@ -193,14 +197,31 @@ int expand_level(g5_bfs *b)
     G5_HIP(g5::exclusive_scan_i64(b->d_qoff, b->qlen, b->d_cnt + kEdges, b->d_scratch, s));
     if (b->qlen > 0) {
         // qoff[qlen] = total, so src_of never reads past the prefix
-        hipLaunchKernelGGL(k_copy_cnt, 1, 1, 0, s, b->d_cnt, (int)kEdges, (int)kEdges);
-        G5_HIP(hipMemcpyAsync(b->d_qoff + b->qlen, b->d_cnt + kEdges, 8, hipMemcpyDeviceToDevice, s));
+        if (b->prof) G5_HIP(hipEventRecord(b->ek0, s));
         hipLaunchKernelGGL(k_expand, kExpandGrid, kExpandBlock, 0, s, (const uint32_t *)b->d_queue,
                            (const int64_t *)b->d_qoff, (const int64_t *)b->d_cnt,
                            (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_cols,
                            (const uint64_t *)b->d_visited, b->d_out, b->d_parent);
         G5_HIP(hipGetLastError());
+        if (b->prof) G5_HIP(hipEventRecord(b->ek1, s));
     }
+    return G5_OK;
+}
+
+// After the level's counters are on the host: k_expand time and its
+// algorithmic bytes, 16 B row-offset pair per frontier vertex, 4 B column id
+// + 8 B visited word per frontier edge, 8 B out word + 4 B parent per newly
+// reached vertex.
+int account_level(g5_bfs *b, int64_t qlen, int64_t n_new)
+{
+    if (!b->prof || qlen == 0) return G5_OK;
+    float ms = 0;
+    G5_HIP(hipEventElapsedTime(&ms, b->ek0, b->ek1));
+    const int64_t E = b->h_cnt[kEdges];
+    b->exp_ms += ms;
+    b->exp_launches += 1;
+    b->exp_edges += E;
+    b->exp_bytes += 16.0 * qlen + 12.0 * E + 12.0 * n_new;
     return G5_OK;
 }
 
@@ -260,6 +281,8 @@ extern "C" int g5_bfs_create(g5_graph *g, g5_bfs **out)
     if (e == hipSuccess) e = hipHostMalloc(&b->h_cnt, kNCnt * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&b->ek0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ek1);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_bfs_create allocation", __FILE__, __LINE__));
     *out = b;
     return G5_OK;
@@ -276,6 +299,8 @@ extern "C" int g5_bfs_free(g5_bfs *b)
     if (b->h_cnt) (void)hipHostFree(b->h_cnt);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ek0) (void)hipEventDestroy(b->ek0);
+    if (b->ek1) (void)hipEventDestroy(b->ek1);
     b->codec.release();
     if (b->s) (void)hipStreamDestroy(b->s);
     delete b;
@@ -327,7 +352,7 @@ extern "C" int g5_bfs_outqueue(g5_bfs *b, uint32_t *q, int64_t *n)
 
 extern "C" int g5_bfs_set_outqueue(g5_bfs *b, const uint32_t *q, int64_t n)
 {
-    G5_REQUIRE(b && n >= 0 && (q || n == 0 || !q), G5_E_ARG, "g5_bfs_set_outqueue: bad arguments");
+    G5_REQUIRE(b && n >= 0, G5_E_ARG, "g5_bfs_set_outqueue: bad arguments");
     G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_outqueue: g5_bfs_start not called");
     g5_graph *g = b->g;
     if (q) {
@@ -438,6 +463,8 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
         G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
         rc = sync_counters(b);
         if (rc) return rc;
+        rc = account_level(b, b->qlen, b->h_cnt[kNew]);
+        if (rc) return rc;
         if (b->h_cnt[kNew] == 0) break;
         b->qlen = b->h_cnt[kQLen];
         if (codec) {
@@ -454,5 +481,24 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
     G5_HIP(hipEventSynchronize(b->ev1));
     if (ms) G5_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
     if (nlevels) *nlevels = b->depth + 1;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_profile(g5_bfs *b, int enable)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_profile: null handle");
+    b->prof = enable != 0;
+    b->exp_ms = b->exp_bytes = 0;
+    b->exp_launches = b->exp_edges = 0;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_stats(g5_bfs *b, double *ms, int64_t *launches, double *bytes, int64_t *edges)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_stats: null handle");
+    if (ms) *ms = b->exp_ms;
+    if (launches) *launches = b->exp_launches;
+    if (bytes) *bytes = b->exp_bytes;
+    if (edges) *edges = b->exp_edges;
     return G5_OK;
 }

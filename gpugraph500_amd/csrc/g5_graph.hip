@@ -13,6 +13,7 @@
 // the unique/select passes are rocPRIM device primitives (construction is not
 // on the BFS hot path).
 #include <cstring>
+#include <random>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -495,5 +496,52 @@ extern "C" int g5_kronecker_edges(int scale, uint64_t seed1, uint64_t seed2, int
                        (const uint32_t *)g_skip.d, ks, scale, (int64_t *)buf.p);
     G5_HIP(hipGetLastError());
     G5_HIP(hipMemcpy(edges, buf.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+    return G5_OK;
+}
+
+namespace {
+
+__global__ void k_cand_degrees(const int64_t *__restrict__ rowptr, int64_t row_start, int64_t n_rows,
+                               const int64_t *__restrict__ cand, int n, int64_t *__restrict__ deg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = cand[i] - row_start;
+    deg[i] = (r >= 0 && r < n_rows) ? rowptr[r + 1] - rowptr[r] : 0;
+}
+
+}  // namespace
+
+// Root selection of cpu_2d/main.cpp:351-411: draws from a default-seeded
+// std::knuth_b through uniform_int_distribution<int64>(0, vertices-1); a draw
+// is kept if it was not drawn before and its CSR degree is nonzero; at most
+// 32 draws per requested root.  The draw sequence does not depend on which
+// draws are kept, so all candidate degrees are fetched in one launch.
+extern "C" int g5_select_roots(const g5_graph *g, int64_t vertices, int nroots, int64_t *roots, int *found)
+{
+    G5_REQUIRE(g && roots && found && nroots >= 0 && vertices >= 1, G5_E_ARG, "g5_select_roots: bad arguments");
+    const int maxtries = nroots * 32;
+    std::knuth_b gen;
+    std::uniform_int_distribution<int64_t> dist(0, vertices - 1);
+    std::vector<int64_t> cand(maxtries), deg(maxtries);
+    for (int i = 0; i < maxtries; ++i) cand[i] = dist(gen);
+    *found = 0;
+    if (maxtries == 0) return G5_OK;
+    DevBuf dc, dd;
+    G5_HIP(hipMalloc(&dc.p, (size_t)maxtries * 8));
+    G5_HIP(hipMalloc(&dd.p, (size_t)maxtries * 8));
+    G5_HIP(hipMemcpy(dc.p, cand.data(), (size_t)maxtries * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_cand_degrees, (maxtries + 255) / 256, 256, 0, nullptr, (const int64_t *)g->d_rowptr,
+                       g->row_start, g->n_rows, (const int64_t *)dc.p, maxtries, (int64_t *)dd.p);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpy(deg.data(), dd.p, (size_t)maxtries * 8, hipMemcpyDeviceToHost));
+    int n = 0;
+    for (int t = 0; t < maxtries && n < nroots; ++t) {
+        bool dup = false;
+        for (int i = 0; i < n; ++i) dup |= roots[i] == cand[t];
+        if (dup) continue;
+        if (deg[t] > 0) roots[n++] = cand[t];
+    }
+    *found = n;
     return G5_OK;
 }

@@ -70,6 +70,11 @@ int g5_graph_free(g5_graph *g);
  * to edges[2*(end-start)] (generate_kronecker_range, graph_generator.c:157-187). */
 int g5_kronecker_edges(int scale, uint64_t seed1, uint64_t seed2, int64_t start, int64_t end,
                        int64_t *edges);
+/* The benchmark's search keys (cpu_2d/main.cpp:351-411): up to nroots
+ * distinct vertices with nonzero degree, drawn from a default-seeded
+ * std::knuth_b via uniform_int_distribution<int64>(0, vertices-1), at most
+ * 32 draws per root; *found = number selected. */
+int g5_select_roots(const g5_graph *g, int64_t vertices, int nroots, int64_t *roots, int *found);
 
 /* ---- local expansion plug-in: the CUDA_BFS hooks (cuda/cuda_bfs.h:68-83) ---- */
 int g5_bfs_create(g5_graph *g, g5_bfs **out);
@@ -99,6 +104,12 @@ int g5_bfs_result(g5_bfs *b, int64_t *pred, int32_t *level);
  * delivers it to non-root ranks (globalbfs.hh:864-905). *ms = device time. */
 enum { G5_RUN_CODEC = 1 };
 int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
+/* Per-kernel accounting of the expansion kernel across g5_bfs_run calls:
+ * enable (resets the totals), then read total device ms, launches,
+ * algorithmic bytes (16 B per frontier vertex, 12 B per frontier edge, 12 B
+ * per newly reached vertex) and frontier edges. */
+int g5_bfs_profile(g5_bfs *b, int enable);
+int g5_bfs_stats(g5_bfs *b, double *ms, int64_t *launches, double *bytes, int64_t *edges);
 /* TEPS numerator of the last search: generated undirected edges with a
  * reached endpoint (validate/validate.h:439-485). */
 int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);

@@ -40,3 +40,10 @@ def test_upload_rejects_bad_csr():
         g5.Graph.upload(np.array([0, 1], np.int64), np.array([70], np.uint32), 64)  # col out of range
     with pytest.raises(g5.G5Error):
         g5.Graph.upload(np.zeros(11, np.int64), np.zeros(0, np.uint32), 10)  # rows not multiple of 64
+
+
+@pytest.mark.parametrize("scale", [10, 16])
+def test_select_roots_matches_oracle(scale):
+    g = g5.Graph.kronecker(scale)
+    ref = O.build_csr(O.kronecker_edges(scale))
+    assert np.array_equal(g.select_roots(1 << scale, 64), O.select_roots(ref, 1 << scale))
