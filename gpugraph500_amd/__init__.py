@@ -70,6 +70,14 @@ def lib():
     _sig(L, "g5_bfs_result", I, [P, P, P])
     _sig(L, "g5_bfs_run", I, [P, c_i64, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_bfs_nedge", I, [P, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_msbfs_create", I, [P, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_msbfs_free", I, [P])
+    _sig(L, "g5_msbfs_run", I, [P, P, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_msbfs_result", I, [P, I, P, P])
+    _sig(L, "g5_msbfs_nedge", I, [P, P])
+    _sig(L, "g5_msbfs_profile", I, [P, I])
+    _sig(L, "g5_msbfs_stats", I, [P, I, P, P, P])
+    _sig(L, "g5_msbfs_info", I, [P, P, P, P])
     _sig(L, "g5_s4_max_words", c_sz, [c_sz])
     _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
@@ -257,6 +265,65 @@ class BFS:
     def free(self):
         if self.h:
             lib().g5_bfs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class MSBFS:
+    """Batched multi-source BFS over up to 64 roots (uint64 lane masks)."""
+
+    KERNELS = ("k_pull_thread", "k_pull_wave", "k_pull_block")
+
+    def __init__(self, graph):
+        self.graph = graph
+        self.h = c_vp()
+        check(lib().g5_msbfs_create(graph.h, ctypes.byref(self.h)))
+        self.nroots = 0
+
+    def run(self, roots):
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        nl = c_i32()
+        ms = ctypes.c_float()
+        check(lib().g5_msbfs_run(self.h, _p(r), r.size, ctypes.byref(nl), ctypes.byref(ms)))
+        self.nroots = r.size
+        return nl.value, ms.value
+
+    def result(self, lane):
+        pred = np.empty(self.graph.n_cols, np.int64)
+        lev = np.empty(self.graph.n_cols, np.int32)
+        check(lib().g5_msbfs_result(self.h, lane, _p(pred), _p(lev)))
+        return pred, lev
+
+    def nedge(self):
+        out = np.empty(64, np.int64)
+        check(lib().g5_msbfs_nedge(self.h, _p(out)))
+        return out[:self.nroots].copy()
+
+    def profile(self, enable=True):
+        check(lib().g5_msbfs_profile(self.h, 1 if enable else 0))
+
+    def stats(self):
+        res = {}
+        for k, name in enumerate(self.KERNELS):
+            ms, bytes_ = ctypes.c_double(), ctypes.c_double()
+            launches = c_i64()
+            check(lib().g5_msbfs_stats(self.h, k, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(bytes_)))
+            res[name] = {"ms": ms.value, "launches": launches.value, "bytes": bytes_.value}
+        return res
+
+    def bins(self):
+        a, b, c = c_i64(), c_i64(), c_i64()
+        check(lib().g5_msbfs_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def free(self):
+        if self.h:
+            lib().g5_msbfs_free(self.h)
             self.h = None
 
     def __del__(self):

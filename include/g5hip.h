@@ -39,6 +39,7 @@ enum {
 
 typedef struct g5_graph g5_graph;
 typedef struct g5_bfs g5_bfs;
+typedef struct g5_msbfs g5_msbfs;
 
 /* ---- library ---- */
 const char *g5_last_error(void);
@@ -113,6 +114,28 @@ int g5_bfs_stats(g5_bfs *b, double *ms, int64_t *launches, double *bytes, int64_
 /* TEPS numerator of the last search: generated undirected edges with a
  * reached endpoint (validate/validate.h:439-485). */
 int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);
+
+/* ---- batched multi-source BFS (no counterpart in the reference: it stands
+ * beside the single-root plug-in; results equal nroots separate searches
+ * with the minimum-id parent rule of cpu_2d/cpubfs_bin.cpp:150-173) ----
+ * Needs a diagonal (1x1) block.  Up to 64 roots per run, one uint64 lane mask
+ * per vertex; parents and levels are kept as int32[n][64] in HBM. */
+int g5_msbfs_create(g5_graph *g, g5_msbfs **out);
+int g5_msbfs_free(g5_msbfs *m);
+/* Searches all roots; *nlevels = levels of the deepest search, *ms = device time. */
+int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32_t *nlevels, float *ms);
+/* Root `lane`'s result as the single-root search returns it: pred int64[n]
+ * (-1 unreached, root -> root) and level int32[n]; either may be NULL. */
+int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level);
+/* TEPS numerators nedge[nroots] of the last run (validate/validate.h:439-485). */
+int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
+/* Per-kernel accounting (kernel 0: thread-per-vertex, 1: wave-per-vertex,
+ * 2: workgroup-per-vertex pull): enabling runs levels as direct launches
+ * timed with events (otherwise each level replays a captured hipGraph). */
+int g5_msbfs_profile(g5_msbfs *m, int enable);
+int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *launches, double *bytes);
+/* Sizes of the three degree bins (<= 32, <= 2048, > 2048 neighbours). */
+int g5_msbfs_info(g5_msbfs *m, int64_t *n_small, int64_t *n_medium, int64_t *n_large);
 
 /* ---- s4-bp128-d4 codec (codecfactory.h:103-105) on the device ----
  * Word stream identical to the reference codec's output for 16-byte aligned

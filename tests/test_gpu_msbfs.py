@@ -1,0 +1,110 @@
+"""Batched multi-source BFS: every lane's levels and parents bit-exact
+against a separate single-root oracle search (minimum-id parent rule)."""
+import numpy as np
+import pytest
+
+import gpugraph500_amd as g5
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+_CACHE = {}
+
+
+def kron(scale):
+    if scale not in _CACHE:
+        e = O.kronecker_edges(scale)
+        ref = O.build_csr(e)
+        g = g5.Graph.upload(ref.rowptr, ref.cols, ref.n_rows, gdeg=ref.gdeg)
+        _CACHE[scale] = (e, ref, g)
+    return _CACHE[scale]
+
+
+@pytest.mark.parametrize("scale", [10, 14, 16])
+def test_64_lanes_bit_exact(scale):
+    e, ref, g = kron(scale)
+    roots = O.select_roots(ref, 1 << scale)
+    m = g5.MSBFS(g)
+    nl, ms = m.run(roots)
+    ne = m.nedge()
+    maxl = 0
+    for lane, r in enumerate(roots.tolist()):
+        pred, lev = m.result(lane)
+        opred, olev, onl = O.bfs_bin(ref, r)
+        assert np.array_equal(lev, olev), f"levels differ, lane {lane} root {r}"
+        assert np.array_equal(pred, opred), f"parents differ, lane {lane} root {r}"
+        assert ne[lane] == O.nedge(ref, olev)
+        maxl = max(maxl, onl)
+    assert nl == maxl
+
+
+def test_fewer_lanes_and_repeat_runs():
+    e, ref, g = kron(12)
+    roots = O.select_roots(ref, 1 << 12)
+    m = g5.MSBFS(g)
+    for k in (1, 7, 33):
+        sub = roots[-k:]
+        m.run(sub)
+        for lane, r in enumerate(sub.tolist()):
+            pred, lev = m.result(lane)
+            opred, olev, _ = O.bfs_bin(ref, r)
+            assert np.array_equal(lev, olev) and np.array_equal(pred, opred)
+
+
+def test_duplicate_roots_share_a_vertex():
+    e, ref, g = kron(10)
+    r = int(O.select_roots(ref, 1 << 10)[0])
+    m = g5.MSBFS(g)
+    m.run([r, r, r])
+    opred, olev, _ = O.bfs_bin(ref, r)
+    for lane in range(3):
+        pred, lev = m.result(lane)
+        assert np.array_equal(lev, olev) and np.array_equal(pred, opred)
+
+
+def test_profile_path_equals_graph_path():
+    e, ref, g = kron(14)
+    roots = O.select_roots(ref, 1 << 14)
+    m = g5.MSBFS(g)
+    m.profile(True)
+    m.run(roots)
+    st = m.stats()
+    assert sum(v["launches"] for v in st.values()) > 0
+    pa = [m.result(l) for l in (0, 31, 63)]
+    m.profile(False)
+    m.run(roots)
+    pb = [m.result(l) for l in (0, 31, 63)]
+    for (p1, l1), (p2, l2) in zip(pa, pb):
+        assert np.array_equal(p1, p2) and np.array_equal(l1, l2)
+
+
+def test_tiny_graphs_all_roots_in_one_batch():
+    rng = np.random.default_rng(77)
+    for trial in range(20):
+        n = int(rng.integers(2, 60))
+        m_ = int(rng.integers(1, 5 * n))
+        edges = rng.integers(0, n, size=(m_, 2))
+        # add one hub so the wave and block kernels see work
+        hub = np.stack([np.zeros(3000, np.int64), rng.integers(0, n, 3000)], 1) if trial % 5 == 0 else np.zeros((0, 2), np.int64)
+        ref = O.build_csr(np.concatenate([edges, hub]), 64)
+        g = g5.Graph.upload(ref.rowptr, ref.cols, 64, gdeg=ref.gdeg)
+        roots = [v for v in range(n) if ref.rowptr[v + 1] > ref.rowptr[v]][:64]
+        if not roots:
+            continue
+        ms = g5.MSBFS(g)
+        ms.run(roots)
+        for lane, r in enumerate(roots):
+            pred, lev = ms.result(lane)
+            opred, olev, _ = O.bfs_bin(ref, r)
+            assert np.array_equal(lev, olev) and np.array_equal(pred, opred), (trial, r)
+        ms.free()
+        g.free()
+
+
+def test_bad_roots_rejected():
+    e, ref, g = kron(10)
+    m = g5.MSBFS(g)
+    with pytest.raises(g5.G5Error):
+        m.run(np.arange(65))
+    with pytest.raises(g5.G5Error):
+        m.run([ref.n_rows])
