@@ -87,6 +87,10 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--roots", type=int, default=64)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="WORDS",
+                    help="only run the plain copy kernel (k_or over one map) on WORDS uint64 words, "
+                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the 2D row/column exchange "
@@ -97,6 +101,14 @@ def main():
     import gpugraph500_amd as g5
 
     g5.init(0)
+    if args.copy_calibration:
+        n = args.copy_calibration
+        src = g5.DeviceArray(n, np.uint64)
+        dst = g5.DeviceArray(n, np.uint64)
+        for _ in range(3):
+            g5.check(g5.lib().g5_vreduce_or(src.ptr, 1, n, dst.ptr))
+        print(json.dumps({"copy_calibration_words": n, "bytes_read": 8 * n, "bytes_written": 8 * n}))
+        return 0
     t0 = time.perf_counter()
     g = g5.Graph.kronecker(args.scale)
     log(f"graph: scale {args.scale}, {g.nglobal} vertices, {g.nnz} nonzeros, "
@@ -134,8 +146,13 @@ def main():
     m.profile(True)            # one run as direct launches with per-kernel events
     m.run(roots)
     kst = m.stats()
+    if args.levels:
+        for i, rec in enumerate(m.levels()):
+            log(f"level {i + 1} {'push' if rec[9] else 'pull'}: ms {rec[0]:.2f}/{rec[1]:.2f}/{rec[2]:.2f}/{rec[3]:.2f} "
+                f"edges {rec[4]:.3e}/{rec[5]:.3e}/{rec[6]:.3e}/{rec[7]:.3e} new {rec[8]:.3e} "
+                f"frontier {rec[10]:.3e} v {rec[11]:.3e} e")
     m.profile(False)
-    dom = max(kst, key=lambda k: kst[k]["ms"])
+    dom = max((k for k in kst if k != "push"), key=lambda k: kst[k]["ms"])
     d = kst[dom]
     dom_gbs = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

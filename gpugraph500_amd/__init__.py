@@ -78,6 +78,8 @@ def lib():
     _sig(L, "g5_msbfs_profile", I, [P, I])
     _sig(L, "g5_msbfs_stats", I, [P, I, P, P, P])
     _sig(L, "g5_msbfs_info", I, [P, P, P, P])
+    _sig(L, "g5_msbfs_levels", I, [P, P, I, ctypes.POINTER(I)])
+    _sig(L, "g5_msbfs_set_mode", I, [P, I])
     _sig(L, "g5_s4_max_words", c_sz, [c_sz])
     _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
@@ -277,7 +279,7 @@ class BFS:
 class MSBFS:
     """Batched multi-source BFS over up to 64 roots (uint64 lane masks)."""
 
-    KERNELS = ("k_pull_thread", "k_pull_wave", "k_pull_block")
+    KERNELS = ("k_pull_thread", "k_pull_wave", "k_pull_block", "push")
 
     def __init__(self, graph):
         self.graph = graph
@@ -315,6 +317,17 @@ class MSBFS:
             check(lib().g5_msbfs_stats(self.h, k, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(bytes_)))
             res[name] = {"ms": ms.value, "launches": launches.value, "bytes": bytes_.value}
         return res
+
+    def levels(self):
+        n = ctypes.c_int()
+        check(lib().g5_msbfs_levels(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1) * 12, np.float64)
+        check(lib().g5_msbfs_levels(self.h, _p(out), n.value, ctypes.byref(n)))
+        return out[:n.value * 12].reshape(-1, 12)
+
+    def set_mode(self, mode):
+        """0: direction per level, 1: pull only, 2: push only."""
+        check(lib().g5_msbfs_set_mode(self.h, mode))
 
     def bins(self):
         a, b, c = c_i64(), c_i64(), c_i64()

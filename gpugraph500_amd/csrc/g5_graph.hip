@@ -379,6 +379,28 @@ int build_kronecker(int scale, int edgefactor, uint64_t s1, uint64_t s2, g5_grap
 
 }  // namespace
 
+namespace g5 {
+
+hipError_t sort_u64(uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, hipStream_t s)
+{
+    if (n <= 1) return hipSuccess;
+    rocprim::double_buffer<uint64_t> db(keys, tmp);
+    size_t tb = 0;
+    hipError_t e = rocprim::radix_sort_keys(nullptr, tb, db, (size_t)n, 0u, (unsigned)end_bit, s);
+    if (e != hipSuccess) return e;
+    void *t = nullptr;
+    e = hipMallocAsync(&t, tb ? tb : 1, s);
+    if (e != hipSuccess) return e;
+    e = rocprim::radix_sort_keys(t, tb, db, (size_t)n, 0u, (unsigned)end_bit, s);
+    hipError_t e2 = hipFreeAsync(t, s);
+    if (e != hipSuccess) return e;
+    if (e2 != hipSuccess) return e2;
+    if (db.current() != keys) return hipMemcpyAsync(keys, db.current(), (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+    return hipSuccess;
+}
+
+}  // namespace g5
+
 extern "C" int g5_graph_upload(const int64_t *rowptr, const uint32_t *cols, int64_t n_rows,
                                int64_t n_cols, const int64_t *gdeg, int64_t row_start,
                                int64_t col_start, g5_graph **out)

@@ -74,6 +74,9 @@ hipError_t fill_u32(uint32_t *d, int64_t n, uint32_t v, hipStream_t s);
 hipError_t fill_i32(int32_t *d, int64_t n, int32_t v, hipStream_t s);
 hipError_t fill_i64(int64_t *d, int64_t n, int64_t v, hipStream_t s);
 int grid_for(int64_t n, int block, int max_grid = 1 << 20);
+// Ascending radix sort of n keys on bits [0, end_bit); tmp holds n keys.
+// The result ends in keys.
+hipError_t sort_u64(uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, hipStream_t s);
 
 }  // namespace g5
 

@@ -10,17 +10,33 @@
 // found.  A vertex is only written by the work item that owns it, so the
 // level needs no atomics on the masks: visited[v] |= found, next[v] = found.
 //
-// Work is binned statically by degree (per graph, once): one thread per
-// vertex up to 32 neighbours, one wavefront (64 neighbours per step, first
-// hits per lane from an exclusive prefix-OR across lanes) up to 2048, one
-// 256-thread workgroup beyond.  Each level's launch sequence is captured once
-// into a hipGraph per frontier/next parity and replayed; the level number
-// lives in device memory.
+// Sparse frontiers (fewer than nnz/16 frontier edges) are pushed instead:
+// next[v] |= frontier[u] & ~visited[v] along the frontier's edges
+// (edge-balanced tiles, atomicOr), then the new (vertex, root) parent slots
+// are lowered with atomicMin over the same edges -- the same minimum-id
+// parent.  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
+// an empty frontier word.
 //
-// Output layout in HBM: parent and level int32[n][64] (vertex-major: a
-// vertex's 64 lanes share one 256-byte line).  Per-root arrays are gathered
-// by g5_msbfs_result.
+// Relabelling: the search runs on a copy of the block whose vertices are
+// renumbered by descending degree (ties by id).  Frontier words of the hubs,
+// which most neighbour lists point at, then sit in a small id range that the
+// L2 and the Infinity Cache hold, instead of being scattered over the whole
+// n * 8 bytes.  Every row keeps its neighbours in ascending ORIGINAL id
+// order and parents are recorded as original ids, so the first hit of a
+// scan is still the minimum-id parent.  The degree bins are contiguous id
+// ranges: one thread per vertex up to 32 neighbours, one wavefront (64
+// neighbours per step, first hits from an exclusive prefix-OR across lanes)
+// up to 2048, one 256-thread workgroup beyond.  Each pull level's launch
+// sequence is captured once into a hipGraph per frontier parity and
+// replayed; the level number lives in device memory.
+//
+// Output layout in HBM: parent int32[n][64] (original ids) and level
+// uint16[n][64] (the reference validator's depth width, validate.h:48-58),
+// vertex-major in the relabelled order, valid where the vertex's visited bit
+// for that lane is set.  Per-root arrays in original ids are gathered by
+// g5_msbfs_result.
 #include <algorithm>
+#include <vector>
 
 #include "g5_internal.hpp"
 
@@ -29,14 +45,27 @@ namespace {
 constexpr int kSmallMax = 32;
 constexpr int kMediumMax = 2048;
 constexpr int kBlock = 256;
+constexpr int kLevelRec = 12;
+// push when the frontier's edges are fewer than nnz / kPushRatio
+constexpr int64_t kPushRatio = 16;
 
-// counters: per degree class c (0 small, 1 medium, 2 large) slots 3c+0 new
-// (vertex, root) pairs, 3c+1 edges read, 3c+2 vertices with an unvisited lane
-enum MsCnt { kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kDepth = 9, kMsN = 16 };
+// counters: per kernel class c (0 thread pull, 1 wave pull, 2 block pull,
+// 3 push) slots 4c+0 new (vertex, root) pairs, 4c+1 edges read, 4c+2
+// vertices worked on, 4c+3 frontier words gathered; then the level number,
+// the next frontier's active vertices and their edges, the push queue
+// length, and the push frontier queue length / edge total.
+enum MsCnt {
+    kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3,
+    kDepth = 16, kActVerts = 17, kActEdges = 18, kNq = 19, kFqLen = 20, kFqEdges = 21, kMsN = 24
+};
 
-struct Lists {
-    uint32_t *small = nullptr, *medium = nullptr, *large = nullptr;
-    int64_t n_small = 0, n_medium = 0, n_large = 0;
+// The degree-ordered copy of the block.
+struct Relabeled {
+    uint32_t *new2old = nullptr, *old2new = nullptr;
+    int64_t *rowptr = nullptr;
+    uint32_t *cols = nullptr;  // new ids, rows in ascending original-id order
+    uint32_t *gdeg = nullptr;
+    int64_t n_large = 0, n_medium = 0, n_small = 0;  // bins: [0,L) [L,L+M) [L+M,L+M+S)
 };
 
 }  // namespace
@@ -46,11 +75,15 @@ struct g5_msbfs {
     hipStream_t s = nullptr;
     int64_t n = 0;                       // vertices (rows == cols)
     uint64_t *d_visited = nullptr, *d_fa = nullptr, *d_fb = nullptr;
-    int32_t *d_parent = nullptr, *d_level = nullptr;
+    uint64_t *d_act = nullptr;           // two n/64-word active bitmaps (parity a, b)
+    uint32_t *d_fq = nullptr, *d_nq = nullptr;  // push: frontier queue, newly reached queue
+    int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
+    int32_t *d_parent = nullptr;
+    uint16_t *d_level = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
     int64_t *d_scratch = nullptr;
     int64_t *d_roots = nullptr;
-    Lists lists;
+    Relabeled rl;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nroots = 0;
@@ -58,12 +91,16 @@ struct g5_msbfs {
     // accounting
     bool prof = false;
     hipEvent_t ek[4] = {nullptr, nullptr, nullptr, nullptr};
-    double k_ms[3] = {0, 0, 0}, k_bytes[3] = {0, 0, 0};
-    int64_t k_launches[3] = {0, 0, 0};
+    double k_ms[4] = {0, 0, 0, 0}, k_bytes[4] = {0, 0, 0, 0};
+    int64_t k_launches[4] = {0, 0, 0, 0};
+    // per level of the last profiled run (kLevelRec doubles each)
+    std::vector<double> lvl;
+    int mode = 0;  // 0 direction chosen per level, 1 pull only, 2 push only
 };
 
 namespace {
 
+// parent slots of vertex v for the roots in `bits` := u (an original id)
 __device__ __forceinline__ void write_parents(int32_t *__restrict__ parent, int64_t v, uint64_t bits, uint32_t u)
 {
     while (bits) {
@@ -86,19 +123,47 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
     return __lane_id() == 0 ? 0ull : prev;
 }
 
+// One 64-neighbour step of vertex v's scan, neighbours [base, base+64) of
+// [.., e): lanes gather the frontier words of active neighbours restricted to
+// `want`; the lane holding the first occurrence of a root's bit writes that
+// root's parent.  Returns the new bits (wave-uniform); found is wave-uniform.
+__device__ __forceinline__ uint64_t wave_step(int64_t base, int64_t e, uint64_t want, uint64_t found, int64_t v,
+                                              const uint32_t *__restrict__ cols,
+                                              const uint64_t *__restrict__ frontier,
+                                              const uint64_t *__restrict__ active,
+                                              const uint32_t *__restrict__ new2old, int32_t *__restrict__ parent,
+                                              int64_t &ngath)
+{
+    const int64_t j = base + __lane_id();
+    uint32_t u = 0;
+    uint64_t fresh = 0;
+    if (j < e) {
+        u = cols[j];
+        if (active[u >> 6] >> (u & 63) & 1) {
+            fresh = frontier[u] & want & ~found;
+            ++ngath;
+        }
+    }
+    if (__ballot(fresh != 0) == 0) return 0;
+    const uint64_t first = fresh & ~lanes_excl_or(fresh);
+    if (first) write_parents(parent, v, first, new2old[u]);
+    return g5_wave_or(fresh);
+}
+
 // One thread per vertex (degree <= kSmallMax).
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
-    const uint32_t *__restrict__ list, int64_t nlist, const int64_t *__restrict__ rowptr,
-    const uint32_t *__restrict__ cols, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
-    uint64_t *__restrict__ next, int32_t *__restrict__ parent, int32_t *__restrict__ level,
-    int64_t *__restrict__ cnt_all)
+    int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
 {
-    const int32_t depth = (int32_t)cnt_all[kDepth];
+    const uint16_t depth = (uint16_t)cnt_all[kDepth];
     int64_t *__restrict__ cnt = cnt_all + 0;
+    int64_t ngath = 0;
     int64_t nbits = 0, nedges = 0, nverts = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nlist;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t v = list[i];
+        const uint32_t v = (uint32_t)(first_v + i);
         const uint64_t vis = visited[v];
         if (vis == ~0ull) {
             next[v] = 0;
@@ -110,10 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
         int64_t j = b;
         for (; j < e; ++j) {
             const uint32_t u = cols[j];
+            if (!(active[u >> 6] >> (u & 63) & 1)) continue;
+            ++ngath;
             const uint64_t f = frontier[u] & unv & ~found;
             if (f) {
                 found |= f;
-                write_parents(parent, v, f, u);
+                write_parents(parent, v, f, new2old[u]);
                 if (found == unv) {
                     ++j;
                     break;
@@ -137,7 +204,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     nbits = g5_wave_sum(nbits);
     nedges = g5_wave_sum(nedges);
     nverts = g5_wave_sum(nverts);
+    ngath = g5_wave_sum(ngath);
     if (__lane_id() == 0) {
+        if (ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
         if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
         if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
         if (nverts) atomicAdd((unsigned long long *)&cnt[kVisitedVerts], (unsigned long long)nverts);
@@ -146,18 +215,19 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
 
 // One wavefront per vertex (kSmallMax < degree <= kMediumMax).
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
-    const uint32_t *__restrict__ list, int64_t nlist, const int64_t *__restrict__ rowptr,
-    const uint32_t *__restrict__ cols, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
-    uint64_t *__restrict__ next, int32_t *__restrict__ parent, int32_t *__restrict__ level,
-    int64_t *__restrict__ cnt_all)
+    int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
 {
-    const int32_t depth = (int32_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 3;
+    const uint16_t depth = (uint16_t)cnt_all[kDepth];
+    int64_t *__restrict__ cnt = cnt_all + 4;
+    int64_t ngath = 0;
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nbits = 0, nedges = 0, nverts = 0;
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += waves) {
-        const uint32_t v = list[i];
+        const uint32_t v = (uint32_t)(first_v + i);
         const uint64_t vis = visited[v];
         if (vis == ~0ull) {
             if (lane == 0) next[v] = 0;
@@ -168,16 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
         for (; base < e; base += 64) {
-            const int64_t j = base + lane;
-            uint32_t u = 0;
-            uint64_t f = 0;
-            if (j < e) {
-                u = cols[j];
-                f = frontier[u] & unv;
-            }
-            const uint64_t first = f & ~lanes_excl_or(f) & ~found;
-            if (first) write_parents(parent, v, first, u);
-            found |= g5_wave_or(f);
+            found |= wave_step(base, e, unv, found, v, cols, frontier, active, new2old, parent, ngath);
             if (found == unv) {
                 base += 64;
                 break;
@@ -192,7 +253,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
         nedges += min(base, e) - b;
         ++nverts;
     }
+    ngath = g5_wave_sum(ngath);
     if (lane == 0) {
+        if (ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
         if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
         if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
         if (nverts) atomicAdd((unsigned long long *)&cnt[kVisitedVerts], (unsigned long long)nverts);
@@ -201,19 +264,20 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
 
 // One workgroup per vertex (degree > kMediumMax): 256 neighbours per round.
 __global__ __launch_bounds__(kBlock) void k_pull_block(
-    const uint32_t *__restrict__ list, int64_t nlist, const int64_t *__restrict__ rowptr,
-    const uint32_t *__restrict__ cols, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
-    uint64_t *__restrict__ next, int32_t *__restrict__ parent, int32_t *__restrict__ level,
-    int64_t *__restrict__ cnt_all)
+    int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
 {
     __shared__ uint64_t s_wave[kBlock / 64];
     __shared__ uint64_t s_vis;
-    const int32_t depth = (int32_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 6;
+    const uint16_t depth = (uint16_t)cnt_all[kDepth];
+    int64_t *__restrict__ cnt = cnt_all + 8;
+    int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     int64_t nbits = 0, nedges = 0, nverts = 0;
     for (int64_t i = blockIdx.x; i < nlist; i += gridDim.x) {
-        const uint32_t v = list[i];
+        const uint32_t v = (uint32_t)(first_v + i);
         if (threadIdx.x == 0) s_vis = visited[v];
         __syncthreads();
         const uint64_t vis = s_vis;
@@ -232,10 +296,15 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             uint64_t f = 0;
             if (j < e) {
                 u = cols[j];
-                f = frontier[u] & unv;
+                if (active[u >> 6] >> (u & 63) & 1) {
+                    f = frontier[u] & unv;
+                    ++ngath;
+                }
             }
-            const uint64_t in_wave = lanes_excl_or(f);
-            const uint64_t wave_all = __shfl(in_wave | f, 63, 64);
+            f &= ~found;
+            const bool any = __ballot(f != 0) != 0;
+            const uint64_t in_wave = any ? lanes_excl_or(f) : 0ull;
+            const uint64_t wave_all = any ? __shfl(in_wave | f, 63, 64) : 0ull;
             if (lane == 0) s_wave[wv] = wave_all;
             __syncthreads();
             uint64_t before = 0, all = 0;
@@ -245,8 +314,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
                 before |= (w < wv) ? t : 0ull;
                 all |= t;
             }
-            const uint64_t first = f & ~(in_wave | before) & ~found;
-            if (first) write_parents(parent, v, first, u);
+            const uint64_t first = f & ~(in_wave | before);
+            if (first) write_parents(parent, v, first, new2old[u]);
             found |= all;
             __syncthreads();
             if (found == unv) {
@@ -263,6 +332,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             ++nverts;
         }
     }
+    ngath = g5_wave_sum(ngath);
+    if (lane == 0 && ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
     if (threadIdx.x == 0) {
         if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
         if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
@@ -270,84 +341,245 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     }
 }
 
+// ---- push (top-down) levels for sparse frontiers ----
+// The frontier's vertices u (active bitmap -> sorted queue) push their lane
+// bits along their edges: next[v] |= frontier[u] & ~visited[v].  Parents are
+// then the minimum frontier neighbour per (v, root): every slot of a newly
+// reached pair is set to INT32_MAX, and a second pass over the same edges
+// lowers it with atomicMin.  Work is balanced by edges: each block tile takes
+// 2048 consecutive edges of the frontier and finds each edge's source by
+// binary search in the frontier's degree prefix.
+constexpr int kPushTile = 2048;
+constexpr int kPushGrid = 4096;
+
+__device__ __forceinline__ int64_t fq_src(const int64_t *__restrict__ qoff, int64_t lo, int64_t hi, int64_t e)
+{
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (qoff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__restrict__ cnt,
+                             const int64_t *__restrict__ rowptr, int64_t *__restrict__ qoff)
+{
+    const int64_t n = cnt[kFqLen];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = fq[i];
+        qoff[i] = rowptr[u + 1] - rowptr[u];
+    }
+}
+
+template <bool kMin>
+__global__ __launch_bounds__(kBlock) void k_push_edges(
+    const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
+    uint32_t *__restrict__ nq, int64_t *__restrict__ cnt)
+{
+    __shared__ int64_t s_lo, s_hi;
+    const int64_t qn = cnt[kFqLen], E = cnt[kFqEdges];
+    for (int64_t t0 = blockIdx.x * (int64_t)kPushTile; t0 < E; t0 += (int64_t)gridDim.x * kPushTile) {
+        const int64_t t1 = min(E, t0 + (int64_t)kPushTile);
+        if (threadIdx.x == 0) {
+            s_lo = fq_src(qoff, 0, qn - 1, t0);
+            s_hi = fq_src(qoff, s_lo, qn - 1, t1 - 1);
+        }
+        __syncthreads();
+        const int64_t lo = s_lo, hi = s_hi;
+        for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
+            const int64_t si = fq_src(qoff, lo, hi, e);
+            const uint32_t u = fq[si];
+            const uint32_t v = cols[rowptr[u] + (e - qoff[si])];
+            const uint64_t f = frontier[u] & ~visited[v];
+            if (!f) continue;
+            if (!kMin) {
+                if ((next[v] & f) != f) {
+                    const uint64_t old = atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
+                    if (old == 0) nq[atomicAdd((unsigned long long *)&cnt[kNq], 1ull)] = v;
+                }
+            } else {
+                uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
+                const int32_t uo = (int32_t)new2old[u];
+                while (bits) {
+                    const int r = __ffsll((long long)bits) - 1;
+                    int32_t *slot = &parent[(int64_t)v * 64 + r];
+                    if (*slot > uo) atomicMin(slot, uo);
+                    bits &= bits - 1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!kMin && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)&cnt[12 + kEdgesRead], (unsigned long long)E);
+}
+
+// wave per newly reached vertex: stage = 0 parent slots := INT32_MAX,
+// stage = 1 visited |= next, level := depth, counts
+template <int kStage>
+__global__ __launch_bounds__(kBlock) void k_push_vertices(const uint32_t *__restrict__ nq, uint64_t *__restrict__ next,
+                                                          uint64_t *__restrict__ visited, int32_t *__restrict__ parent,
+                                                          uint16_t *__restrict__ level, int64_t *__restrict__ cnt)
+{
+    const int64_t n = cnt[kNq];
+    const uint16_t depth = (uint16_t)cnt[kDepth];
+    const int lane = __lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+    int64_t nbits = 0, nv = 0;
+    for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t v = nq[i];
+        const uint64_t w = next[v];
+        if (kStage == 0) {
+            if (w >> lane & 1) parent[(int64_t)v * 64 + lane] = 0x7FFFFFFF;
+        } else {
+            if (w >> lane & 1) level[(int64_t)v * 64 + lane] = depth;
+            if (lane == 0) {
+                visited[v] |= w;
+                nbits += __popcll(w);
+                ++nv;
+            }
+        }
+    }
+    if (kStage == 1 && lane == 0 && nbits) {
+        atomicAdd((unsigned long long *)&cnt[12 + kNewBits], (unsigned long long)nbits);
+        atomicAdd((unsigned long long *)&cnt[12 + kVisitedVerts], (unsigned long long)nv);
+    }
+}
+
 __global__ void k_level_begin(int64_t *cnt)
 {
-    for (int i = 0; i < 9; ++i) cnt[i] = 0;
+    for (int i = 0; i < kDepth; ++i) cnt[i] = 0;
+    cnt[kActVerts] = 0;
+    cnt[kActEdges] = 0;
+    cnt[kNq] = 0;
+}
+
+// active[w] bit i = (next[64w + i] != 0): one wavefront per 64-vertex word
+__global__ __launch_bounds__(256) void k_build_active(const uint64_t *__restrict__ next, int64_t nwords,
+                                                      const int64_t *__restrict__ rowptr,
+                                                      uint64_t *__restrict__ active, int64_t *__restrict__ cnt)
+{
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    int64_t nv = 0, ne = 0;
+    for (int64_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += waves) {
+        const int64_t v = w * 64 + __lane_id();
+        const bool on = next[v] != 0;
+        const uint64_t bits = __ballot(on);
+        if (__lane_id() == 0) active[w] = bits;
+        if (on) {
+            ++nv;
+            ne += rowptr[v + 1] - rowptr[v];
+        }
+    }
+    nv = g5_wave_sum(nv);
+    ne = g5_wave_sum(ne);
+    if (__lane_id() == 0 && nv) {
+        atomicAdd((unsigned long long *)&cnt[kActVerts], (unsigned long long)nv);
+        atomicAdd((unsigned long long *)&cnt[kActEdges], (unsigned long long)ne);
+    }
 }
 
 __global__ void k_level_end(int64_t *cnt) { cnt[kDepth] += 1; }
 
 __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__ fa, uint64_t *__restrict__ fb,
-                          int64_t n, uint64_t unused_lanes)
+                          uint64_t *__restrict__ act, int64_t n, uint64_t unused_lanes)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         visited[i] = unused_lanes;
         fa[i] = 0;
         fb[i] = 0;
+        if (i < 2 * (n / 64)) act[i] = 0;
     }
 }
 
-__global__ void k_ms_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
-{
-    const int64_t n4 = n / 4;
-    int4 *p4 = reinterpret_cast<int4 *>(p);
-    const int4 q = make_int4(v, v, v, v);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p4[i] = q;
-}
-
 // Roots: one thread; lanes of duplicate roots share the vertex.
-__global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, uint64_t *__restrict__ visited,
-                           uint64_t *__restrict__ frontier, int32_t *__restrict__ parent,
-                           int32_t *__restrict__ level, int64_t *__restrict__ cnt)
+__global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
+                           uint64_t *__restrict__ visited, uint64_t *__restrict__ frontier,
+                           uint64_t *__restrict__ active, int32_t *__restrict__ parent,
+                           uint16_t *__restrict__ level, int64_t *__restrict__ cnt)
 {
     for (int r = 0; r < nroots; ++r) {
-        const int64_t v = roots[r];
+        const int64_t v = old2new[roots[r]];
         visited[v] |= 1ull << r;
         frontier[v] |= 1ull << r;
-        parent[v * 64 + r] = (int32_t)v;
+        active[v >> 6] |= 1ull << (v & 63);
+        parent[v * 64 + r] = (int32_t)roots[r];
         level[v * 64 + r] = 0;
     }
     cnt[kDepth] = 1;
 }
 
-// Classification of vertices with nonzero degree into three bitmaps.
-__global__ void k_classify(const int64_t *__restrict__ rowptr, int64_t nwords, uint64_t *__restrict__ bs,
-                           uint64_t *__restrict__ bm, uint64_t *__restrict__ bl)
+// Degree-order keys: (2^32-1 - degree) << 32 | id, so an ascending sort
+// gives descending degree, ties by id; also counts the three degree bins.
+__global__ void k_degree_keys(const int64_t *__restrict__ rowptr, int64_t n, uint64_t *__restrict__ keys,
+                              unsigned long long *__restrict__ bins)
 {
-    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < nwords;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t s = 0, m = 0, l = 0;
-        for (int i = 0; i < 64; ++i) {
-            const int64_t v = w * 64 + i;
-            const int64_t d = rowptr[v + 1] - rowptr[v];
-            if (d <= 0) continue;
-            if (d <= kSmallMax) s |= 1ull << i;
-            else if (d <= kMediumMax) m |= 1ull << i;
-            else l |= 1ull << i;
-        }
-        bs[w] = s;
-        bm[w] = m;
-        bl[w] = l;
+    unsigned long long c[3] = {0, 0, 0};
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = rowptr[v + 1] - rowptr[v];
+        keys[v] = ((uint64_t)(0xFFFFFFFFu - (uint32_t)d) << 32) | (uint64_t)v;
+        if (d > kMediumMax) ++c[0];
+        else if (d > kSmallMax) ++c[1];
+        else if (d > 0) ++c[2];
+    }
+    for (int k = 0; k < 3; ++k) {
+        const unsigned long long t = (unsigned long long)g5_wave_sum((int64_t)c[k]);
+        if (__lane_id() == 0 && t) atomicAdd(&bins[k], t);
     }
 }
 
-// per-lane extraction: pred/level of root `lane` as int64/int32[n]
-__global__ void k_ms_extract(const int32_t *__restrict__ parent, const int32_t *__restrict__ level, int64_t n,
+__global__ void k_perm(const uint64_t *__restrict__ keys, int64_t n, uint32_t *__restrict__ new2old,
+                       uint32_t *__restrict__ old2new, const int64_t *__restrict__ rowptr,
+                       const uint32_t *__restrict__ gdeg, int64_t *__restrict__ ndeg, uint32_t *__restrict__ ngdeg)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = (uint32_t)keys[i];
+        new2old[i] = o;
+        old2new[o] = (uint32_t)i;
+        ndeg[i] = rowptr[o + 1] - rowptr[o];
+        ngdeg[i] = gdeg[o];
+    }
+}
+
+// one wavefront per relabelled row: copy the original row (ascending
+// original ids) mapped to new ids
+__global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ rowptr,
+                                                   const uint32_t *__restrict__ cols,
+                                                   const uint32_t *__restrict__ new2old,
+                                                   const uint32_t *__restrict__ old2new, int64_t n,
+                                                   const int64_t *__restrict__ nrowptr, uint32_t *__restrict__ ncols)
+{
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t o = new2old[i];
+        const int64_t b = rowptr[o], d = rowptr[o + 1] - b, dst = nrowptr[i];
+        for (int64_t j = __lane_id(); j < d; j += 64) ncols[dst + j] = old2new[cols[b + j]];
+    }
+}
+
+// per-lane extraction: pred/level of root `lane` as int64/int32[n]; a lane's
+// entries are meaningful only where its visited bit is set
+__global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const int32_t *__restrict__ parent,
+                             const uint16_t *__restrict__ level, const uint64_t *__restrict__ visited, int64_t n,
                              int lane, int64_t *__restrict__ pred, int32_t *__restrict__ lev)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t l = level[i * 64 + lane];
-        lev[i] = l;
-        pred[i] = l < 0 ? -1 : (int64_t)parent[i * 64 + lane];
+        const int64_t v = old2new[i];
+        const bool reached = visited[v] >> lane & 1;
+        lev[i] = reached ? (int32_t)level[v * 64 + lane] : -1;
+        pred[i] = reached ? (int64_t)parent[v * 64 + lane] : -1;
     }
 }
 
 // nedge per lane: sum of generated degrees over reached vertices
-__global__ __launch_bounds__(256) void k_ms_nedge(const int32_t *__restrict__ level, const uint32_t *__restrict__ gdeg,
+__global__ __launch_bounds__(256) void k_ms_nedge(const uint64_t *__restrict__ visited, const uint32_t *__restrict__ gdeg,
                                                   int64_t n, unsigned long long *__restrict__ out)
 {
     // thread t of a wave handles lane t for a strip of vertices
@@ -356,47 +588,89 @@ __global__ __launch_bounds__(256) void k_ms_nedge(const int32_t *__restrict__ le
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     for (int64_t v = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6); v < n; v += waves) {
         const uint32_t d = gdeg[v];
-        if (d && level[v * 64 + lane] >= 0) acc += d;
+        if (d && (visited[v] >> lane & 1)) acc += d;
     }
     atomicAdd(&out[lane], acc);
 }
 
 // prof: events between the three pull kernels (the uncaptured path)
-int level_launches(g5_msbfs *m, hipStream_t s, const uint64_t *front, uint64_t *next, bool prof = false)
+int level_launches(g5_msbfs *m, hipStream_t s, int p, bool prof = false)
 {
-    g5_graph *g = m->g;
+    const Relabeled &R = m->rl;
+    const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
+    uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
+    const int64_t nw = m->n / 64;
+    const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     hipLaunchKernelGGL(k_level_begin, 1, 1, 0, s, m->d_cnt);
     if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
-    if (m->lists.n_large)
-        hipLaunchKernelGGL(k_pull_block, (unsigned)std::min<int64_t>(m->lists.n_large, 4096), kBlock, 0, s,
-                           (const uint32_t *)m->lists.large, m->lists.n_large, (const int64_t *)g->d_rowptr,
-                           (const uint32_t *)g->d_cols, m->d_visited, front, next, m->d_parent, m->d_level,
+    if (R.n_large)
+        hipLaunchKernelGGL(k_pull_block, (unsigned)std::min<int64_t>(R.n_large, 4096), kBlock, 0, s, (int64_t)0,
+                           R.n_large, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
                            m->d_cnt);
     if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
-    if (m->lists.n_medium)
-        hipLaunchKernelGGL(k_pull_wave, g5::grid_for((m->lists.n_medium + 3) / 4, 1, 16384), kBlock, 0, s,
-                           (const uint32_t *)m->lists.medium, m->lists.n_medium, (const int64_t *)g->d_rowptr,
-                           (const uint32_t *)g->d_cols, m->d_visited, front, next, m->d_parent, m->d_level,
+    if (R.n_medium)
+        hipLaunchKernelGGL(k_pull_wave, g5::grid_for((R.n_medium + 3) / 4, 1, 16384), kBlock, 0, s, R.n_large,
+                           R.n_medium, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
                            m->d_cnt);
     if (prof) G5_HIP(hipEventRecord(m->ek[2], s));
-    if (m->lists.n_small)
-        hipLaunchKernelGGL(k_pull_thread, g5::grid_for(m->lists.n_small, kBlock, 16384), kBlock, 0, s,
-                           (const uint32_t *)m->lists.small, m->lists.n_small, (const int64_t *)g->d_rowptr,
-                           (const uint32_t *)g->d_cols, m->d_visited, front, next, m->d_parent, m->d_level,
+    if (R.n_small)
+        hipLaunchKernelGGL(k_pull_thread, g5::grid_for(R.n_small, kBlock, 16384), kBlock, 0, s,
+                           R.n_large + R.n_medium, R.n_small, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
                            m->d_cnt);
     if (prof) G5_HIP(hipEventRecord(m->ek[3], s));
+    hipLaunchKernelGGL(k_build_active, g5::grid_for((nw + 3) / 4, 1, 16384), 256, 0, s, (const uint64_t *)next, nw,
+                       (const int64_t *)R.rowptr, act_next, m->d_cnt);
     hipLaunchKernelGGL(k_level_end, 1, 1, 0, s, m->d_cnt);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
+}
+
+// One push level on parity p: direct launches (the frontier queue's length
+// sizes the degree scan on the host).  fqn = active vertices of the frontier.
+int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
+{
+    const Relabeled &R = m->rl;
+    const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
+    uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
+    const int64_t nw = m->n / 64;
+    const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
+    hipLaunchKernelGGL(k_level_begin, 1, 1, 0, s, m->d_cnt);
+    G5_HIP(g5::fill_u64(next, m->n, 0, s));
+    if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
+    G5_HIP(g5::bitmap_to_queue(act, nw, m->d_fq, m->d_cnt + kFqLen, m->d_scratch, s));
+    hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fqn, 256, 8192), 256, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff);
+    G5_HIP(g5::exclusive_scan_i64(m->d_qoff, fqn, m->d_cnt + kFqEdges, m->d_scratch, s));
+    hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       m->d_nq, m->d_cnt);
+    hipLaunchKernelGGL(k_push_vertices<0>, 2048, kBlock, 0, s, (const uint32_t *)m->d_nq, next, m->d_visited,
+                       m->d_parent, m->d_level, m->d_cnt);
+    hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       m->d_nq, m->d_cnt);
+    hipLaunchKernelGGL(k_push_vertices<1>, 2048, kBlock, 0, s, (const uint32_t *)m->d_nq, next, m->d_visited,
+                       m->d_parent, m->d_level, m->d_cnt);
+    if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
+    hipLaunchKernelGGL(k_build_active, g5::grid_for((nw + 3) / 4, 1, 16384), 256, 0, s, (const uint64_t *)next, nw,
+                       (const int64_t *)R.rowptr, act_next, m->d_cnt);
+    hipLaunchKernelGGL(k_level_end, 1, 1, 0, s, m->d_cnt);
+    G5_HIP(hipGetLastError());
+    return G5_OK;
 }
 
 int capture_graphs(g5_msbfs *m)
 {
     for (int p = 0; p < 2; ++p) {
-        const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
-        uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
         hipGraph_t gr = nullptr;
         G5_HIP(hipStreamBeginCapture(m->s, hipStreamCaptureModeThreadLocal));
-        int rc = level_launches(m, m->s, front, next);
+        int rc = level_launches(m, m->s, p);
         hipError_t e = hipStreamEndCapture(m->s, &gr);
         if (rc) return rc;
         G5_HIP(e);
@@ -407,40 +681,59 @@ int capture_graphs(g5_msbfs *m)
     return G5_OK;
 }
 
-int build_lists(g5_msbfs *m)
+// The degree-ordered copy of the block (see the file comment).
+int build_relabel(g5_msbfs *m)
 {
     g5_graph *g = m->g;
     hipStream_t s = m->s;
-    const int64_t nw = m->n / 64;
-    uint64_t *bits = nullptr;
-    G5_HIP(hipMalloc(&bits, (size_t)std::max<int64_t>(nw, 1) * 8 * 3));
-    hipLaunchKernelGGL(k_classify, g5::grid_for(nw, 256, 8192), 256, 0, s, (const int64_t *)g->d_rowptr, nw,
-                       bits, bits + nw, bits + 2 * nw);
-    uint32_t **dst[3] = {&m->lists.small, &m->lists.medium, &m->lists.large};
-    int64_t *cntp[3] = {&m->lists.n_small, &m->lists.n_medium, &m->lists.n_large};
-    for (int c = 0; c < 3; ++c) {
-        uint32_t *tmp = nullptr;
-        hipError_t e = hipMalloc(&tmp, (size_t)std::max<int64_t>(m->n, 1) * 4);
-        if (e == hipSuccess) e = g5::bitmap_to_queue(bits + c * nw, nw, tmp, m->d_cnt, m->d_scratch, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(m->h_cnt, m->d_cnt, 8, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            (void)hipFree(tmp);
-            (void)hipFree(bits);
-            return g5::fail_hip(e, "msbfs lists", __FILE__, __LINE__);
-        }
-        *cntp[c] = m->h_cnt[0];
-        e = hipMalloc(dst[c], (size_t)std::max<int64_t>(*cntp[c], 1) * 4);
-        if (e == hipSuccess && *cntp[c])
-            e = hipMemcpyAsync(*dst[c], tmp, (size_t)*cntp[c] * 4, hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) {
-            (void)hipFree(bits);
-            return g5::fail_hip(e, "msbfs lists", __FILE__, __LINE__);
-        }
+    Relabeled &R = m->rl;
+    const int64_t n = m->n;
+    uint64_t *keys = nullptr, *tmp = nullptr;
+    unsigned long long *bins = nullptr;
+    int64_t *ndeg = nullptr;
+    auto cleanup = [&]() {
+        if (keys) (void)hipFree(keys);
+        if (tmp) (void)hipFree(tmp);
+        if (bins) (void)hipFree(bins);
+        if (ndeg) (void)hipFree(ndeg);
+    };
+    hipError_t e = hipMalloc(&keys, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc(&tmp, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc(&bins, 3 * 8);
+    if (e == hipSuccess) e = hipMalloc(&ndeg, (size_t)(n + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&R.new2old, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&R.old2new, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&R.gdeg, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
+    if (e == hipSuccess) e = hipMemsetAsync(bins, 0, 3 * 8, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_degree_keys, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)g->d_rowptr, n,
+                           keys, bins);
+        e = hipGetLastError();
     }
-    (void)hipFree(bits);
+    if (e == hipSuccess) e = g5::sort_u64(keys, tmp, n, 64, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_perm, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint64_t *)keys, n, R.new2old,
+                           R.old2new, (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_gdeg, ndeg, R.gdeg);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = g5::exclusive_scan_i64(ndeg, n, ndeg + n, m->d_scratch, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(R.rowptr, ndeg, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_perm_rows, g5::grid_for((n + 3) / 4, 1, 65536), 256, 0, s,
+                           (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_cols, (const uint32_t *)R.new2old,
+                           (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols);
+        e = hipGetLastError();
+    }
+    unsigned long long hb[3] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(hb, bins, 3 * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) return g5::fail_hip(e, "msbfs relabel", __FILE__, __LINE__);
+    R.n_large = (int64_t)hb[0];
+    R.n_medium = (int64_t)hb[1];
+    R.n_small = (int64_t)hb[2];
     return G5_OK;
 }
 
@@ -465,17 +758,21 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_visited, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_fa, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_fb, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_act, (size_t)(n / 64) * 2 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_fq, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&m->d_nq, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&m->d_qoff, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
-    if (e == hipSuccess) e = hipMalloc(&m->d_level, (size_t)n * 64 * 4);
+    if (e == hipSuccess) e = hipMalloc(&m->d_level, (size_t)n * 64 * 2);
     if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kMsN * 8);
     if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kMsN * 8, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n / 64 + 1) + 16) * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_msbfs_create allocation", __FILE__, __LINE__));
-    int rc = build_lists(m);
+    int rc = build_relabel(m);
     if (rc) return fail(rc);
     rc = capture_graphs(m);
     if (rc) return fail(rc);
@@ -489,8 +786,9 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     if (m->s) (void)hipStreamSynchronize(m->s);
     for (int p = 0; p < 2; ++p)
         if (m->graph[p]) (void)hipGraphExecDestroy(m->graph[p]);
-    void *dev[] = {m->d_visited, m->d_fa,        m->d_fb,           m->d_parent,       m->d_level,
-                   m->d_cnt,     m->d_scratch,   m->d_roots,        m->lists.small,    m->lists.medium,   m->lists.large};
+    void *dev[] = {m->d_fq, m->d_nq, m->d_qoff, m->d_act, m->d_visited, m->d_fa,        m->d_fb,           m->d_parent,       m->d_level,
+                   m->d_cnt,     m->d_scratch,   m->d_roots,        m->rl.new2old,     m->rl.old2new,     m->rl.rowptr,
+                   m->rl.cols,   m->rl.gdeg};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
@@ -517,16 +815,29 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     G5_HIP(hipEventRecord(m->ev0, s));
     const uint64_t unused = nroots == 64 ? 0ull : ~((1ull << nroots) - 1);
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
-                       m->n, unused);
-    hipLaunchKernelGGL(k_ms_fill_i32, g5::grid_for(m->n * 16, 256, 16384), 256, 0, s, m->d_level, m->n * 64,
-                       (int32_t)-1);
-    hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, m->d_visited, m->d_fa,
-                       m->d_parent, m->d_level, m->d_cnt);
+                       m->d_act, m->n, unused);
+    hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
+                       m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_level, m->d_cnt);
     G5_HIP(hipGetLastError());
     int p = 0, levels = 1;
+    m->lvl.clear();
+    // the first frontier is the set of distinct roots; its edge count is not
+    // known on the host, and a handful of roots always takes the push path
+    int64_t act_verts = 0, act_edges = -1;
+    {
+        int64_t tmp[64];
+        std::copy(roots, roots + nroots, tmp);
+        std::sort(tmp, tmp + nroots);
+        act_verts = std::unique(tmp, tmp + nroots) - tmp;
+    }
+    const int64_t nnz = m->g->nnz;
     for (;;) {
-        if (m->prof) {
-            int rc = level_launches(m, s, p == 0 ? m->d_fa : m->d_fb, p == 0 ? m->d_fb : m->d_fa, true);
+        const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * kPushRatio < nnz));
+        if (push) {
+            int rc = push_level(m, s, p, act_verts, m->prof);
+            if (rc) return rc;
+        } else if (m->prof) {
+            int rc = level_launches(m, s, p, true);
             if (rc) return rc;
         } else {
             G5_HIP(hipGraphLaunch(m->graph[p], s));
@@ -534,26 +845,55 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kMsN * 8, hipMemcpyDeviceToHost, s));
         G5_HIP(hipStreamSynchronize(s));
         int64_t new_bits = 0;
-        for (int c = 0; c < 3; ++c) new_bits += m->h_cnt[3 * c + kNewBits];
+        for (int c = 0; c < 4; ++c) new_bits += m->h_cnt[4 * c + kNewBits];
         if (m->prof) {
-            // per class c (events ek[2-c] .. ek[3-c]): 16 B row offsets + 8 B
-            // visited read + 8 B next write per vertex with an unvisited lane,
-            // 4 B column + 8 B frontier word per edge read, 4 B parent + 4 B
-            // level per new (vertex, root) pair
-            const int order[3] = {2, 1, 0};  // launch order: large, medium, small
-            for (int k = 0; k < 3; ++k) {
-                const int c = order[k];
-                const int64_t nl = c == 0 ? m->lists.n_small : c == 1 ? m->lists.n_medium : m->lists.n_large;
-                if (!nl) continue;
+            double rec[kLevelRec] = {0};
+            rec[8] = (double)new_bits;
+            rec[9] = push ? 1.0 : 0.0;
+            rec[10] = (double)act_verts;
+            rec[11] = (double)act_edges;
+            if (push) {
+                // 16 B row offsets per frontier vertex; per frontier edge and
+                // pass (two passes) 4 B column + 8 B frontier word + 8 B
+                // visited word + 8 B next word; 16 B visited read-modify-write
+                // per newly reached vertex; 4 B parent init + 4 B parent + 4 B
+                // level per new (vertex, root) pair
                 float kms = 0;
-                G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
-                m->k_ms[c] += kms;
-                m->k_launches[c] += 1;
-                m->k_bytes[c] += 32.0 * m->h_cnt[3 * c + kVisitedVerts] + 12.0 * m->h_cnt[3 * c + kEdgesRead] +
-                                 8.0 * m->h_cnt[3 * c + kNewBits];
+                G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
+                const double E = (double)m->h_cnt[kFqEdges];
+                m->k_ms[3] += kms;
+                m->k_launches[3] += 1;
+                m->k_bytes[3] += 16.0 * m->h_cnt[kFqLen] + 56.0 * E + 16.0 * m->h_cnt[12 + kVisitedVerts] +
+                                 12.0 * m->h_cnt[12 + kNewBits];
+                rec[3] = kms;
+                rec[7] = E;
+            } else {
+                // per class c (events ek[2-c] .. ek[3-c]): 16 B row offsets + 8 B
+                // visited read + 8 B next write per vertex with an unvisited
+                // lane, 4 B column + 1 bit of the active bitmap per edge read,
+                // 8 B per frontier word gathered, 4 B parent + 4 B level per
+                // new (vertex, root) pair
+                const int order[3] = {2, 1, 0};  // launch order: large, medium, small
+                for (int k = 0; k < 3; ++k) {
+                    const int c = order[k];
+                    const int64_t nl = c == 0 ? m->rl.n_small : c == 1 ? m->rl.n_medium : m->rl.n_large;
+                    if (!nl) continue;
+                    float kms = 0;
+                    G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
+                    m->k_ms[c] += kms;
+                    m->k_launches[c] += 1;
+                    m->k_bytes[c] += 32.0 * m->h_cnt[4 * c + kVisitedVerts] + 4.125 * m->h_cnt[4 * c + kEdgesRead] +
+                                     8.0 * m->h_cnt[4 * c + kGathers] + 8.0 * m->h_cnt[4 * c + kNewBits];
+                    rec[c] = kms;
+                    rec[4 + c] = (double)m->h_cnt[4 * c + kEdgesRead];
+                }
             }
+            m->lvl.insert(m->lvl.end(), rec, rec + kLevelRec);
         }
         if (new_bits == 0) break;
+        G5_REQUIRE(levels < 65535, G5_E_STATE, "g5_msbfs_run: more than 65534 levels (uint16 depth limit)");
+        act_verts = m->h_cnt[kActVerts];
+        act_edges = m->h_cnt[kActEdges];
         ++levels;
         p ^= 1;
     }
@@ -573,7 +913,8 @@ extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *le
     hipError_t e = hipMalloc(&dl, (size_t)m->n * 4);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s,
-                           (const int32_t *)m->d_parent, (const int32_t *)m->d_level, m->n, lane, dp, dl);
+                           (const uint32_t *)m->rl.old2new, (const int32_t *)m->d_parent,
+                           (const uint16_t *)m->d_level, (const uint64_t *)m->d_visited, m->n, lane, dp, dl);
         e = hipGetLastError();
     }
     if (e == hipSuccess && pred) e = hipMemcpyAsync(pred, dp, (size_t)m->n * 8, hipMemcpyDeviceToHost, m->s);
@@ -593,7 +934,7 @@ extern "C" int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge)
     hipError_t e = hipMemsetAsync(d, 0, 64 * 8, m->s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_ms_nedge, g5::grid_for(m->n / 4 + 1, 1, 8192), 256, 0, m->s,
-                           (const int32_t *)m->d_level, (const uint32_t *)m->g->d_gdeg, m->n, d);
+                           (const uint64_t *)m->d_visited, (const uint32_t *)m->rl.gdeg, m->n, d);
         e = hipGetLastError();
     }
     unsigned long long h[64];
@@ -609,16 +950,23 @@ extern "C" int g5_msbfs_profile(g5_msbfs *m, int enable)
 {
     G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_profile: null handle");
     m->prof = enable != 0;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 4; ++c) {
         m->k_ms[c] = m->k_bytes[c] = 0;
         m->k_launches[c] = 0;
     }
     return G5_OK;
 }
 
+extern "C" int g5_msbfs_set_mode(g5_msbfs *m, int mode)
+{
+    G5_REQUIRE(m && mode >= 0 && mode <= 2, G5_E_ARG, "g5_msbfs_set_mode: mode must be 0, 1 or 2");
+    m->mode = mode;
+    return G5_OK;
+}
+
 extern "C" int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *launches, double *bytes)
 {
-    G5_REQUIRE(m && kernel >= 0 && kernel < 3, G5_E_ARG, "g5_msbfs_stats: bad arguments");
+    G5_REQUIRE(m && kernel >= 0 && kernel < 4, G5_E_ARG, "g5_msbfs_stats: bad arguments");
     if (ms) *ms = m->k_ms[kernel];
     if (launches) *launches = m->k_launches[kernel];
     if (bytes) *bytes = m->k_bytes[kernel];
@@ -628,8 +976,18 @@ extern "C" int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *laun
 extern "C" int g5_msbfs_info(g5_msbfs *m, int64_t *n_small, int64_t *n_medium, int64_t *n_large)
 {
     G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_info: null handle");
-    if (n_small) *n_small = m->lists.n_small;
-    if (n_medium) *n_medium = m->lists.n_medium;
-    if (n_large) *n_large = m->lists.n_large;
+    if (n_small) *n_small = m->rl.n_small;
+    if (n_medium) *n_medium = m->rl.n_medium;
+    if (n_large) *n_large = m->rl.n_large;
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev)
+{
+    G5_REQUIRE(m && nlev, G5_E_ARG, "g5_msbfs_levels: null argument");
+    const int n = (int)(m->lvl.size() / kLevelRec);
+    *nlev = n;
+    if (out)
+        for (int i = 0; i < std::min(n, cap) * kLevelRec; ++i) out[i] = m->lvl[i];
     return G5_OK;
 }

@@ -129,11 +129,20 @@ int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32_t *nlevels
 int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level);
 /* TEPS numerators nedge[nroots] of the last run (validate/validate.h:439-485). */
 int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
-/* Per-kernel accounting (kernel 0: thread-per-vertex, 1: wave-per-vertex,
- * 2: workgroup-per-vertex pull): enabling runs levels as direct launches
- * timed with events (otherwise each level replays a captured hipGraph). */
+/* Direction per level: 0 (default) push while the frontier's edges are
+ * fewer than nnz/16, pull otherwise; 1 pull only; 2 push only.  Results are
+ * identical in every mode. */
+int g5_msbfs_set_mode(g5_msbfs *m, int mode);
+/* Per-kernel accounting (kernel 0: thread-per-vertex pull, 1: wave-per-vertex
+ * pull, 2: workgroup-per-vertex pull, 3: push level): enabling runs pull
+ * levels as direct launches timed with events (otherwise each pull level
+ * replays a captured hipGraph). */
 int g5_msbfs_profile(g5_msbfs *m, int enable);
 int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *launches, double *bytes);
+/* Per level of the last profiled run, 12 doubles each: kernel ms (0..3 as in
+ * g5_msbfs_stats), edges read (0..3), new (vertex, root) pairs, 1 if the
+ * level pushed, the frontier's vertices and edges (-1 = not counted).  *nlev = levels recorded; at most cap are copied. */
+int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev);
 /* Sizes of the three degree bins (<= 32, <= 2048, > 2048 neighbours). */
 int g5_msbfs_info(g5_msbfs *m, int64_t *n_small, int64_t *n_medium, int64_t *n_large);
 

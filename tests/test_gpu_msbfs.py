@@ -20,11 +20,12 @@ def kron(scale):
     return _CACHE[scale]
 
 
-@pytest.mark.parametrize("scale", [10, 14, 16])
-def test_64_lanes_bit_exact(scale):
+@pytest.mark.parametrize("scale,mode", [(10, 0), (14, 0), (16, 0), (14, 1), (14, 2), (16, 2)])
+def test_64_lanes_bit_exact(scale, mode):
     e, ref, g = kron(scale)
     roots = O.select_roots(ref, 1 << scale)
     m = g5.MSBFS(g)
+    m.set_mode(mode)
     nl, ms = m.run(roots)
     ne = m.nedge()
     maxl = 0
@@ -38,10 +39,12 @@ def test_64_lanes_bit_exact(scale):
     assert nl == maxl
 
 
-def test_fewer_lanes_and_repeat_runs():
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_fewer_lanes_and_repeat_runs(mode):
     e, ref, g = kron(12)
     roots = O.select_roots(ref, 1 << 12)
     m = g5.MSBFS(g)
+    m.set_mode(mode)
     for k in (1, 7, 33):
         sub = roots[-k:]
         m.run(sub)
@@ -92,11 +95,13 @@ def test_tiny_graphs_all_roots_in_one_batch():
         if not roots:
             continue
         ms = g5.MSBFS(g)
-        ms.run(roots)
-        for lane, r in enumerate(roots):
-            pred, lev = ms.result(lane)
-            opred, olev, _ = O.bfs_bin(ref, r)
-            assert np.array_equal(lev, olev) and np.array_equal(pred, opred), (trial, r)
+        for mode in (0, 1, 2):
+            ms.set_mode(mode)
+            ms.run(roots)
+            for lane, r in enumerate(roots):
+                pred, lev = ms.result(lane)
+                opred, olev, _ = O.bfs_bin(ref, r)
+                assert np.array_equal(lev, olev) and np.array_equal(pred, opred), (trial, mode, r)
         ms.free()
         g.free()
 
