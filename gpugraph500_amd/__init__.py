@@ -77,7 +77,7 @@ def lib():
     _sig(L, "g5_msbfs_nedge", I, [P, P])
     _sig(L, "g5_msbfs_profile", I, [P, I])
     _sig(L, "g5_msbfs_stats", I, [P, I, P, P, P])
-    _sig(L, "g5_msbfs_info", I, [P, P, P, P])
+    _sig(L, "g5_msbfs_info", I, [P, P, I])
     _sig(L, "g5_msbfs_levels", I, [P, P, I, ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_set_mode", I, [P, I])
     _sig(L, "g5_s4_max_words", c_sz, [c_sz])
@@ -279,7 +279,7 @@ class BFS:
 class MSBFS:
     """Batched multi-source BFS over up to 64 roots (uint64 lane masks)."""
 
-    KERNELS = ("k_pull_thread", "k_pull_wave", "k_pull_block", "push")
+    KERNELS = ("k_pull_block", "k_pull_group<64>", "k_pull_thread", "push")
 
     def __init__(self, graph):
         self.graph = graph
@@ -330,9 +330,9 @@ class MSBFS:
         check(lib().g5_msbfs_set_mode(self.h, mode))
 
     def bins(self):
-        a, b, c = c_i64(), c_i64(), c_i64()
-        check(lib().g5_msbfs_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        out = np.zeros(3, np.int64)
+        check(lib().g5_msbfs_info(self.h, _p(out), 3))
+        return out.tolist()
 
     def free(self):
         if self.h:

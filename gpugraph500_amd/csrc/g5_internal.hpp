@@ -15,7 +15,7 @@
 #include <cstdio>
 #include <string>
 
-#include "../../include/g5hip.h"
+#include "g5hip.h"
 
 namespace g5 {
 

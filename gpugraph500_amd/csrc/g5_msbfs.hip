@@ -42,18 +42,23 @@
 
 namespace {
 
-constexpr int kSmallMax = 32;
 constexpr int kMediumMax = 2048;
+// degree bins in launch order: one workgroup per vertex (> 2048), one
+// wavefront per vertex (33..2048), one thread per vertex (1..32)
+constexpr int kBins = 3;
+constexpr int kBinMaxDeg[kBins] = {1 << 30, 2048, 32};
+constexpr int kPushUnit = kBins;  // accounting unit of push levels
+constexpr int kUnits = kBins + 1;
 constexpr int kBlock = 256;
-constexpr int kLevelRec = 12;
+constexpr int kLevelRec = 12;  // per level: ms[4], edges[4], new pairs, pushed, frontier vertices, edges
 // push when the frontier's edges are fewer than nnz / kPushRatio
 constexpr int64_t kPushRatio = 16;
 
-// counters: per kernel class c (0 thread pull, 1 wave pull, 2 block pull,
-// 3 push) slots 4c+0 new (vertex, root) pairs, 4c+1 edges read, 4c+2
-// vertices worked on, 4c+3 frontier words gathered; then the level number,
-// the next frontier's active vertices and their edges, the push queue
-// length, and the push frontier queue length / edge total.
+// counters: per accounting unit b (the kBins pull bins, then kPushUnit)
+// slots 4b+0 new (vertex, root) pairs, 4b+1 edges read, 4b+2 vertices
+// worked on, 4b+3 frontier words gathered; then the level number, the next
+// frontier's active vertices and their edges, the push queue length, and the
+// push frontier queue length / edge total.
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3,
     kDepth = 16, kActVerts = 17, kActEdges = 18, kNq = 19, kFqLen = 20, kFqEdges = 21, kMsN = 24
@@ -65,7 +70,7 @@ struct Relabeled {
     int64_t *rowptr = nullptr;
     uint32_t *cols = nullptr;  // new ids, rows in ascending original-id order
     uint32_t *gdeg = nullptr;
-    int64_t n_large = 0, n_medium = 0, n_small = 0;  // bins: [0,L) [L,L+M) [L+M,L+M+S)
+    int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
 };
 
 }  // namespace
@@ -90,9 +95,9 @@ struct g5_msbfs {
     int64_t roots[64];
     // accounting
     bool prof = false;
-    hipEvent_t ek[4] = {nullptr, nullptr, nullptr, nullptr};
-    double k_ms[4] = {0, 0, 0, 0}, k_bytes[4] = {0, 0, 0, 0};
-    int64_t k_launches[4] = {0, 0, 0, 0};
+    hipEvent_t ek[kBins + 1] = {};
+    double k_ms[kUnits] = {}, k_bytes[kUnits] = {};
+    int64_t k_launches[kUnits] = {};
     // per level of the last profiled run (kLevelRec doubles each)
     std::vector<double> lvl;
     int mode = 0;  // 0 direction chosen per level, 1 pull only, 2 push only
@@ -123,42 +128,15 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
     return __lane_id() == 0 ? 0ull : prev;
 }
 
-// One 64-neighbour step of vertex v's scan, neighbours [base, base+64) of
-// [.., e): lanes gather the frontier words of active neighbours restricted to
-// `want`; the lane holding the first occurrence of a root's bit writes that
-// root's parent.  Returns the new bits (wave-uniform); found is wave-uniform.
-__device__ __forceinline__ uint64_t wave_step(int64_t base, int64_t e, uint64_t want, uint64_t found, int64_t v,
-                                              const uint32_t *__restrict__ cols,
-                                              const uint64_t *__restrict__ frontier,
-                                              const uint64_t *__restrict__ active,
-                                              const uint32_t *__restrict__ new2old, int32_t *__restrict__ parent,
-                                              int64_t &ngath)
-{
-    const int64_t j = base + __lane_id();
-    uint32_t u = 0;
-    uint64_t fresh = 0;
-    if (j < e) {
-        u = cols[j];
-        if (active[u >> 6] >> (u & 63) & 1) {
-            fresh = frontier[u] & want & ~found;
-            ++ngath;
-        }
-    }
-    if (__ballot(fresh != 0) == 0) return 0;
-    const uint64_t first = fresh & ~lanes_excl_or(fresh);
-    if (first) write_parents(parent, v, first, new2old[u]);
-    return g5_wave_or(fresh);
-}
-
-// One thread per vertex (degree <= kSmallMax).
+// One thread per vertex (degree <= 32): a sequential ascending scan.
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
 {
     const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 0;
+    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
     int64_t ngath = 0;
     int64_t nbits = 0, nedges = 0, nverts = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nlist;
@@ -213,48 +191,83 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     }
 }
 
-// One wavefront per vertex (kSmallMax < degree <= kMediumMax).
-__global__ __launch_bounds__(kBlock) void k_pull_wave(
+// G lanes per vertex (used with G = 64 for degrees 33..2048): a step loads
+// G neighbours at once; the group's exclusive prefix-OR gives each root's
+// first hit, the group's inclusive OR the new bits.  Control flow is uniform
+// within a group (v, found and the step count are the group's), so the
+// width-G shuffles only read lanes of the same group.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_pull_group(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
 {
+    constexpr int kGroups = kBlock / G;
     const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 4;
-    int64_t ngath = 0;
-    const int lane = __lane_id();
-    const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-    int64_t nbits = 0, nedges = 0, nverts = 0;
-    for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += waves) {
+    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
+    const int gl = threadIdx.x & (G - 1);
+    const int gbase = (int)(__lane_id() & ~(G - 1));
+    const int64_t stride = (int64_t)gridDim.x * kGroups;
+    int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kGroups + threadIdx.x / G; i < nlist; i += stride) {
         const uint32_t v = (uint32_t)(first_v + i);
         const uint64_t vis = visited[v];
         if (vis == ~0ull) {
-            if (lane == 0) next[v] = 0;
+            if (gl == 0) next[v] = 0;
             continue;
         }
         const uint64_t unv = ~vis;
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
-        for (; base < e; base += 64) {
-            found |= wave_step(base, e, unv, found, v, cols, frontier, active, new2old, parent, ngath);
+        for (; base < e; base += G) {
+            const int64_t j = base + gl;
+            uint32_t u = 0;
+            uint64_t fresh = 0;
+            if (j < e) {
+                u = cols[j];
+                if (active[u >> 6] >> (u & 63) & 1) {
+                    fresh = frontier[u] & unv & ~found;
+                    ++ngath;
+                }
+            }
+            const uint64_t bal = __ballot(fresh != 0);
+            const uint64_t gmask = G == 64 ? bal : (bal >> gbase) & ((1ull << (G & 63)) - 1);
+            if (gmask == 0) continue;
+            uint64_t inc = fresh;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const uint64_t t = __shfl_up(inc, d, G);
+                if (gl >= d) inc |= t;
+            }
+            const uint64_t prev = __shfl_up(inc, 1, G);
+            const uint64_t first = fresh & ~(gl ? prev : 0ull);
+            if (first) write_parents(parent, v, first, new2old[u]);
+            found |= __shfl(inc, G - 1, G);
             if (found == unv) {
-                base += 64;
+                base += G;
                 break;
             }
         }
-        if (found >> lane & 1) level[(int64_t)v * 64 + lane] = depth;
-        if (lane == 0) {
+        if (found) {
+#pragma unroll
+            for (int r = gl; r < 64; r += G)
+                if (found >> r & 1) level[(int64_t)v * 64 + r] = depth;
+        }
+        if (gl == 0) {
             next[v] = found;
             if (found) visited[v] = vis | found;
+            nbits += __popcll(found);
+            nedges += min(base, e) - b;
+            ++nverts;
         }
-        nbits += __popcll(found);
-        nedges += min(base, e) - b;
-        ++nverts;
     }
+    nbits = g5_wave_sum(nbits);
+    nedges = g5_wave_sum(nedges);
+    nverts = g5_wave_sum(nverts);
     ngath = g5_wave_sum(ngath);
-    if (lane == 0) {
+    if (__lane_id() == 0) {
         if (ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
         if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
         if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
@@ -267,12 +280,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all)
+    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
 {
     __shared__ uint64_t s_wave[kBlock / 64];
     __shared__ uint64_t s_vis;
     const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 8;
+    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     int64_t nbits = 0, nedges = 0, nverts = 0;
@@ -414,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
         }
         __syncthreads();
     }
-    if (!kMin && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)&cnt[12 + kEdgesRead], (unsigned long long)E);
+    if (!kMin && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kEdgesRead], (unsigned long long)E);
 }
 
 // wave per newly reached vertex: stage = 0 parent slots := INT32_MAX,
@@ -444,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_push_vertices(const uint32_t *__rest
         }
     }
     if (kStage == 1 && lane == 0 && nbits) {
-        atomicAdd((unsigned long long *)&cnt[12 + kNewBits], (unsigned long long)nbits);
-        atomicAdd((unsigned long long *)&cnt[12 + kVisitedVerts], (unsigned long long)nv);
+        atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kNewBits], (unsigned long long)nbits);
+        atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kVisitedVerts], (unsigned long long)nv);
     }
 }
 
@@ -514,20 +527,19 @@ __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const 
 }
 
 // Degree-order keys: (2^32-1 - degree) << 32 | id, so an ascending sort
-// gives descending degree, ties by id; also counts the three degree bins.
+// gives descending degree, ties by id; also counts the degree bins.
 __global__ void k_degree_keys(const int64_t *__restrict__ rowptr, int64_t n, uint64_t *__restrict__ keys,
                               unsigned long long *__restrict__ bins)
 {
-    unsigned long long c[3] = {0, 0, 0};
+    unsigned long long c[kBins] = {0, 0, 0};
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
          v += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = rowptr[v + 1] - rowptr[v];
         keys[v] = ((uint64_t)(0xFFFFFFFFu - (uint32_t)d) << 32) | (uint64_t)v;
-        if (d > kMediumMax) ++c[0];
-        else if (d > kSmallMax) ++c[1];
-        else if (d > 0) ++c[2];
+        for (int k = 0; k < kBins; ++k)
+            if (d > 0 && d <= kBinMaxDeg[k] && (k + 1 == kBins || d > kBinMaxDeg[k + 1])) ++c[k];
     }
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kBins; ++k) {
         const unsigned long long t = (unsigned long long)g5_wave_sum((int64_t)c[k]);
         if (__lane_id() == 0 && t) atomicAdd(&bins[k], t);
     }
@@ -603,25 +615,26 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool prof = false)
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     hipLaunchKernelGGL(k_level_begin, 1, 1, 0, s, m->d_cnt);
-    if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
-    if (R.n_large)
-        hipLaunchKernelGGL(k_pull_block, (unsigned)std::min<int64_t>(R.n_large, 4096), kBlock, 0, s, (int64_t)0,
-                           R.n_large, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
-                           m->d_cnt);
-    if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
-    if (R.n_medium)
-        hipLaunchKernelGGL(k_pull_wave, g5::grid_for((R.n_medium + 3) / 4, 1, 16384), kBlock, 0, s, R.n_large,
-                           R.n_medium, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
-                           m->d_cnt);
-    if (prof) G5_HIP(hipEventRecord(m->ek[2], s));
-    if (R.n_small)
-        hipLaunchKernelGGL(k_pull_thread, g5::grid_for(R.n_small, kBlock, 16384), kBlock, 0, s,
-                           R.n_large + R.n_medium, R.n_small, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                           (const uint32_t *)R.new2old, m->d_visited, front, act, next, m->d_parent, m->d_level,
-                           m->d_cnt);
-    if (prof) G5_HIP(hipEventRecord(m->ek[3], s));
+    for (int k = 0; k < kBins; ++k) {
+        if (prof) G5_HIP(hipEventRecord(m->ek[k], s));
+        const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
+        if (!cnt) continue;
+#define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.new2old, \
+                     m->d_visited, front, act, next, m->d_parent, m->d_level, m->d_cnt, k
+        switch (k) {
+        case 0:
+            hipLaunchKernelGGL(k_pull_block, (unsigned)std::min<int64_t>(cnt, 4096), kBlock, 0, s, G5_PULL_ARGS);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_pull_group<64>, g5::grid_for(cnt, kBlock / 64, 16384), kBlock, 0, s, G5_PULL_ARGS);
+            break;
+        default:
+            hipLaunchKernelGGL(k_pull_thread, g5::grid_for(cnt, kBlock, 16384), kBlock, 0, s, G5_PULL_ARGS);
+            break;
+        }
+#undef G5_PULL_ARGS
+    }
+    if (prof) G5_HIP(hipEventRecord(m->ek[kBins], s));
     hipLaunchKernelGGL(k_build_active, g5::grid_for((nw + 3) / 4, 1, 16384), 256, 0, s, (const uint64_t *)next, nw,
                        (const int64_t *)R.rowptr, act_next, m->d_cnt);
     hipLaunchKernelGGL(k_level_end, 1, 1, 0, s, m->d_cnt);
@@ -699,14 +712,14 @@ int build_relabel(g5_msbfs *m)
     };
     hipError_t e = hipMalloc(&keys, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&tmp, (size_t)n * 8);
-    if (e == hipSuccess) e = hipMalloc(&bins, 3 * 8);
+    if (e == hipSuccess) e = hipMalloc(&bins, kBins * 8);
     if (e == hipSuccess) e = hipMalloc(&ndeg, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.new2old, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.old2new, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.gdeg, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
-    if (e == hipSuccess) e = hipMemsetAsync(bins, 0, 3 * 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(bins, 0, kBins * 8, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_degree_keys, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)g->d_rowptr, n,
                            keys, bins);
@@ -726,14 +739,13 @@ int build_relabel(g5_msbfs *m)
                            (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols);
         e = hipGetLastError();
     }
-    unsigned long long hb[3] = {0, 0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(hb, bins, 3 * 8, hipMemcpyDeviceToHost, s);
+    unsigned long long hb[kBins] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(hb, bins, kBins * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     if (e != hipSuccess) return g5::fail_hip(e, "msbfs relabel", __FILE__, __LINE__);
-    R.n_large = (int64_t)hb[0];
-    R.n_medium = (int64_t)hb[1];
-    R.n_small = (int64_t)hb[2];
+    R.bin_first[0] = 0;
+    for (int k = 0; k < kBins; ++k) R.bin_first[k + 1] = R.bin_first[k] + (int64_t)hb[k];
     return G5_OK;
 }
 
@@ -770,7 +782,7 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
-    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
+    for (int k = 0; k <= kBins && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_msbfs_create allocation", __FILE__, __LINE__));
     int rc = build_relabel(m);
     if (rc) return fail(rc);
@@ -792,8 +804,10 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
-    hipEvent_t evs[] = {m->ev0, m->ev1, m->ek[0], m->ek[1], m->ek[2], m->ek[3]};
+    hipEvent_t evs[] = {m->ev0, m->ev1};
     for (hipEvent_t ev : evs)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : m->ek)
         if (ev) (void)hipEventDestroy(ev);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
@@ -845,7 +859,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kMsN * 8, hipMemcpyDeviceToHost, s));
         G5_HIP(hipStreamSynchronize(s));
         int64_t new_bits = 0;
-        for (int c = 0; c < 4; ++c) new_bits += m->h_cnt[4 * c + kNewBits];
+        for (int c = 0; c < kUnits; ++c) new_bits += m->h_cnt[4 * c + kNewBits];
         if (m->prof) {
             double rec[kLevelRec] = {0};
             rec[8] = (double)new_bits;
@@ -856,36 +870,33 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
                 // 16 B row offsets per frontier vertex; per frontier edge and
                 // pass (two passes) 4 B column + 8 B frontier word + 8 B
                 // visited word + 8 B next word; 16 B visited read-modify-write
-                // per newly reached vertex; 4 B parent init + 4 B parent + 4 B
+                // per newly reached vertex; 4 B parent init + 4 B parent + 2 B
                 // level per new (vertex, root) pair
                 float kms = 0;
                 G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
                 const double E = (double)m->h_cnt[kFqEdges];
-                m->k_ms[3] += kms;
-                m->k_launches[3] += 1;
-                m->k_bytes[3] += 16.0 * m->h_cnt[kFqLen] + 56.0 * E + 16.0 * m->h_cnt[12 + kVisitedVerts] +
-                                 12.0 * m->h_cnt[12 + kNewBits];
-                rec[3] = kms;
-                rec[7] = E;
+                m->k_ms[kPushUnit] += kms;
+                m->k_launches[kPushUnit] += 1;
+                m->k_bytes[kPushUnit] += 16.0 * m->h_cnt[kFqLen] + 56.0 * E +
+                                         16.0 * m->h_cnt[4 * kPushUnit + kVisitedVerts] +
+                                         10.0 * m->h_cnt[4 * kPushUnit + kNewBits];
+                rec[kPushUnit] = kms;
+                rec[4 + kPushUnit] = E;
             } else {
-                // per class c (events ek[2-c] .. ek[3-c]): 16 B row offsets + 8 B
-                // visited read + 8 B next write per vertex with an unvisited
-                // lane, 4 B column + 1 bit of the active bitmap per edge read,
-                // 8 B per frontier word gathered, 4 B parent + 4 B level per
-                // new (vertex, root) pair
-                const int order[3] = {2, 1, 0};  // launch order: large, medium, small
-                for (int k = 0; k < 3; ++k) {
-                    const int c = order[k];
-                    const int64_t nl = c == 0 ? m->rl.n_small : c == 1 ? m->rl.n_medium : m->rl.n_large;
-                    if (!nl) continue;
+                // per bin: 16 B row offsets + 8 B visited read + 8 B next
+                // write per vertex with an unvisited lane; 4 B column + 1 bit
+                // of the active bitmap per edge read; 8 B per frontier word
+                // gathered; 4 B parent + 2 B level per new (vertex, root) pair
+                for (int k = 0; k < kBins; ++k) {
+                    if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
                     float kms = 0;
                     G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
-                    m->k_ms[c] += kms;
-                    m->k_launches[c] += 1;
-                    m->k_bytes[c] += 32.0 * m->h_cnt[4 * c + kVisitedVerts] + 4.125 * m->h_cnt[4 * c + kEdgesRead] +
-                                     8.0 * m->h_cnt[4 * c + kGathers] + 8.0 * m->h_cnt[4 * c + kNewBits];
-                    rec[c] = kms;
-                    rec[4 + c] = (double)m->h_cnt[4 * c + kEdgesRead];
+                    m->k_ms[k] += kms;
+                    m->k_launches[k] += 1;
+                    m->k_bytes[k] += 32.0 * m->h_cnt[4 * k + kVisitedVerts] + 4.125 * m->h_cnt[4 * k + kEdgesRead] +
+                                     8.0 * m->h_cnt[4 * k + kGathers] + 6.0 * m->h_cnt[4 * k + kNewBits];
+                    rec[k] = kms;
+                    rec[4 + k] = (double)m->h_cnt[4 * k + kEdgesRead];
                 }
             }
             m->lvl.insert(m->lvl.end(), rec, rec + kLevelRec);
@@ -950,7 +961,7 @@ extern "C" int g5_msbfs_profile(g5_msbfs *m, int enable)
 {
     G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_profile: null handle");
     m->prof = enable != 0;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kUnits; ++c) {
         m->k_ms[c] = m->k_bytes[c] = 0;
         m->k_launches[c] = 0;
     }
@@ -966,19 +977,17 @@ extern "C" int g5_msbfs_set_mode(g5_msbfs *m, int mode)
 
 extern "C" int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *launches, double *bytes)
 {
-    G5_REQUIRE(m && kernel >= 0 && kernel < 4, G5_E_ARG, "g5_msbfs_stats: bad arguments");
+    G5_REQUIRE(m && kernel >= 0 && kernel < kUnits, G5_E_ARG, "g5_msbfs_stats: bad arguments");
     if (ms) *ms = m->k_ms[kernel];
     if (launches) *launches = m->k_launches[kernel];
     if (bytes) *bytes = m->k_bytes[kernel];
     return G5_OK;
 }
 
-extern "C" int g5_msbfs_info(g5_msbfs *m, int64_t *n_small, int64_t *n_medium, int64_t *n_large)
+extern "C" int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap)
 {
-    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_info: null handle");
-    if (n_small) *n_small = m->rl.n_small;
-    if (n_medium) *n_medium = m->rl.n_medium;
-    if (n_large) *n_large = m->rl.n_large;
+    G5_REQUIRE(m && bins, G5_E_ARG, "g5_msbfs_info: null argument");
+    for (int k = 0; k < kBins && k < cap; ++k) bins[k] = m->rl.bin_first[k + 1] - m->rl.bin_first[k];
     return G5_OK;
 }
 

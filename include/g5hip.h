@@ -133,18 +133,19 @@ int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
  * fewer than nnz/16, pull otherwise; 1 pull only; 2 push only.  Results are
  * identical in every mode. */
 int g5_msbfs_set_mode(g5_msbfs *m, int mode);
-/* Per-kernel accounting (kernel 0: thread-per-vertex pull, 1: wave-per-vertex
- * pull, 2: workgroup-per-vertex pull, 3: push level): enabling runs pull
- * levels as direct launches timed with events (otherwise each pull level
- * replays a captured hipGraph). */
+/* Per-kernel accounting.  Units 0..2 are the pull kernels of the degree bins
+ * (0: workgroup per vertex, degree > 2048; 1: wavefront per vertex,
+ * 33..2048; 2: thread per vertex, 1..32), unit 3 the push levels.  Enabling runs pull levels as direct launches timed with
+ * events (otherwise each pull level replays a captured hipGraph); bytes are
+ * the algorithmic bytes of the unit's launches. */
 int g5_msbfs_profile(g5_msbfs *m, int enable);
-int g5_msbfs_stats(g5_msbfs *m, int kernel, double *ms, int64_t *launches, double *bytes);
-/* Per level of the last profiled run, 12 doubles each: kernel ms (0..3 as in
- * g5_msbfs_stats), edges read (0..3), new (vertex, root) pairs, 1 if the
- * level pushed, the frontier's vertices and edges (-1 = not counted).  *nlev = levels recorded; at most cap are copied. */
+int g5_msbfs_stats(g5_msbfs *m, int unit, double *ms, int64_t *launches, double *bytes);
+/* Per level of the last profiled run, 12 doubles each: ms of units 0..3 (as
+ * in g5_msbfs_stats), edges read by units 0..3, new (vertex, root) pairs, 1
+ * if the level pushed, the frontier's vertices and edges.  *nlev = levels recorded; at most cap are copied. */
 int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev);
-/* Sizes of the three degree bins (<= 32, <= 2048, > 2048 neighbours). */
-int g5_msbfs_info(g5_msbfs *m, int64_t *n_small, int64_t *n_medium, int64_t *n_large);
+/* Vertex counts of the three degree bins (units 0..2 above), up to cap. */
+int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
 
 /* ---- s4-bp128-d4 codec (codecfactory.h:103-105) on the device ----
  * Word stream identical to the reference codec's output for 16-byte aligned
