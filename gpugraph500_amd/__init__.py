@@ -279,7 +279,7 @@ class BFS:
 class MSBFS:
     """Batched multi-source BFS over up to 64 roots (uint64 lane masks)."""
 
-    KERNELS = ("k_pull_block", "k_pull_group<64>", "k_pull_thread", "push")
+    KERNELS = ("k_pull_block", "k_pull_wave", "k_pull_thread", "push")
 
     def __init__(self, graph):
         self.graph = graph

@@ -30,11 +30,15 @@
 // sequence is captured once into a hipGraph per frontier parity and
 // replayed; the level number lives in device memory.
 //
-// Output layout in HBM: parent int32[n][64] (original ids) and level
-// uint16[n][64] (the reference validator's depth width, validate.h:48-58),
-// vertex-major in the relabelled order, valid where the vertex's visited bit
-// for that lane is set.  Per-root arrays in original ids are gathered by
-// g5_msbfs_result.
+// Output in HBM: parent int32[n][64] (original ids), vertex-major in the
+// relabelled order, valid where the vertex's visited bit for that lane is
+// set.  The pull kernels store each vertex's new parents with one masked,
+// non-temporal store of its 256 B record: the records are not read again
+// during the search, so they should not displace frontier words from the
+// caches.  The search stores no levels: like the reference validator, which
+// derives depths from the predecessor map (validate/validate.h:107-232),
+// g5_msbfs_result computes a root's levels from its parent tree (pointer
+// jumping) when it gathers the per-root arrays in original ids.
 #include <algorithm>
 #include <vector>
 
@@ -53,16 +57,29 @@ constexpr int kBlock = 256;
 constexpr int kLevelRec = 12;  // per level: ms[4], edges[4], new pairs, pushed, frontier vertices, edges
 // push when the frontier's edges are fewer than nnz / kPushRatio
 constexpr int64_t kPushRatio = 16;
+// pull levels consult the active bitmap when the frontier's edges are fewer
+// than nnz / kActiveRatio (denser frontiers make nearly every check a hit)
+constexpr int64_t kActiveRatio = 2;
 
-// counters: per accounting unit b (the kBins pull bins, then kPushUnit)
-// slots 4b+0 new (vertex, root) pairs, 4b+1 edges read, 4b+2 vertices
-// worked on, 4b+3 frontier words gathered; then the level number, the next
-// frontier's active vertices and their edges, the push queue length, and the
-// push frontier queue length / edge total.
+// Level counters.  Per accounting unit u (the kBins pull bins, then
+// kPushUnit) kUnitSlots accumulated slots at u * kUnitSlots: new (vertex,
+// root) pairs, edges read, vertices worked on, frontier words gathered, and
+// the next frontier's vertices and edges.  Then set-once slots: the level
+// number, the newly reached queue length (push), the frontier queue length
+// and its edge total (push).
 enum MsCnt {
-    kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3,
-    kDepth = 16, kActVerts = 17, kActEdges = 18, kNq = 19, kFqLen = 20, kFqEdges = 21, kMsN = 24
+    kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
+    kUnitSlots = 6,
+    kAccum = kUnits * kUnitSlots,
+    kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3,
+    kCntStride = 32
 };
+// Many workgroups adding into one address serialise in one L2 channel, so
+// the accumulated slots are striped: workgroup b adds into stripe b %
+// kStripes (stripes 256 B apart) and k_level_end folds the stripes into
+// stripe 0, which also holds the set-once slots.
+constexpr int kStripes = 32;
+static_assert(kAccum + 4 <= kCntStride, "counter stripe too small");
 
 // The degree-ordered copy of the block.
 struct Relabeled {
@@ -84,12 +101,11 @@ struct g5_msbfs {
     uint32_t *d_fq = nullptr, *d_nq = nullptr;  // push: frontier queue, newly reached queue
     int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
     int32_t *d_parent = nullptr;
-    uint16_t *d_level = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
     int64_t *d_scratch = nullptr;
     int64_t *d_roots = nullptr;
     Relabeled rl;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2][2] = {};  // [frontier parity][active-bitmap check]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nroots = 0;
     int64_t roots[64];
@@ -115,6 +131,40 @@ __device__ __forceinline__ void write_parents(int32_t *__restrict__ parent, int6
     }
 }
 
+// lane-to-wave broadcast; l must be wave-uniform
+__device__ __forceinline__ uint32_t lane_read(uint32_t x, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
+}
+__device__ __forceinline__ uint64_t lane_read64(uint64_t x, int l)
+{
+    return ((uint64_t)lane_read((uint32_t)(x >> 32), l) << 32) | lane_read((uint32_t)x, l);
+}
+
+// Adds the workgroup's totals of N per-thread counts to slots c[0..N-1] of
+// the workgroup's counter stripe, one atomic per nonzero slot (every thread
+// of the kBlock-thread workgroup must call it).
+template <int N>
+__device__ __forceinline__ void block_add(int64_t *__restrict__ c, const int64_t (&x)[N])
+{
+    __shared__ int64_t s_sum[kBlock / 64][N];
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int64_t t = g5_wave_sum(x[k]);
+        if (__lane_id() == 0) s_sum[wv][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += s_sum[w][threadIdx.x];
+        if (t)
+            atomicAdd((unsigned long long *)&c[(blockIdx.x % kStripes) * kCntStride + threadIdx.x],
+                      (unsigned long long)t);
+    }
+}
+
 // exclusive prefix OR over the lanes of a wave
 __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 {
@@ -128,167 +178,221 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
     return __lane_id() == 0 ? 0ull : prev;
 }
 
-// One thread per vertex (degree <= 32): a sequential ascending scan.
+// One thread per vertex (degree <= 32): a sequential ascending scan.  Each
+// thread keeps its first kHits hits (parent id, new lanes) in registers; once
+// the wave's 64 vertices are scanned, the wave writes them out vertex by
+// vertex, lane r storing root r's parent, so every store instruction covers
+// one vertex's contiguous 256 B parent record instead of 64 scattered slots.
+// Hits beyond kHits (rare: they need more than kHits distinct first-hit
+// neighbours) are stored directly.  The scan loads kThreadUnroll neighbours
+// and their frontier words before using any of them, so that several
+// gathers are in flight per thread.
+// kActive (all pull kernels): skip neighbours whose bit in the active bitmap
+// is clear before gathering their frontier word (pays only when few sources
+// are active).
+constexpr int kHits = 4;
+constexpr int kThreadUnroll = 4;
+
+template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin)
 {
-    const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
-    int64_t ngath = 0;
-    int64_t nbits = 0, nedges = 0, nverts = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nlist;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = __lane_id();
+    int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
+    // wave-uniform loop: the wave takes 64 consecutive vertices per step
+    for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < nlist;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = base + lane;
         const uint32_t v = (uint32_t)(first_v + i);
-        const uint64_t vis = visited[v];
-        if (vis == ~0ull) {
-            next[v] = 0;
-            continue;
+        uint64_t found = 0, spilled = 0;
+        uint32_t hu[kHits];
+        uint64_t hf[kHits];
+#pragma unroll
+        for (int k = 0; k < kHits; ++k) {
+            hu[k] = 0;
+            hf[k] = 0;
         }
-        const uint64_t unv = ~vis;
-        uint64_t found = 0;
-        const int64_t b = rowptr[v], e = rowptr[v + 1];
-        int64_t j = b;
-        for (; j < e; ++j) {
-            const uint32_t u = cols[j];
-            if (!(active[u >> 6] >> (u & 63) & 1)) continue;
-            ++ngath;
-            const uint64_t f = frontier[u] & unv & ~found;
-            if (f) {
-                found |= f;
-                write_parents(parent, v, f, new2old[u]);
-                if (found == unv) {
-                    ++j;
-                    break;
+        if (i < nlist) {
+            const uint64_t vis = visited[v];
+            if (vis != ~0ull) {
+                const uint64_t unv = ~vis;
+                const int64_t b = rowptr[v], e = rowptr[v + 1];
+                int64_t j = b;
+                int nh = 0;
+                while (j < e && found != unv) {
+                    uint32_t u[kThreadUnroll];
+                    uint64_t fr[kThreadUnroll];
+#pragma unroll
+                    for (int k = 0; k < kThreadUnroll; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
+#pragma unroll
+                    for (int k = 0; k < kThreadUnroll; ++k) {
+                        const bool take = j + k < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
+                        fr[k] = take ? frontier[u[k]] : 0ull;
+                        ngath += take;
+                    }
+                    j += kThreadUnroll;
+                    // in neighbour order: the first hit of a lane is its parent
+#pragma unroll
+                    for (int k = 0; k < kThreadUnroll; ++k) {
+                        const uint64_t f = fr[k] & unv & ~found;
+                        if (!f) continue;
+                        found |= f;
+                        const uint32_t uo = new2old[u[k]];
+                        if (nh < kHits) {
+#pragma unroll
+                            for (int h = 0; h < kHits; ++h)
+                                if (h == nh) {
+                                    hu[h] = uo;
+                                    hf[h] = f;
+                                }
+                            ++nh;
+                        } else {
+                            write_parents(parent, v, f, uo);
+                            spilled |= f;
+                        }
+                    }
                 }
+                j = j < e ? j : e;
+                if (found) {
+                    visited[v] = vis | found;
+                    ++nnext;
+                    enext += e - b;
+                }
+                nedges += j - b;
+                ++nverts;
+                nbits += __popcll(found);
             }
+            next[v] = found;
         }
-        next[v] = found;
-        nedges += j - b;
-        ++nverts;
-        if (found) {
-            visited[v] = vis | found;
-            nbits += __popcll(found);
-            uint64_t bits = found;
-            while (bits) {
-                const int r = __ffsll((long long)bits) - 1;
-                level[(int64_t)v * 64 + r] = depth;
-                bits &= bits - 1;
+        // write-out: one vertex per step, lane = root
+        uint64_t todo = __ballot(found != 0);
+        while (todo) {
+            const int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint64_t fj = lane_read64(found, src);
+            const uint64_t sj = lane_read64(spilled, src);
+            const int64_t rec = (int64_t)lane_read(v, src) * 64 + lane;
+            uint32_t p = 0;
+#pragma unroll
+            for (int k = 0; k < kHits; ++k) {
+                const uint64_t fk = lane_read64(hf[k], src);
+                const uint32_t uk = lane_read(hu[k], src);
+                if (fk >> lane & 1) p = uk;
             }
+            if ((fj & ~sj) >> lane & 1) __builtin_nontemporal_store((int32_t)p, &parent[rec]);
         }
     }
-    nbits = g5_wave_sum(nbits);
-    nedges = g5_wave_sum(nedges);
-    nverts = g5_wave_sum(nverts);
-    ngath = g5_wave_sum(ngath);
-    if (__lane_id() == 0) {
-        if (ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
-        if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
-        if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
-        if (nverts) atomicAdd((unsigned long long *)&cnt[kVisitedVerts], (unsigned long long)nverts);
-    }
+    block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
 }
 
-// G lanes per vertex (used with G = 64 for degrees 33..2048): a step loads
-// G neighbours at once; the group's exclusive prefix-OR gives each root's
-// first hit, the group's inclusive OR the new bits.  Control flow is uniform
-// within a group (v, found and the step count are the group's), so the
-// width-G shuffles only read lanes of the same group.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_pull_group(
+// One wavefront per vertex (degrees 33..2048): a step loads 64 neighbours at
+// once; the wave's exclusive prefix-OR gives each root's first hit, the
+// inclusive OR the new bits.  First-hit lanes drop the parent id into the
+// wave's 64-slot LDS row (slot r = root r); when the vertex is done lane r
+// stores slot r, so the vertex's parents leave in one contiguous store.
+// Each iteration loads kWaveUnroll steps (64 neighbours each) and their
+// frontier words before processing the steps in order.
+constexpr int kWaveUnroll = 1;
+
+template <bool kActive>
+__global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin)
 {
-    constexpr int kGroups = kBlock / G;
-    const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
-    const int gl = threadIdx.x & (G - 1);
-    const int gbase = (int)(__lane_id() & ~(G - 1));
-    const int64_t stride = (int64_t)gridDim.x * kGroups;
-    int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kGroups + threadIdx.x / G; i < nlist; i += stride) {
+    __shared__ int32_t s_par[kBlock / 64][64];
+    int32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
+    const int lane = __lane_id();
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
+    int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0, nnext = 0, enext = 0;
+    for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += stride) {
         const uint32_t v = (uint32_t)(first_v + i);
         const uint64_t vis = visited[v];
         if (vis == ~0ull) {
-            if (gl == 0) next[v] = 0;
+            if (lane == 0) next[v] = 0;
             continue;
         }
         const uint64_t unv = ~vis;
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
-        for (; base < e; base += G) {
-            const int64_t j = base + gl;
-            uint32_t u = 0;
-            uint64_t fresh = 0;
-            if (j < e) {
-                u = cols[j];
-                if (active[u >> 6] >> (u & 63) & 1) {
-                    fresh = frontier[u] & unv & ~found;
-                    ++ngath;
-                }
-            }
-            const uint64_t bal = __ballot(fresh != 0);
-            const uint64_t gmask = G == 64 ? bal : (bal >> gbase) & ((1ull << (G & 63)) - 1);
-            if (gmask == 0) continue;
-            uint64_t inc = fresh;
+        while (base < e && found != unv) {
+            uint32_t u[kWaveUnroll];
+            uint64_t fr[kWaveUnroll];
 #pragma unroll
-            for (int d = 1; d < G; d <<= 1) {
-                const uint64_t t = __shfl_up(inc, d, G);
-                if (gl >= d) inc |= t;
+            for (int k = 0; k < kWaveUnroll; ++k) {
+                const int64_t j = base + 64 * k + lane;
+                u[k] = j < e ? cols[j] : 0u;
             }
-            const uint64_t prev = __shfl_up(inc, 1, G);
-            const uint64_t first = fresh & ~(gl ? prev : 0ull);
-            if (first) write_parents(parent, v, first, new2old[u]);
-            found |= __shfl(inc, G - 1, G);
-            if (found == unv) {
-                base += G;
-                break;
+#pragma unroll
+            for (int k = 0; k < kWaveUnroll; ++k) {
+                const int64_t j = base + 64 * k + lane;
+                const bool take = j < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
+                fr[k] = take ? frontier[u[k]] : 0ull;
+                ngath += take;
+            }
+            base += 64 * kWaveUnroll;
+#pragma unroll
+            for (int k = 0; k < kWaveUnroll; ++k) {
+                const uint64_t fresh = fr[k] & unv & ~found;
+                if (__ballot(fresh != 0) == 0) continue;
+                const uint64_t prev = lanes_excl_or(fresh);
+                uint64_t first = fresh & ~prev;
+                if (first) {
+                    const int32_t uo = (int32_t)new2old[u[k]];
+                    while (first) {
+                        sp[__ffsll((long long)first) - 1] = uo;
+                        first &= first - 1;
+                    }
+                }
+                found |= __shfl(prev | fresh, 63, 64);
             }
         }
         if (found) {
-#pragma unroll
-            for (int r = gl; r < 64; r += G)
-                if (found >> r & 1) level[(int64_t)v * 64 + r] = depth;
+            // the row's LDS writes come from other lanes of this wave; LDS
+            // operations of one wave complete in issue order
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (found >> lane & 1) __builtin_nontemporal_store(sp[lane], &parent[(int64_t)v * 64 + lane]);
+            __builtin_amdgcn_wave_barrier();
         }
-        if (gl == 0) {
+        if (lane == 0) {
             next[v] = found;
-            if (found) visited[v] = vis | found;
+            if (found) {
+                visited[v] = vis | found;
+                ++nnext;
+                enext += e - b;
+            }
             nbits += __popcll(found);
             nedges += min(base, e) - b;
             ++nverts;
         }
     }
-    nbits = g5_wave_sum(nbits);
-    nedges = g5_wave_sum(nedges);
-    nverts = g5_wave_sum(nverts);
-    ngath = g5_wave_sum(ngath);
-    if (__lane_id() == 0) {
-        if (ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
-        if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
-        if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
-        if (nverts) atomicAdd((unsigned long long *)&cnt[kVisitedVerts], (unsigned long long)nverts);
-    }
+    block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
 }
 
-// One workgroup per vertex (degree > kMediumMax): 256 neighbours per round.
+// One workgroup per vertex (degree > kMediumMax): kBlockUnroll sub-rounds of
+// 256 neighbours per pair of barriers.
+constexpr int kBlockUnroll = 1;
+
+template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint16_t *__restrict__ level, int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin)
 {
-    __shared__ uint64_t s_wave[kBlock / 64];
+    __shared__ uint64_t s_wave[kBlockUnroll][kBlock / 64];
     __shared__ uint64_t s_vis;
-    const uint16_t depth = (uint16_t)cnt_all[kDepth];
-    int64_t *__restrict__ cnt = cnt_all + 4 * bin;
+    __shared__ int32_t s_par[64];
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
-    int64_t nbits = 0, nedges = 0, nverts = 0;
+    int64_t nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         const uint32_t v = (uint32_t)(first_v + i);
         if (threadIdx.x == 0) s_vis = visited[v];
@@ -303,67 +407,86 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
-        for (; base < e; base += kBlock) {
-            const int64_t j = base + threadIdx.x;
-            uint32_t u = 0;
-            uint64_t f = 0;
-            if (j < e) {
-                u = cols[j];
-                if (active[u >> 6] >> (u & 63) & 1) {
-                    f = frontier[u] & unv;
-                    ++ngath;
-                }
-            }
-            f &= ~found;
-            const bool any = __ballot(f != 0) != 0;
-            const uint64_t in_wave = any ? lanes_excl_or(f) : 0ull;
-            const uint64_t wave_all = any ? __shfl(in_wave | f, 63, 64) : 0ull;
-            if (lane == 0) s_wave[wv] = wave_all;
-            __syncthreads();
-            uint64_t before = 0, all = 0;
+        // found is uniform across the workgroup: it changes only after the
+        // barrier, from the shared per-wave ORs
+        while (base < e && found != unv) {
+            uint32_t u[kBlockUnroll];
+            uint64_t f[kBlockUnroll], in_wave[kBlockUnroll];
 #pragma unroll
-            for (int w = 0; w < kBlock / 64; ++w) {
-                const uint64_t t = s_wave[w];
-                before |= (w < wv) ? t : 0ull;
-                all |= t;
+            for (int k = 0; k < kBlockUnroll; ++k) {
+                const int64_t j = base + kBlock * k + threadIdx.x;
+                u[k] = j < e ? cols[j] : 0u;
             }
-            const uint64_t first = f & ~(in_wave | before);
-            if (first) write_parents(parent, v, first, new2old[u]);
-            found |= all;
+#pragma unroll
+            for (int k = 0; k < kBlockUnroll; ++k) {
+                const int64_t j = base + kBlock * k + threadIdx.x;
+                const bool take = j < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
+                f[k] = take ? frontier[u[k]] & unv & ~found : 0ull;
+                ngath += take;
+            }
+            base += kBlock * kBlockUnroll;
+#pragma unroll
+            for (int k = 0; k < kBlockUnroll; ++k) {
+                const bool any = __ballot(f[k] != 0) != 0;
+                in_wave[k] = any ? lanes_excl_or(f[k]) : 0ull;
+                const uint64_t wave_all = any ? __shfl(in_wave[k] | f[k], 63, 64) : 0ull;
+                if (lane == 0) s_wave[k][wv] = wave_all;
+            }
             __syncthreads();
-            if (found == unv) {
-                base += kBlock;
-                break;
+            // sub-round k's first hits exclude everything hit in earlier
+            // sub-rounds and, within sub-round k, in earlier waves and lanes
+            uint64_t earlier = 0;
+#pragma unroll
+            for (int k = 0; k < kBlockUnroll; ++k) {
+                uint64_t before = earlier, all = 0;
+#pragma unroll
+                for (int w = 0; w < kBlock / 64; ++w) {
+                    const uint64_t t = s_wave[k][w];
+                    before |= (w < wv) ? t : 0ull;
+                    all |= t;
+                }
+                uint64_t first = f[k] & ~(in_wave[k] | before);
+                if (first) {
+                    const int32_t uo = (int32_t)new2old[u[k]];
+                    while (first) {
+                        s_par[__ffsll((long long)first) - 1] = uo;
+                        first &= first - 1;
+                    }
+                }
+                earlier |= all;
             }
+            found |= earlier;
+            __syncthreads();
         }
-        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) level[(int64_t)v * 64 + threadIdx.x] = depth;
+        // s_par is complete: the round's second barrier follows its writes
+        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) __builtin_nontemporal_store(s_par[threadIdx.x], &parent[(int64_t)v * 64 + threadIdx.x]);
         if (threadIdx.x == 0) {
             next[v] = found;
-            if (found) visited[v] = vis | found;
+            if (found) {
+                visited[v] = vis | found;
+                ++nnext;
+                enext += e - b;
+            }
             nbits += __popcll(found);
             nedges += min(base, e) - b;
             ++nverts;
         }
     }
-    ngath = g5_wave_sum(ngath);
-    if (lane == 0 && ngath) atomicAdd((unsigned long long *)&cnt[kGathers], (unsigned long long)ngath);
-    if (threadIdx.x == 0) {
-        if (nbits) atomicAdd((unsigned long long *)&cnt[kNewBits], (unsigned long long)nbits);
-        if (nedges) atomicAdd((unsigned long long *)&cnt[kEdgesRead], (unsigned long long)nedges);
-        if (nverts) atomicAdd((unsigned long long *)&cnt[kVisitedVerts], (unsigned long long)nverts);
-    }
+    block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
 }
 
 // ---- push (top-down) levels for sparse frontiers ----
 // The frontier's vertices u (active bitmap -> sorted queue) push their lane
-// bits along their edges: next[v] |= frontier[u] & ~visited[v].  Parents are
-// then the minimum frontier neighbour per (v, root): every slot of a newly
-// reached pair is set to INT32_MAX, and a second pass over the same edges
-// lowers it with atomicMin.  Work is balanced by edges: each block tile takes
+// bits along their edges: next[v] |= frontier[u] & ~visited[v].  The newly
+// reached vertices (next's active bitmap -> sorted queue) then get their
+// parents, the minimum frontier neighbour per (v, root): every slot of a
+// newly reached pair is set to INT32_MAX, and a second pass over the same
+// edges lowers it with atomicMin.  Work is balanced by edges: each block tile takes
 // 2048 consecutive edges of the frontier and finds each edge's source by
 // binary search in the frontier's degree prefix.
 constexpr int kPushTile = 2048;
 constexpr int kPushGrid = 4096;
+constexpr int kPushVertGrid = 2048;
 
 __device__ __forceinline__ int64_t fq_src(const int64_t *__restrict__ qoff, int64_t lo, int64_t hi, int64_t e)
 {
@@ -391,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint32_t *__restrict__ nq, int64_t *__restrict__ cnt)
+    int64_t *__restrict__ cnt)
 {
     __shared__ int64_t s_lo, s_hi;
     const int64_t qn = cnt[kFqLen], E = cnt[kFqEdges];
@@ -408,13 +531,11 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
             const uint32_t u = fq[si];
             const uint32_t v = cols[rowptr[u] + (e - qoff[si])];
             const uint64_t f = frontier[u] & ~visited[v];
-            if (!f) continue;
             if (!kMin) {
-                if ((next[v] & f) != f) {
-                    const uint64_t old = atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
-                    if (old == 0) nq[atomicAdd((unsigned long long *)&cnt[kNq], 1ull)] = v;
-                }
-            } else {
+                // no return value needed: the newly reached queue is built
+                // from next's active bitmap afterwards
+                if (f) atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
+            } else if (f) {
                 uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
                 const int32_t uo = (int32_t)new2old[u];
                 while (bits) {
@@ -427,75 +548,101 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
         }
         __syncthreads();
     }
-    if (!kMin && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kEdgesRead], (unsigned long long)E);
+    if (!kMin && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd((unsigned long long *)&cnt[kUnitSlots * kPushUnit + kEdgesRead], (unsigned long long)E);
 }
 
-// wave per newly reached vertex: stage = 0 parent slots := INT32_MAX,
-// stage = 1 visited |= next, level := depth, counts
-template <int kStage>
-__global__ __launch_bounds__(kBlock) void k_push_vertices(const uint32_t *__restrict__ nq, uint64_t *__restrict__ next,
-                                                          uint64_t *__restrict__ visited, int32_t *__restrict__ parent,
-                                                          uint16_t *__restrict__ level, int64_t *__restrict__ cnt)
+// Newly reached pairs' parent slots := INT32_MAX before the atomicMin pass.
+// A wave loads 64 queue entries at once, then stores each vertex's 256 B
+// parent record with one masked store (lane = root).
+__global__ __launch_bounds__(kBlock) void k_push_init_parents(const uint32_t *__restrict__ nq,
+                                                              const uint64_t *__restrict__ next,
+                                                              int32_t *__restrict__ parent,
+                                                              const int64_t *__restrict__ cnt)
 {
     const int64_t n = cnt[kNq];
-    const uint16_t depth = (uint16_t)cnt[kDepth];
     const int lane = __lane_id();
-    const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = base + lane;
+        const uint32_t v = i < n ? nq[i] : 0u;
+        const uint64_t w = i < n ? next[v] : 0ull;
+        uint64_t todo = __ballot(w != 0);
+        while (todo) {
+            const int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            if (lane_read64(w, src) >> lane & 1) parent[(int64_t)lane_read(v, src) * 64 + lane] = 0x7FFFFFFF;
+        }
+    }
+}
+
+// visited |= next over the newly reached queue, and the push unit's counts
+__global__ __launch_bounds__(kBlock) void k_push_commit(const uint32_t *__restrict__ nq,
+                                                        const uint64_t *__restrict__ next,
+                                                        uint64_t *__restrict__ visited, int64_t *__restrict__ cnt)
+{
+    const int64_t n = cnt[kNq];
     int64_t nbits = 0, nv = 0;
-    for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const uint32_t v = nq[i];
         const uint64_t w = next[v];
-        if (kStage == 0) {
-            if (w >> lane & 1) parent[(int64_t)v * 64 + lane] = 0x7FFFFFFF;
-        } else {
-            if (w >> lane & 1) level[(int64_t)v * 64 + lane] = depth;
-            if (lane == 0) {
-                visited[v] |= w;
-                nbits += __popcll(w);
-                ++nv;
-            }
-        }
+        visited[v] |= w;
+        nbits += __popcll(w);
+        ++nv;
     }
-    if (kStage == 1 && lane == 0 && nbits) {
-        atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kNewBits], (unsigned long long)nbits);
-        atomicAdd((unsigned long long *)&cnt[4 * kPushUnit + kVisitedVerts], (unsigned long long)nv);
-    }
+    block_add(cnt + kUnitSlots * kPushUnit, {nbits, (int64_t)0, nv});
 }
 
+// one workgroup of kBlock threads: clear every stripe's accumulated slots
 __global__ void k_level_begin(int64_t *cnt)
 {
-    for (int i = 0; i < kDepth; ++i) cnt[i] = 0;
-    cnt[kActVerts] = 0;
-    cnt[kActEdges] = 0;
-    cnt[kNq] = 0;
+    for (int i = threadIdx.x; i < kStripes * kCntStride; i += blockDim.x)
+        if (i % kCntStride < kAccum) cnt[i] = 0;
+    if (threadIdx.x == 0) cnt[kNq] = 0;
 }
 
-// active[w] bit i = (next[64w + i] != 0): one wavefront per 64-vertex word
-__global__ __launch_bounds__(256) void k_build_active(const uint64_t *__restrict__ next, int64_t nwords,
-                                                      const int64_t *__restrict__ rowptr,
-                                                      uint64_t *__restrict__ active, int64_t *__restrict__ cnt)
+// active[w] bit i = (next[64w + i] != 0).  A wave takes four 64-vertex words
+// per step (four independent loads in flight per lane).  kCount: also add
+// the active vertices and their edges to the push unit's next-frontier
+// slots (pull levels count them in the pull kernels).
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restrict__ next, int64_t nwords,
+                                                         const int64_t *__restrict__ rowptr,
+                                                         uint64_t *__restrict__ active, int64_t *__restrict__ cnt)
 {
-    const int64_t waves = (int64_t)gridDim.x * 4;
+    const int lane = __lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nv = 0, ne = 0;
-    for (int64_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += waves) {
-        const int64_t v = w * 64 + __lane_id();
-        const bool on = next[v] != 0;
-        const uint64_t bits = __ballot(on);
-        if (__lane_id() == 0) active[w] = bits;
-        if (on) {
-            ++nv;
-            ne += rowptr[v + 1] - rowptr[v];
+    for (int64_t w0 = (blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6)) * 4; w0 < nwords; w0 += waves * 4) {
+        uint64_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = w0 + k < nwords ? next[(w0 + k) * 64 + lane] : 0ull;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t bits = __ballot(x[k] != 0);
+            if (lane == k) mine = bits;
+            if (kCount && x[k]) {
+                const int64_t v = (w0 + k) * 64 + lane;
+                ++nv;
+                ne += rowptr[v + 1] - rowptr[v];
+            }
         }
+        if (lane < 4 && w0 + lane < nwords) active[w0 + lane] = mine;
     }
-    nv = g5_wave_sum(nv);
-    ne = g5_wave_sum(ne);
-    if (__lane_id() == 0 && nv) {
-        atomicAdd((unsigned long long *)&cnt[kActVerts], (unsigned long long)nv);
-        atomicAdd((unsigned long long *)&cnt[kActEdges], (unsigned long long)ne);
-    }
+    if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});
 }
 
-__global__ void k_level_end(int64_t *cnt) { cnt[kDepth] += 1; }
+// one workgroup: fold the stripes into stripe 0, then advance the level
+__global__ void k_level_end(int64_t *cnt)
+{
+    for (int i = threadIdx.x; i < kAccum; i += blockDim.x) {
+        int64_t t = 0;
+        for (int st = 0; st < kStripes; ++st) t += cnt[st * kCntStride + i];
+        cnt[i] = t;
+    }
+    if (threadIdx.x == 0) cnt[kDepth] += 1;
+}
 
 __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__ fa, uint64_t *__restrict__ fb,
                           uint64_t *__restrict__ act, int64_t n, uint64_t unused_lanes)
@@ -513,7 +660,7 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
                            uint64_t *__restrict__ visited, uint64_t *__restrict__ frontier,
                            uint64_t *__restrict__ active, int32_t *__restrict__ parent,
-                           uint16_t *__restrict__ level, int64_t *__restrict__ cnt)
+                           int64_t *__restrict__ cnt)
 {
     for (int r = 0; r < nroots; ++r) {
         const int64_t v = old2new[roots[r]];
@@ -521,7 +668,6 @@ __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const 
         frontier[v] |= 1ull << r;
         active[v >> 6] |= 1ull << (v & 63);
         parent[v * 64 + r] = (int32_t)roots[r];
-        level[v * 64 + r] = 0;
     }
     cnt[kDepth] = 1;
 }
@@ -575,18 +721,55 @@ __global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ r
     }
 }
 
-// per-lane extraction: pred/level of root `lane` as int64/int32[n]; a lane's
-// entries are meaningful only where its visited bit is set
+// per-lane extraction: pred of root `lane` as int64[n] in original ids (-1
+// where the lane's visited bit is clear)
 __global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const int32_t *__restrict__ parent,
-                             const uint16_t *__restrict__ level, const uint64_t *__restrict__ visited, int64_t n,
-                             int lane, int64_t *__restrict__ pred, int32_t *__restrict__ lev)
+                             const uint64_t *__restrict__ visited, int64_t n, int lane, int64_t *__restrict__ pred)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = old2new[i];
-        const bool reached = visited[v] >> lane & 1;
-        lev[i] = reached ? (int32_t)level[v * 64 + lane] : -1;
-        pred[i] = reached ? (int64_t)parent[v * 64 + lane] : -1;
+        pred[i] = (visited[v] >> lane & 1) ? (int64_t)parent[v * 64 + lane] : -1;
+    }
+}
+
+// Levels from a parent tree by pointer jumping.  Each vertex holds one
+// 64-bit word (distance << 32 | ancestor): the ancestor is `distance` tree
+// edges above it, and the word is read and written whole, so a pass that
+// mixes old and new words still sees consistent pairs.  A pass replaces
+// (d, a) by (d + d_a, a_a) until every ancestor is the root (d_root = 0).
+__global__ void k_jump_init(const int64_t *__restrict__ pred, int64_t n, uint64_t *__restrict__ w)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = pred[i];
+        w[i] = p < 0 ? ~0ull : p == i ? (uint64_t)i : (1ull << 32) | (uint64_t)p;
+    }
+}
+
+__global__ void k_jump(uint64_t *__restrict__ w, int64_t n, int *__restrict__ changed)
+{
+    int ch = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (x == ~0ull) continue;
+        const uint32_t a = (uint32_t)x;
+        const uint64_t y = __hip_atomic_load(&w[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((y >> 32) == 0 || y == ~0ull) continue;  // a is the root (an unreached a: broken tree)
+        __hip_atomic_store(&w[i], ((x >> 32) + (y >> 32)) << 32 | (uint32_t)y, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        ch = 1;
+    }
+    if (__ballot(ch) && __lane_id() == 0) atomicOr(changed, 1);
+}
+
+__global__ void k_jump_levels(const uint64_t *__restrict__ w, int64_t n, int32_t *__restrict__ lev)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = w[i];
+        lev[i] = x == ~0ull ? -1 : (int32_t)(x >> 32);
     }
 }
 
@@ -605,8 +788,9 @@ __global__ __launch_bounds__(256) void k_ms_nedge(const uint64_t *__restrict__ v
     atomicAdd(&out[lane], acc);
 }
 
-// prof: events between the three pull kernels (the uncaptured path)
-int level_launches(g5_msbfs *m, hipStream_t s, int p, bool prof = false)
+// act: pull kernels consult the active bitmap; prof: events between the
+// three pull kernels (the uncaptured path)
+int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof = false)
 {
     const Relabeled &R = m->rl;
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
@@ -614,30 +798,26 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool prof = false)
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
-    hipLaunchKernelGGL(k_level_begin, 1, 1, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt);
     for (int k = 0; k < kBins; ++k) {
         if (prof) G5_HIP(hipEventRecord(m->ek[k], s));
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.new2old, \
-                     m->d_visited, front, act, next, m->d_parent, m->d_level, m->d_cnt, k
-        switch (k) {
-        case 0:
-            hipLaunchKernelGGL(k_pull_block, (unsigned)std::min<int64_t>(cnt, 4096), kBlock, 0, s, G5_PULL_ARGS);
-            break;
-        case 1:
-            hipLaunchKernelGGL(k_pull_group<64>, g5::grid_for(cnt, kBlock / 64, 16384), kBlock, 0, s, G5_PULL_ARGS);
-            break;
-        default:
-            hipLaunchKernelGGL(k_pull_thread, g5::grid_for(cnt, kBlock, 16384), kBlock, 0, s, G5_PULL_ARGS);
-            break;
-        }
+                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k
+        const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
+                              : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
+                                       : g5::grid_for(cnt, kBlock, 16384);
+        auto kern = k == 0 ? (act_check ? k_pull_block<true> : k_pull_block<false>)
+                    : k == 1 ? (act_check ? k_pull_wave<true> : k_pull_wave<false>)
+                             : (act_check ? k_pull_thread<true> : k_pull_thread<false>);
+        hipLaunchKernelGGL(kern, grid, kBlock, 0, s, G5_PULL_ARGS);
 #undef G5_PULL_ARGS
     }
     if (prof) G5_HIP(hipEventRecord(m->ek[kBins], s));
-    hipLaunchKernelGGL(k_build_active, g5::grid_for((nw + 3) / 4, 1, 16384), 256, 0, s, (const uint64_t *)next, nw,
+    hipLaunchKernelGGL(k_build_active<false>, g5::grid_for(nw, 16, 4096), kBlock, 0, s, (const uint64_t *)next, nw,
                        (const int64_t *)R.rowptr, act_next, m->d_cnt);
-    hipLaunchKernelGGL(k_level_end, 1, 1, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
 
@@ -651,7 +831,7 @@ int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
-    hipLaunchKernelGGL(k_level_begin, 1, 1, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt);
     G5_HIP(g5::fill_u64(next, m->n, 0, s));
     if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
     G5_HIP(g5::bitmap_to_queue(act, nw, m->d_fq, m->d_cnt + kFqLen, m->d_scratch, s));
@@ -661,36 +841,38 @@ int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
     hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_nq, m->d_cnt);
-    hipLaunchKernelGGL(k_push_vertices<0>, 2048, kBlock, 0, s, (const uint32_t *)m->d_nq, next, m->d_visited,
-                       m->d_parent, m->d_level, m->d_cnt);
+                       m->d_cnt);
+    hipLaunchKernelGGL(k_build_active<true>, g5::grid_for(nw, 16, 4096), kBlock, 0, s, (const uint64_t *)next,
+                       nw, (const int64_t *)R.rowptr, act_next, m->d_cnt);
+    G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
+    hipLaunchKernelGGL(k_push_init_parents, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
+                       (const uint64_t *)next, m->d_parent, (const int64_t *)m->d_cnt);
     hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_nq, m->d_cnt);
-    hipLaunchKernelGGL(k_push_vertices<1>, 2048, kBlock, 0, s, (const uint32_t *)m->d_nq, next, m->d_visited,
-                       m->d_parent, m->d_level, m->d_cnt);
+                       m->d_cnt);
+    hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
+                       (const uint64_t *)next, m->d_visited, m->d_cnt);
     if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
-    hipLaunchKernelGGL(k_build_active, g5::grid_for((nw + 3) / 4, 1, 16384), 256, 0, s, (const uint64_t *)next, nw,
-                       (const int64_t *)R.rowptr, act_next, m->d_cnt);
-    hipLaunchKernelGGL(k_level_end, 1, 1, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
     G5_HIP(hipGetLastError());
     return G5_OK;
 }
 
 int capture_graphs(g5_msbfs *m)
 {
-    for (int p = 0; p < 2; ++p) {
-        hipGraph_t gr = nullptr;
-        G5_HIP(hipStreamBeginCapture(m->s, hipStreamCaptureModeThreadLocal));
-        int rc = level_launches(m, m->s, p);
-        hipError_t e = hipStreamEndCapture(m->s, &gr);
-        if (rc) return rc;
-        G5_HIP(e);
-        e = hipGraphInstantiate(&m->graph[p], gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        G5_HIP(e);
-    }
+    for (int p = 0; p < 2; ++p)
+        for (int a = 0; a < 2; ++a) {
+            hipGraph_t gr = nullptr;
+            G5_HIP(hipStreamBeginCapture(m->s, hipStreamCaptureModeThreadLocal));
+            int rc = level_launches(m, m->s, p, a != 0);
+            hipError_t e = hipStreamEndCapture(m->s, &gr);
+            if (rc) return rc;
+            G5_HIP(e);
+            e = hipGraphInstantiate(&m->graph[p][a], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            G5_HIP(e);
+        }
     return G5_OK;
 }
 
@@ -775,9 +957,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_nq, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_qoff, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
-    if (e == hipSuccess) e = hipMalloc(&m->d_level, (size_t)n * 64 * 2);
-    if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kMsN * 8);
-    if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kMsN * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kStripes * kCntStride * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kCntStride * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
@@ -796,9 +977,10 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
 {
     if (!m) return G5_OK;
     if (m->s) (void)hipStreamSynchronize(m->s);
-    for (int p = 0; p < 2; ++p)
-        if (m->graph[p]) (void)hipGraphExecDestroy(m->graph[p]);
-    void *dev[] = {m->d_fq, m->d_nq, m->d_qoff, m->d_act, m->d_visited, m->d_fa,        m->d_fb,           m->d_parent,       m->d_level,
+    for (auto &gp : m->graph)
+        for (hipGraphExec_t ge : gp)
+            if (ge) (void)hipGraphExecDestroy(ge);
+    void *dev[] = {m->d_fq, m->d_nq, m->d_qoff, m->d_act, m->d_visited, m->d_fa,        m->d_fb,           m->d_parent,
                    m->d_cnt,     m->d_scratch,   m->d_roots,        m->rl.new2old,     m->rl.old2new,     m->rl.rowptr,
                    m->rl.cols,   m->rl.gdeg};
     for (void *p : dev)
@@ -831,7 +1013,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
                        m->d_act, m->n, unused);
     hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
-                       m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_level, m->d_cnt);
+                       m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
     G5_HIP(hipGetLastError());
     int p = 0, levels = 1;
     m->lvl.clear();
@@ -847,19 +1029,26 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     const int64_t nnz = m->g->nnz;
     for (;;) {
         const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * kPushRatio < nnz));
+        const bool act_check = act_edges < 0 || act_edges * kActiveRatio < nnz;
         if (push) {
             int rc = push_level(m, s, p, act_verts, m->prof);
             if (rc) return rc;
         } else if (m->prof) {
-            int rc = level_launches(m, s, p, true);
+            int rc = level_launches(m, s, p, act_check, true);
             if (rc) return rc;
         } else {
-            G5_HIP(hipGraphLaunch(m->graph[p], s));
+            G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
         }
-        G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kMsN * 8, hipMemcpyDeviceToHost, s));
+        G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kCntStride * 8, hipMemcpyDeviceToHost, s));
         G5_HIP(hipStreamSynchronize(s));
         int64_t new_bits = 0;
-        for (int c = 0; c < kUnits; ++c) new_bits += m->h_cnt[4 * c + kNewBits];
+        const int64_t *h = m->h_cnt;
+        int64_t next_verts = 0, next_edges = 0;
+        for (int c = 0; c < kUnits; ++c) {
+            new_bits += h[kUnitSlots * c + kNewBits];
+            next_verts += h[kUnitSlots * c + kNextVerts];
+            next_edges += h[kUnitSlots * c + kNextEdges];
+        }
         if (m->prof) {
             double rec[kLevelRec] = {0};
             rec[8] = (double)new_bits;
@@ -870,41 +1059,41 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
                 // 16 B row offsets per frontier vertex; per frontier edge and
                 // pass (two passes) 4 B column + 8 B frontier word + 8 B
                 // visited word + 8 B next word; 16 B visited read-modify-write
-                // per newly reached vertex; 4 B parent init + 4 B parent + 2 B
-                // level per new (vertex, root) pair
+                // per newly reached vertex; 4 B parent init + 4 B parent per
+                // new (vertex, root) pair
                 float kms = 0;
                 G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
                 const double E = (double)m->h_cnt[kFqEdges];
                 m->k_ms[kPushUnit] += kms;
                 m->k_launches[kPushUnit] += 1;
                 m->k_bytes[kPushUnit] += 16.0 * m->h_cnt[kFqLen] + 56.0 * E +
-                                         16.0 * m->h_cnt[4 * kPushUnit + kVisitedVerts] +
-                                         10.0 * m->h_cnt[4 * kPushUnit + kNewBits];
+                                         16.0 * h[kUnitSlots * kPushUnit + kVisitedVerts] +
+                                         8.0 * h[kUnitSlots * kPushUnit + kNewBits];
                 rec[kPushUnit] = kms;
                 rec[4 + kPushUnit] = E;
             } else {
                 // per bin: 16 B row offsets + 8 B visited read + 8 B next
-                // write per vertex with an unvisited lane; 4 B column + 1 bit
-                // of the active bitmap per edge read; 8 B per frontier word
-                // gathered; 4 B parent + 2 B level per new (vertex, root) pair
+                // write per vertex with an unvisited lane; 4 B column (+ 1 bit
+                // of the active bitmap when checked) per edge read; 8 B per
+                // frontier word gathered; 4 B parent per new (vertex, root) pair
                 for (int k = 0; k < kBins; ++k) {
                     if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
                     float kms = 0;
                     G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
                     m->k_ms[k] += kms;
                     m->k_launches[k] += 1;
-                    m->k_bytes[k] += 32.0 * m->h_cnt[4 * k + kVisitedVerts] + 4.125 * m->h_cnt[4 * k + kEdgesRead] +
-                                     8.0 * m->h_cnt[4 * k + kGathers] + 6.0 * m->h_cnt[4 * k + kNewBits];
+                    m->k_bytes[k] += 32.0 * h[kUnitSlots * k + kVisitedVerts] +
+                                     (act_check ? 4.125 : 4.0) * h[kUnitSlots * k + kEdgesRead] +
+                                     8.0 * h[kUnitSlots * k + kGathers] + 4.0 * h[kUnitSlots * k + kNewBits];
                     rec[k] = kms;
-                    rec[4 + k] = (double)m->h_cnt[4 * k + kEdgesRead];
+                    rec[4 + k] = (double)h[kUnitSlots * k + kEdgesRead];
                 }
             }
             m->lvl.insert(m->lvl.end(), rec, rec + kLevelRec);
         }
         if (new_bits == 0) break;
-        G5_REQUIRE(levels < 65535, G5_E_STATE, "g5_msbfs_run: more than 65534 levels (uint16 depth limit)");
-        act_verts = m->h_cnt[kActVerts];
-        act_edges = m->h_cnt[kActEdges];
+        act_verts = next_verts;
+        act_edges = next_edges;
         ++levels;
         p ^= 1;
     }
@@ -918,22 +1107,55 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
 extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level)
 {
     G5_REQUIRE(m && lane >= 0 && lane < m->nroots, G5_E_ARG, "g5_msbfs_result: lane out of range");
+    const int64_t n = m->n;
+    hipStream_t s = m->s;
     int64_t *dp = nullptr;
+    uint64_t *dw = nullptr;
     int32_t *dl = nullptr;
-    G5_HIP(hipMalloc(&dp, (size_t)m->n * 8));
-    hipError_t e = hipMalloc(&dl, (size_t)m->n * 4);
+    int *dflag = nullptr;
+    auto cleanup = [&]() {
+        if (dp) (void)hipFree(dp);
+        if (dw) (void)hipFree(dw);
+        if (dl) (void)hipFree(dl);
+        if (dflag) (void)hipFree(dflag);
+    };
+    hipError_t e = hipMalloc(&dp, (size_t)n * 8);
+    if (e == hipSuccess && level) e = hipMalloc(&dw, (size_t)n * 8);
+    if (e == hipSuccess && level) e = hipMalloc(&dl, (size_t)n * 4);
+    if (e == hipSuccess && level) e = hipMalloc(&dflag, sizeof(int));
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s,
-                           (const uint32_t *)m->rl.old2new, (const int32_t *)m->d_parent,
-                           (const uint16_t *)m->d_level, (const uint64_t *)m->d_visited, m->n, lane, dp, dl);
+        hipLaunchKernelGGL(k_ms_extract, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint32_t *)m->rl.old2new,
+                           (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, n, lane, dp);
         e = hipGetLastError();
     }
-    if (e == hipSuccess && pred) e = hipMemcpyAsync(pred, dp, (size_t)m->n * 8, hipMemcpyDeviceToHost, m->s);
-    if (e == hipSuccess && level) e = hipMemcpyAsync(level, dl, (size_t)m->n * 4, hipMemcpyDeviceToHost, m->s);
-    if (e == hipSuccess) e = hipStreamSynchronize(m->s);
-    (void)hipFree(dp);
-    if (dl) (void)hipFree(dl);
+    if (e == hipSuccess && pred) e = hipMemcpyAsync(pred, dp, (size_t)n * 8, hipMemcpyDeviceToHost, s);
+    bool converged = level == nullptr;
+    if (e == hipSuccess && level) {
+        hipLaunchKernelGGL(k_jump_init, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)dp, n, dw);
+        e = hipGetLastError();
+        // each pass at least doubles every vertex's jump: 40 passes cover
+        // any depth below 2^32; a parent cycle never converges
+        for (int pass = 0; pass < 40 && e == hipSuccess && !converged; ++pass) {
+            int changed = 0;
+            e = g5::fill_i32(dflag, 1, 0, s);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_jump, g5::grid_for(n, 256, 16384), 256, 0, s, dw, n, dflag);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(&changed, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            converged = changed == 0;
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_jump_levels, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint64_t *)dw, n, dl);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(level, dl, (size_t)n * 4, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
     G5_HIP(e);
+    G5_REQUIRE(converged, G5_E_STATE, "g5_msbfs_result: parent tree of lane %d has a cycle", lane);
     return G5_OK;
 }
 

@@ -119,13 +119,15 @@ int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);
  * beside the single-root plug-in; results equal nroots separate searches
  * with the minimum-id parent rule of cpu_2d/cpubfs_bin.cpp:150-173) ----
  * Needs a diagonal (1x1) block.  Up to 64 roots per run, one uint64 lane mask
- * per vertex; parents and levels are kept as int32[n][64] in HBM. */
+ * per vertex; parents are kept as int32[n][64] in HBM (levels are derived
+ * from the parent tree on extraction, as validate/validate.h:107-232 does). */
 int g5_msbfs_create(g5_graph *g, g5_msbfs **out);
 int g5_msbfs_free(g5_msbfs *m);
 /* Searches all roots; *nlevels = levels of the deepest search, *ms = device time. */
 int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32_t *nlevels, float *ms);
 /* Root `lane`'s result as the single-root search returns it: pred int64[n]
- * (-1 unreached, root -> root) and level int32[n]; either may be NULL. */
+ * (-1 unreached, root -> root) and level int32[n] (-1 unreached); either may
+ * be NULL.  Allocates device scratch; G5_E_STATE if the tree has a cycle. */
 int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level);
 /* TEPS numerators nedge[nroots] of the last run (validate/validate.h:439-485). */
 int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);

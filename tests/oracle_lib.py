@@ -64,6 +64,8 @@ def build_csr(edges, n_rows=None):
     nglobal = L.orc_num_vertices(ptr(e), c_i64(m))
     if n_rows is None:
         n_rows = L.orc_padded_rows(c_i64(nglobal))
+    if n_rows < nglobal:
+        raise ValueError(f"build_csr: n_rows {n_rows} < {nglobal} vertices")
     rowptr = np.zeros(n_rows + 1, np.int64)
     cols = np.zeros(max(2 * m, 1), np.uint32)
     gdeg = np.zeros(n_rows, np.int64)
