@@ -53,6 +53,18 @@ def lib():
     _sig(L, "g5_memcpy_d2h", I, [P, P, c_sz])
     _sig(L, "g5_graph_upload", I, [P, P, c_i64, c_i64, P, c_i64, c_i64, ctypes.POINTER(c_vp)])
     _sig(L, "g5_graph_kronecker", I, [I, I, c_u64, c_u64, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_graph_kronecker_2d", I, [I, I, c_u64, c_u64, I, I, P])
+    _sig(L, "g5_block_bounds", I, [c_i64, I, I, I, I, P, P, P, P])
+    _sig(L, "g5_graph_bounds", I, [P, P, P])
+    _sig(L, "g5_graph_gdeg", I, [P, P])
+    _sig(L, "g5_memcpy_d2d", I, [P, P, c_sz])
+    _sig(L, "g5_fill_u64", I, [P, c_i64, c_u64])
+    _sig(L, "g5_bitmap_to_queue", I, [P, c_i64, P, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_queue_to_bitmap", I, [P, c_i64, P, c_i64])
+    _sig(L, "g5_bfs_out_bitmap", I, [P, ctypes.POINTER(c_vp), ctypes.POINTER(c_i64)])
+    _sig(L, "g5_bfs_set_out_bitmap", I, [P, P])
+    _sig(L, "g5_bfs_set_in_bitmap", I, [P, P])
+    _sig(L, "g5_bfs_pred_device", I, [P, P])
     _sig(L, "g5_graph_info", I, [P, P, P, P, P, P])
     _sig(L, "g5_graph_download", I, [P, P, P, P])
     _sig(L, "g5_graph_free", I, [P])
@@ -152,12 +164,22 @@ class Graph:
         info = [c_i64() for _ in range(5)]
         check(lib().g5_graph_info(self.h, *[ctypes.byref(x) for x in info]))
         self.n_rows, self.n_cols, self.nnz, self.nglobal, self.n_gen_edges = [x.value for x in info]
+        rs, cs = c_i64(), c_i64()
+        check(lib().g5_graph_bounds(self.h, ctypes.byref(rs), ctypes.byref(cs)))
+        self.row_start, self.col_start = rs.value, cs.value
 
     @classmethod
     def kronecker(cls, scale, edgefactor=16, seed1=1, seed2=2):
         h = c_vp()
         check(lib().g5_graph_kronecker(scale, edgefactor, seed1, seed2, ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def kronecker_2d(cls, scale, R, C, edgefactor=16, seed1=1, seed2=2):
+        """The R x C blocks of the graph, as a list indexed [r * C + c]."""
+        hs = (c_vp * (R * C))()
+        check(lib().g5_graph_kronecker_2d(scale, edgefactor, seed1, seed2, R, C, hs))
+        return [cls(c_vp(h)) for h in hs]
 
     @classmethod
     def upload(cls, rowptr, cols, n_rows, n_cols=None, gdeg=None, row_start=0, col_start=0):
@@ -193,6 +215,13 @@ class Graph:
             self.free()
         except Exception:
             pass
+
+
+def block_bounds(nglobal, R, C, r, c):
+    """(row_start, n_rows, col_start, n_cols) of block (r, c) (host only)."""
+    out = [c_i64() for _ in range(4)]
+    check(lib().g5_block_bounds(nglobal, R, C, r, c, *[ctypes.byref(x) for x in out]))
+    return tuple(x.value for x in out)
 
 
 def kronecker_edges(scale, start, end, seed1=1, seed2=2):

@@ -56,7 +56,27 @@ def build(verbose=False, jobs=None):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+DRIVER = os.path.join(HERE, "g500_hip")
+HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), "-I" + HOST]
+
+
+def host_sources():
+    return sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+
+
+def build_host():
+    """The C++ host side (HipBFS, the emulated 2D grid, the g500_hip driver):
+    plain g++ against the C ABI of libg5hip.so, found next to the binary."""
+    tmp = DRIVER + ".tmp"
+    _run([os.environ.get("CXX", "g++")] + HOST_FLAGS + host_sources() +
+         ["-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

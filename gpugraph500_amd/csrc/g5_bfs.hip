@@ -420,6 +420,53 @@ extern "C" int g5_bfs_result(g5_bfs *b, int64_t *pred, int32_t *level)
     return G5_OK;
 }
 
+extern "C" int g5_bfs_out_bitmap(g5_bfs *b, const uint64_t **d_bits, int64_t *n_words)
+{
+    G5_REQUIRE(b && d_bits && n_words, G5_E_ARG, "g5_bfs_out_bitmap: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_out_bitmap: g5_bfs_start not called");
+    *d_bits = b->d_out;
+    *n_words = b->wc;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits)
+{
+    G5_REQUIRE(b, G5_E_ARG, "g5_bfs_set_out_bitmap: null handle");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_out_bitmap: g5_bfs_start not called");
+    if (d_bits && d_bits != b->d_out && b->wc)
+        G5_HIP(hipMemcpyAsync(b->d_out, d_bits, (size_t)b->wc * 8, hipMemcpyDeviceToDevice, b->s));
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
+    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
+                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits)
+{
+    G5_REQUIRE(b && d_bits, G5_E_ARG, "g5_bfs_set_in_bitmap: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_in_bitmap: g5_bfs_start not called");
+    G5_HIP(g5::bitmap_to_queue(d_bits, b->wr, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
+    int rc = sync_counters(b);
+    if (rc) return rc;
+    b->qlen = b->h_cnt[kQLen];
+    b->depth++;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred)
+{
+    G5_REQUIRE(b && d_pred, G5_E_ARG, "g5_bfs_pred_device: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_pred_device: no search");
+    g5_graph *g = b->g;
+    hipLaunchKernelGGL(k_result, g5::grid_for(g->n_cols, 256, 8192), 256, 0, b->s, (const uint32_t *)b->d_parent,
+                       g->n_cols, g->row_start, g->col_start, d_pred);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
 extern "C" int g5_bfs_nedge(g5_bfs *b, int64_t *nedge)
 {
     G5_REQUIRE(b && nedge, G5_E_ARG, "g5_bfs_nedge: null argument");

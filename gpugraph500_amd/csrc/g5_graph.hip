@@ -377,6 +377,178 @@ int build_kronecker(int scale, int edgefactor, uint64_t s1, uint64_t s2, g5_grap
     return G5_OK;
 }
 
+// ---- the R x C blocks of the same graph ----
+
+// Bounds of part i of `parts` over n_pad vertices in 64-vertex units
+// (DistMatrix2d::setupMatrix2, cpu_2d/distmatrix2d.hh:811-815, with ALG = 64).
+void part_range(int64_t n_pad, int parts, int i, int64_t *start, int64_t *len)
+{
+    const int64_t units = (n_pad + 63) / 64;
+    *start = ((int64_t)i * (units / parts) + std::min<int64_t>(i, units % parts)) * 64;
+    *len = (units / parts + (i < units % parts ? 1 : 0)) * 64;
+}
+
+// first index in [b, e) whose column (key & mask) is >= c (columns ascend in a row)
+__device__ __forceinline__ int64_t col_lower_bound(const uint64_t *__restrict__ keys, int64_t b, int64_t e,
+                                                   uint64_t mask, uint64_t c)
+{
+    while (b < e) {
+        const int64_t mid = (b + e) >> 1;
+        if ((keys[mid] & mask) < c) b = mid + 1;
+        else e = mid;
+    }
+    return b;
+}
+
+// Per local row i: the entries of global row rs + i with a column in
+// [cs, ce).  Counts go to cnt64 (nullable) and/or cnt32 (nullable).
+__global__ void k_block_counts(const uint64_t *__restrict__ keys, const int64_t *__restrict__ pos, int64_t rs,
+                               int64_t rl, uint64_t cs, uint64_t ce, uint64_t mask, int64_t *__restrict__ cnt64,
+                               uint32_t *__restrict__ cnt32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rl;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = pos[rs + i], e = pos[rs + i + 1];
+        const int64_t n = col_lower_bound(keys, b, e, mask, ce) - col_lower_bound(keys, b, e, mask, cs);
+        if (cnt64) cnt64[i] = n;
+        if (cnt32) cnt32[i] = (uint32_t)n;
+    }
+}
+
+// one wavefront per local row: copy the row's in-block columns as local ids
+__global__ __launch_bounds__(256) void k_block_fill(const uint64_t *__restrict__ keys, const int64_t *__restrict__ pos,
+                                                    int64_t rs, int64_t rl, uint64_t cs, uint64_t ce, uint64_t mask,
+                                                    const int64_t *__restrict__ rowptr, uint32_t *__restrict__ cols)
+{
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < rl; i += waves) {
+        const int64_t b = pos[rs + i], e = pos[rs + i + 1];
+        const int64_t lo = col_lower_bound(keys, b, e, mask, cs);
+        const int64_t n = rowptr[i + 1] - rowptr[i];
+        for (int64_t k = __lane_id(); k < n; k += 64) cols[rowptr[i] + k] = (uint32_t)((keys[lo + k] & mask) - cs);
+    }
+}
+
+// All R x C blocks of the Kronecker graph, as the R x C ranks of the
+// reference hold them: block (r, c) keeps the symmetrised entries with row in
+// row part r and column in column part c, deduplicated without self-loops
+// (distmatrix2d.hh:1033-1165), local column ids (WOLO); gdeg counts the
+// block's generated entries (duplicates and self-loops included), so a row's
+// generated degree is the sum over its row of blocks.  The vertex count is
+// padded to a multiple of R (the CUDA path's PAD, cuda/cuda_bfs.h:40).
+int build_kronecker_2d(int scale, int edgefactor, uint64_t s1, uint64_t s2, int R, int C, g5_graph **blocks)
+{
+    hipStream_t s = nullptr;
+    int dev = 0;
+    G5_HIP(hipGetDevice(&dev));
+    int rc = g_skip.init(dev);
+    if (rc) return rc;
+    const KronSetup ks = kron_setup(s1, s2);
+    const int64_t m = (int64_t)edgefactor << scale;
+    const int64_t n2 = 2 * m;
+    const int sh = scale < 1 ? 1 : scale;
+    const uint64_t mask = (1ull << sh) - 1;
+    DevBuf ka, kb, tmp, maxid, cnt, pos, deg;
+    G5_HIP(hipMalloc(&ka.p, (size_t)n2 * 8));
+    G5_HIP(hipMalloc(&kb.p, (size_t)n2 * 8));
+    G5_HIP(hipMalloc(&maxid.p, 16));
+    G5_HIP(hipMemset(maxid.p, 0, 16));
+    uint64_t *A = (uint64_t *)ka.p, *B = (uint64_t *)kb.p;
+    hipLaunchKernelGGL(k_gen_keys, g5::grid_for(m, 256, 65536), 256, 0, s, m, (const uint32_t *)g_skip.d, ks, scale,
+                       sh, A, (unsigned long long *)maxid.p);
+    G5_HIP(hipGetLastError());
+    unsigned long long h_max = 0;
+    G5_HIP(hipMemcpy(&h_max, maxid.p, 8, hipMemcpyDeviceToHost));
+    const int64_t nglobal = (int64_t)h_max + 1;
+    const int64_t n_pad = nglobal + (nglobal % R ? R - nglobal % R : 0);
+    int64_t rs[64], rl[64], cs[64], cl[64];
+    for (int r = 0; r < R; ++r) part_range(n_pad, R, r, &rs[r], &rl[r]);
+    for (int c = 0; c < C; ++c) part_range(n_pad, C, c, &cs[c], &cl[c]);
+    const int64_t rows_total = rs[R - 1] + rl[R - 1];
+
+    rocprim::double_buffer<uint64_t> db(A, B);
+    size_t tb = 0;
+    G5_HIP(rocprim::radix_sort_keys(nullptr, tb, db, (size_t)n2, 0u, (unsigned)(2 * sh), s));
+    G5_HIP(hipMalloc(&tmp.p, tb ? tb : 1));
+    G5_HIP(rocprim::radix_sort_keys(tmp.p, tb, db, (size_t)n2, 0u, (unsigned)(2 * sh), s));
+    uint64_t *sorted = db.current(), *other = db.alternate();
+    G5_HIP(hipMalloc(&pos.p, (size_t)(rows_total + 1) * 8));
+    int64_t *d_pos = (int64_t *)pos.p;
+
+    for (int i = 0; i < R * C; ++i) {
+        const int r = i / C, c = i % C;
+        g5_graph *g = new g5_graph();
+        blocks[i] = g;
+        g->device = dev;
+        g->scale = scale;
+        g->n_rows = rl[r];
+        g->n_cols = cl[c];
+        g->row_start = rs[r];
+        g->col_start = cs[c];
+        g->nglobal = nglobal;
+        g->n_gen_edges = m;
+        G5_HIP(hipMalloc(&g->d_rowptr, (size_t)(g->n_rows + 1) * 8));
+        G5_HIP(hipMalloc(&g->d_gdeg, (size_t)std::max<int64_t>(g->n_rows, 1) * 4));
+    }
+    // generated degrees per block from the full list
+    hipLaunchKernelGGL(k_row_bounds, g5::grid_for(n2 + 1, 256, 65536), 256, 0, s, (const uint64_t *)sorted, n2, sh,
+                       rows_total, d_pos);
+    for (int i = 0; i < R * C; ++i) {
+        g5_graph *g = blocks[i];
+        hipLaunchKernelGGL(k_block_counts, g5::grid_for(g->n_rows, 256, 65536), 256, 0, s, (const uint64_t *)sorted,
+                           (const int64_t *)d_pos, g->row_start, g->n_rows, (uint64_t)g->col_start,
+                           (uint64_t)(g->col_start + g->n_cols), mask, (int64_t *)nullptr, g->d_gdeg);
+    }
+    G5_HIP(hipGetLastError());
+
+    // unique, then drop self-loops
+    G5_HIP(hipMalloc(&cnt.p, 16));
+    size_t tu = 0, tsel = 0;
+    G5_HIP(rocprim::unique(nullptr, tu, sorted, other, (size_t *)cnt.p, (size_t)n2, rocprim::equal_to<uint64_t>(), s));
+    G5_HIP(rocprim::select(nullptr, tsel, other, sorted, (size_t *)cnt.p + 1, (size_t)n2, NotSelfLoop{sh, mask}, s));
+    if (tu > tb || tsel > tb) {
+        (void)hipFree(tmp.p);
+        tmp.p = nullptr;
+        G5_HIP(hipMalloc(&tmp.p, std::max(tu, tsel)));
+    }
+    G5_HIP(rocprim::unique(tmp.p, tu, sorted, other, (size_t *)cnt.p, (size_t)n2, rocprim::equal_to<uint64_t>(), s));
+    size_t h_cnt[2] = {0, 0};
+    G5_HIP(hipMemcpy(h_cnt, cnt.p, 8, hipMemcpyDeviceToHost));
+    G5_HIP(rocprim::select(tmp.p, tsel, other, sorted, (size_t *)cnt.p + 1, h_cnt[0], NotSelfLoop{sh, mask}, s));
+    G5_HIP(hipMemcpy(h_cnt, cnt.p, 16, hipMemcpyDeviceToHost));
+    const int64_t nnz_all = (int64_t)h_cnt[1];
+    hipLaunchKernelGGL(k_row_bounds, g5::grid_for(nnz_all + 1, 256, 65536), 256, 0, s, (const uint64_t *)sorted,
+                       nnz_all, sh, rows_total, d_pos);
+    G5_HIP(hipGetLastError());
+
+    // per block: row lengths -> row offsets -> columns
+    int64_t max_rows = 0;
+    for (int r = 0; r < R; ++r) max_rows = std::max(max_rows, rl[r]);
+    G5_HIP(hipMalloc(&deg.p, (size_t)(max_rows + 1) * 8));
+    DevBuf scan;
+    G5_HIP(hipMalloc(&scan.p, (g5::scan_scratch_words(max_rows + 1) + 16) * 8));
+    for (int i = 0; i < R * C; ++i) {
+        g5_graph *g = blocks[i];
+        hipLaunchKernelGGL(k_block_counts, g5::grid_for(g->n_rows, 256, 65536), 256, 0, s, (const uint64_t *)sorted,
+                           (const int64_t *)d_pos, g->row_start, g->n_rows, (uint64_t)g->col_start,
+                           (uint64_t)(g->col_start + g->n_cols), mask, (int64_t *)deg.p, (uint32_t *)nullptr);
+        G5_HIP(hipGetLastError());
+        G5_HIP(g5::exclusive_scan_i64((int64_t *)deg.p, g->n_rows, (int64_t *)deg.p + g->n_rows, (int64_t *)scan.p, s));
+        G5_HIP(hipMemcpyAsync(g->d_rowptr, deg.p, (size_t)(g->n_rows + 1) * 8, hipMemcpyDeviceToDevice, s));
+        int64_t nnz = 0;
+        G5_HIP(hipMemcpy(&nnz, (int64_t *)deg.p + g->n_rows, 8, hipMemcpyDeviceToHost));
+        g->nnz = nnz;
+        G5_HIP(hipMalloc(&g->d_cols, (size_t)std::max<int64_t>(nnz, 1) * 4));
+        hipLaunchKernelGGL(k_block_fill, g5::grid_for((g->n_rows + 3) / 4, 1, 65536), 256, 0, s,
+                           (const uint64_t *)sorted, (const int64_t *)d_pos, g->row_start, g->n_rows,
+                           (uint64_t)g->col_start, (uint64_t)(g->col_start + g->n_cols), mask,
+                           (const int64_t *)g->d_rowptr, g->d_cols);
+        G5_HIP(hipGetLastError());
+    }
+    G5_HIP(hipDeviceSynchronize());
+    return G5_OK;
+}
+
 }  // namespace
 
 namespace g5 {
@@ -462,6 +634,52 @@ extern "C" int g5_graph_kronecker(int scale, int edgefactor, uint64_t seed1, uin
         return rc;
     }
     *out = g;
+    return G5_OK;
+}
+
+extern "C" int g5_block_bounds(int64_t nglobal, int R, int C, int r, int c, int64_t *row_start, int64_t *n_rows,
+                               int64_t *col_start, int64_t *n_cols)
+{
+    G5_REQUIRE(nglobal >= 0 && R >= 1 && C >= 1 && R <= 64 && C <= 64 && r >= 0 && r < R && c >= 0 && c < C &&
+                   row_start && n_rows && col_start && n_cols,
+               G5_E_ARG, "g5_block_bounds: bad arguments");
+    const int64_t n_pad = nglobal + (nglobal % R ? R - nglobal % R : 0);
+    part_range(n_pad, R, r, row_start, n_rows);
+    part_range(n_pad, C, c, col_start, n_cols);
+    return G5_OK;
+}
+
+extern "C" int g5_graph_kronecker_2d(int scale, int edgefactor, uint64_t seed1, uint64_t seed2, int R, int C,
+                                     g5_graph **blocks)
+{
+    G5_REQUIRE(blocks && scale >= 1 && scale <= 30 && edgefactor >= 1 && edgefactor <= 1024 && R >= 1 && C >= 1 &&
+                   R <= 64 && C <= 64,
+               G5_E_ARG, "g5_graph_kronecker_2d: scale in [1,30], edgefactor in [1,1024], 1 <= R, C <= 64");
+    for (int i = 0; i < R * C; ++i) blocks[i] = nullptr;
+    int rc = build_kronecker_2d(scale, edgefactor, seed1, seed2, R, C, blocks);
+    if (rc) {
+        for (int i = 0; i < R * C; ++i) {
+            g5_graph_free(blocks[i]);
+            blocks[i] = nullptr;
+        }
+    }
+    return rc;
+}
+
+extern "C" int g5_graph_gdeg(const g5_graph *g, int64_t *gdeg)
+{
+    G5_REQUIRE(g && gdeg, G5_E_ARG, "g5_graph_gdeg: null argument");
+    std::vector<uint32_t> t((size_t)g->n_rows);
+    if (g->n_rows) G5_HIP(hipMemcpy(t.data(), g->d_gdeg, (size_t)g->n_rows * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < g->n_rows; ++i) gdeg[i] = t[i];
+    return G5_OK;
+}
+
+extern "C" int g5_graph_bounds(const g5_graph *g, int64_t *row_start, int64_t *col_start)
+{
+    G5_REQUIRE(g && row_start && col_start, G5_E_ARG, "g5_graph_bounds: null argument");
+    *row_start = g->row_start;
+    *col_start = g->col_start;
     return G5_OK;
 }
 

@@ -267,3 +267,71 @@ extern "C" int g5_memcpy_d2h(void *dst, const void *src, size_t bytes)
     if (bytes) G5_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return G5_OK;
 }
+
+extern "C" int g5_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes)
+{
+    if (bytes) G5_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+    return G5_OK;
+}
+
+extern "C" int g5_fill_u64(uint64_t *d, int64_t n, uint64_t value)
+{
+    G5_REQUIRE(n >= 0 && (d || n == 0), G5_E_ARG, "g5_fill_u64: bad arguments");
+    G5_HIP(g5::fill_u64(d, n, value, nullptr));
+    G5_HIP(hipStreamSynchronize(nullptr));
+    return G5_OK;
+}
+
+extern "C" int g5_bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint32_t *d_queue, int64_t *n)
+{
+    G5_REQUIRE(n_words >= 0 && n && (n_words == 0 || (d_bits && d_queue)), G5_E_ARG,
+               "g5_bitmap_to_queue: bad arguments");
+    int64_t *d_tmp = nullptr;
+    const size_t words = g5::scan_scratch_words((n_words + 255) / 256 + 1) + (n_words + 255) / 256 + 8;
+    G5_HIP(hipMalloc(&d_tmp, words * 8));
+    hipError_t e = g5::bitmap_to_queue(d_bits, n_words, d_queue, d_tmp, d_tmp + 1, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(n, d_tmp, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d_tmp);
+    G5_HIP(e);
+    return G5_OK;
+}
+
+namespace {
+// ids at or beyond n_words * 64 are counted in *bad and skipped
+__global__ void k_q2bm_checked(const uint32_t *__restrict__ q, int64_t n, uint64_t *__restrict__ bits,
+                               int64_t n_words, unsigned long long *__restrict__ bad)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = q[i];
+        if ((int64_t)(v >> 6) >= n_words) {
+            atomicAdd(bad, 1ull);
+            continue;
+        }
+        atomicOr((unsigned long long *)&bits[v >> 6], 1ull << (v & 63));
+    }
+}
+}  // namespace
+
+extern "C" int g5_queue_to_bitmap(const uint32_t *d_queue, int64_t n, uint64_t *d_bits, int64_t n_words)
+{
+    G5_REQUIRE(n >= 0 && n_words >= 0 && (n == 0 || d_queue) && (n_words == 0 || d_bits), G5_E_ARG,
+               "g5_queue_to_bitmap: bad arguments");
+    G5_HIP(g5::fill_u64(d_bits, n_words, 0, nullptr));
+    if (n == 0) {
+        G5_HIP(hipStreamSynchronize(nullptr));
+        return G5_OK;
+    }
+    unsigned long long *d_bad = nullptr, h_bad = 0;
+    G5_HIP(hipMalloc(&d_bad, 8));
+    hipError_t e = hipMemset(d_bad, 0, 8);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_q2bm_checked, g5::grid_for(n, 256, 16384), 256, 0, nullptr, d_queue, n, d_bits, n_words,
+                           d_bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h_bad, d_bad, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d_bad);
+    G5_HIP(e);
+    G5_REQUIRE(h_bad == 0, G5_E_ARG, "g5_queue_to_bitmap: %llu ids beyond %lld words", h_bad, (long long)n_words);
+    return G5_OK;
+}

@@ -49,6 +49,14 @@ int g5_dmalloc(void **d_ptr, size_t bytes);
 int g5_dfree(void *d_ptr);
 int g5_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
 int g5_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
+int g5_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+int g5_fill_u64(uint64_t *d, int64_t n, uint64_t value);
+/* Device bitmap <-> sorted queue (the frontier representations the 2D
+ * exchange moves; cuda/cuda_bfs.cu:380-475 sorts and uniques on the host).
+ * d_queue needs room for every set bit (n_words * 64 at most); *n = length.
+ * g5_queue_to_bitmap clears d_bits first and rejects ids >= n_words * 64. */
+int g5_bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint32_t *d_queue, int64_t *n);
+int g5_queue_to_bitmap(const uint32_t *d_queue, int64_t n, uint64_t *d_bits, int64_t n_words);
 
 /* ---- the DistMatrix2d block (cpu_2d/distmatrix2d.hh:30-130) ----
  * Upload a host CSR block: rowptr int64[n_rows+1], cols uint32[nnz] local
@@ -63,8 +71,25 @@ int g5_graph_upload(const int64_t *rowptr, const uint32_t *cols, int64_t n_rows,
  * setupMatrix2's rules (cpu_2d/distmatrix2d.hh:745-1166): sorted, deduplicated
  * columns, self-loops dropped, rows padded to 64. */
 int g5_graph_kronecker(int scale, int edgefactor, uint64_t seed1, uint64_t seed2, g5_graph **out);
+/* The R x C blocks of the same graph as R x C ranks hold them (setupMatrix2
+ * at an R x C grid, distmatrix2d.hh:745-1166, with the CUDA path's padding of
+ * the vertex count to a multiple of R, cuda/cuda_bfs.h:40): blocks[r*C + c]
+ * keeps the symmetrised entries with row in row part r and column in column
+ * part c, local ids; gdeg counts the block's generated entries, so a row's
+ * generated degree is the sum over its row of blocks.  1 <= R, C <= 64. */
+int g5_graph_kronecker_2d(int scale, int edgefactor, uint64_t seed1, uint64_t seed2, int R, int C,
+                          g5_graph **blocks);
+/* Block (r, c)'s bounds for a global vertex count: parts in 64-vertex units,
+ * the first (units mod parts) parts one unit longer (distmatrix2d.hh:807-815
+ * with ALG = 64), after padding nglobal to a multiple of R.  Host only. */
+int g5_block_bounds(int64_t nglobal, int R, int C, int r, int c, int64_t *row_start, int64_t *n_rows,
+                    int64_t *col_start, int64_t *n_cols);
 int g5_graph_info(const g5_graph *g, int64_t *n_rows, int64_t *n_cols, int64_t *nnz,
                   int64_t *nglobal, int64_t *n_gen_edges);
+/* global ids of the block's local row 0 and column 0 */
+int g5_graph_bounds(const g5_graph *g, int64_t *row_start, int64_t *col_start);
+/* generated directed degree per local row, gdeg[n_rows] */
+int g5_graph_gdeg(const g5_graph *g, int64_t *gdeg);
 int g5_graph_download(const g5_graph *g, int64_t *rowptr, uint32_t *cols, int64_t *gdeg);
 int g5_graph_free(g5_graph *g);
 /* Edges [start, end) of the Kronecker list generated on the device and copied
@@ -99,6 +124,22 @@ int g5_bfs_set_inqueue(g5_bfs *b, const uint32_t *q, int64_t n);
  * (-1 unreached, root -> root), level int32[n_cols] (-1 unreached); either
  * pointer may be NULL. */
 int g5_bfs_result(g5_bfs *b, int64_t *pred, int32_t *level);
+/* The same hooks with the frontiers left in HBM, for an exchange that moves
+ * device bitmaps (as RCCL would between GPUs) instead of host queues:
+ *   g5_bfs_out_bitmap      the column out-bitmap of the last expand (n_cols/64
+ *                          words, borrowed, valid until the next call on b);
+ *                          the device side of getOutgoingFQ (cuda_bfs.cu:185-228)
+ *   g5_bfs_set_out_bitmap  setModOutgoingFQ from a column-reduced device
+ *                          bitmap (NULL = keep the own out-bitmap)
+ *   g5_bfs_set_in_bitmap   setIncommingFQ + setBackInqueue from a device bitmap
+ *                          over the row slice (n_rows/64 words)
+ *   g5_bfs_pred_device     getBackPredecessor into device memory: int64[n_cols]
+ *                          global ids, -1 where this block found no parent,
+ *                          root -> root; combine blocks with g5_vreduce_min_pred */
+int g5_bfs_out_bitmap(g5_bfs *b, const uint64_t **d_bits, int64_t *n_words);
+int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits);
+int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits);
+int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred);
 /* The whole 1x1 runBFS (globalbfs.hh:548-1263) resident in HBM: no host
  * round trip per level.  flags: G5_RUN_CODEC routes every level's queue
  * through s4-bp128-d4 encode -> decode on the device, as the row broadcast
