@@ -1,25 +1,40 @@
 #!/usr/bin/env python3
 """Headline benchmark: Graph500 BFS on one MI355X (gfx950).
 
-Workload (scale 26 by default): the Graph500 Kronecker graph (A=.57, B=C=.19,
-edgefactor 16, seeds 1 and 2) generated and built into the 1x1 CSR block on
-the device; 64 search keys chosen as cpu_2d/main.cpp:351-411 does.
+Workload: the Graph500 Kronecker graph (A=.57, B=C=.19, edgefactor 16, seeds
+1 and 2) generated and built on the device; 64 search keys chosen as
+cpu_2d/main.cpp:351-411 does.  TEPS_r = nedge_r / t_r with nedge from
+validate/validate.h:439-485; the reported value is the harmonic mean over
+roots (cpu_2d/main.cpp:944-950), in GTEPS.
 
-* single-root: every root searched on its own (g5_bfs_run, resident in HBM);
-  TEPS_r = nedge_r / t_r, t_r = device time of that search.
-* batched: the 64 roots searched together by the multi-source kernel
-  (g5_msbfs_run); TEPS_r = nedge_r / (T_batch / 64).
-A step is one batched search over the 64 roots.  The reported value is the
-harmonic mean of the batched TEPS_r over the roots (cpu_2d/main.cpp:944-950),
-in GTEPS; the single-root harmonic mean and the ratio are reported beside it.
+--config c4 (default), scale 26, 1x1 block
+    A step is one batched search of the 64 roots (g5_msbfs_run), inputs
+    resident in HBM; batched TEPS_r = nedge_r / (T_batch / 64).  The plain
+    single-root search (g5_bfs_run) of every root is measured beside it.
+--config c2, scale 22
+    A step is the 64 single-root searches with every level's queue through
+    s4-bp128-d4 encode -> decode on the device (G5_RUN_CODEC).
+--config c3, scale 26
+    One rank's share of a 4 x 4 grid: the g500_hip driver's emulated grid
+    (device exchange, codec on the fold), rank (0, 0)'s share timed.
 
-    python bench.py [--steps K] [--warmup W] [--scale S] [--roots R]
+roofline.traffic: HBM bytes per launch of the dominant kernel from two
+rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run as child processes before
+this process touches the GPU, each scaled by what the counter reads for a
+plain copy kernel of known size; null when rocprofv3 is unavailable.
+
+    python bench.py [--config c4|c2|c3] [--steps K] [--warmup W] [--scale S]
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import platform
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -29,6 +44,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GTEPS hmean, batched-64 vs single-root, scale 26, 1 MI355X; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+COPY_WORDS = 1 << 26          # copy-kernel calibration: 64 Mi uint64 words read and written
+TRAFFIC_KERNEL = "k_pull_wave"
+DEFAULT_SCALE = {"c4": 26, "c2": 22, "c3": 26}
 
 
 def hmean(x):
@@ -51,12 +69,13 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(g, scale, root, nedge_gpu):
-    """The oracle's CPUBFS_bin restatement (single thread) on this host, one
-    root of the same graph: the bounded sample of the workload."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    return O
 
+
+def _host_csr(g):
     rp, cols, gdeg = g.download()
 
     class _G:
@@ -64,6 +83,14 @@ def cpu_baseline(g, scale, root, nedge_gpu):
 
     ref = _G()
     ref.rowptr, ref.cols, ref.n_rows, ref.gdeg = rp, cols, g.n_rows, gdeg
+    return ref
+
+
+def cpu_baseline(g, scale, root, nedge_gpu):
+    """The oracle's CPUBFS_bin restatement (single thread) on this host, one
+    root of the same graph: the bounded sample of the workload."""
+    O = _oracle()
+    ref = _host_csr(g)
     t0 = time.perf_counter()
     pred, lev, nl = O.bfs_bin(ref, int(root))
     t = time.perf_counter() - t0
@@ -79,36 +106,118 @@ def cpu_baseline(g, scale, root, nedge_gpu):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=26)
-    ap.add_argument("--roots", type=int, default=64)
-    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="WORDS",
-                    help="only run the plain copy kernel (k_or over one map) on WORDS uint64 words, "
-                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the 2D row/column exchange "
-                                   "is emulated on a single device (see DESIGN.md)",
-                          "n_gpus": args.gpus}))
-        return 2
+def cpu_baseline_codec(g, scale, roots, nedge_gpu, budget_s=12.0):
+    """Single-root search plus s4-bp128-d4 encode -> decode of every level's
+    sorted queue, both from the oracle, single thread, on as many roots as
+    fit the budget."""
+    O = _oracle()
+    ref = _host_csr(g)
+    teps, done, t_all = [], 0, 0.0
+    for k, root in enumerate(roots):
+        t0 = time.perf_counter()
+        pred, lev, nl = O.bfs_bin(ref, int(root))
+        for d in range(1, nl + 1):
+            q = np.flatnonzero(lev == d).astype(np.uint32)
+            if q.size:
+                w = O.s4_encode(q)
+                rc, back = O.s4_decode(w, q.size)
+                assert rc == 0 and np.array_equal(back, q)
+        t = time.perf_counter() - t0
+        ne = O.nedge(ref, lev)
+        assert int(ne) == int(nedge_gpu[k])
+        teps.append(ne / t)
+        done += 1
+        t_all += t
+        if t_all > budget_s:
+            break
+    return {
+        "value": hmean(teps) / 1e9,
+        "unit": "GTEPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} of {len(roots)} roots of the scale-{scale} graph, search + codec round trip of every "
+                  f"level queue, {t_all:.1f} s, single thread on {cpu_model()} (nproc {os.cpu_count()})",
+    }
 
+
+# ---------------------------------------------------------------- PMC traffic
+
+def pmc_probe(args):
+    """Run under rocprofv3 --pmc by measure_traffic: two batched searches and
+    the copy calibration kernel."""
     import gpugraph500_amd as g5
-
     g5.init(0)
-    if args.copy_calibration:
-        n = args.copy_calibration
-        src = g5.DeviceArray(n, np.uint64)
-        dst = g5.DeviceArray(n, np.uint64)
-        for _ in range(3):
-            g5.check(g5.lib().g5_vreduce_or(src.ptr, 1, n, dst.ptr))
-        print(json.dumps({"copy_calibration_words": n, "bytes_read": 8 * n, "bytes_written": 8 * n}))
-        return 0
+    g = g5.Graph.kronecker(args.scale)
+    roots = g.select_roots(1 << args.scale, args.roots)
+    m = g5.MSBFS(g)
+    m.run(roots)
+    m.run(roots)
+    m.free()
+    g.free()
+    src = g5.DeviceArray(COPY_WORDS, np.uint64)
+    dst = g5.DeviceArray(COPY_WORDS, np.uint64)
+    for _ in range(3):
+        g5.check(g5.lib().g5_vreduce_or(src.ptr, 1, COPY_WORDS, dst.ptr))
+    g5.check(g5.lib().g5_device_sync())
+    log("pmc probe done")
+    return 0
+
+
+def _pmc_pass(exe, counter, args):
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        cmd = [exe, "--pmc", counter, "--kernel-include-regex", f"{TRAFFIC_KERNEL}|k_or", "-d", d, "-o", "p",
+               "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-probe",
+               "--scale", str(args.scale), "--roots", str(args.roots)]
+        env = dict(os.environ, TMPDIR="/tmp")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}: {r.stdout[-400:]!r}")
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file")
+        kern, copy = [], []
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                if TRAFFIC_KERNEL in name:
+                    kern.append(v)
+                elif "k_or" in name:
+                    copy.append(v)
+        if not kern or not copy:
+            raise RuntimeError(f"rocprofv3 --pmc {counter}: kernels not found")
+        return float(np.mean(kern)), float(np.mean(copy)), len(kern)
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of TRAFFIC_KERNEL from FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled so that the copy kernel reads and writes
+    8 * COPY_WORDS bytes."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, {"error": "rocprofv3 not found"}
+    try:
+        t0 = time.perf_counter()
+        f_k, f_c, n_f = _pmc_pass(exe, "FETCH_SIZE", args)
+        w_k, w_c, n_w = _pmc_pass(exe, "WRITE_SIZE", args)
+    except (RuntimeError, subprocess.TimeoutExpired, OSError) as e:
+        return None, {"error": str(e)[:300]}
+    copy_bytes = 8.0 * COPY_WORDS
+    f_scale = copy_bytes / (f_c * 1024.0)
+    w_scale = copy_bytes / (w_c * 1024.0)
+    traffic = f_k * 1024.0 * f_scale + w_k * 1024.0 * w_scale
+    info = {"kernel": TRAFFIC_KERNEL, "fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k,
+            "fetch_scale": f_scale, "write_scale": w_scale, "launches_sampled": n_f,
+            "seconds": time.perf_counter() - t0}
+    log(f"traffic: {traffic / 1e9:.2f} GB per {TRAFFIC_KERNEL} launch (fetch x{f_scale:.2f}, write x{w_scale:.2f})")
+    return traffic, info
+
+
+# ---------------------------------------------------------------- configs
+
+def bench_c4(args, g5, traffic, traffic_info):
     t0 = time.perf_counter()
     g = g5.Graph.kronecker(args.scale)
     log(f"graph: scale {args.scale}, {g.nglobal} vertices, {g.nnz} nonzeros, "
@@ -131,7 +240,6 @@ def main():
     single_teps = nedge / (np.array(single_ms) / 1e3)
     single_h = hmean(single_teps) / 1e9
     log(f"single-root: hmean {single_h:.3f} GTEPS, mean {np.mean(single_ms):.2f} ms/root")
-
     expand_gbs = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
     single_kernel = {"kernel": "k_expand", "achieved_gbs": expand_gbs,
                      "avg_launch_ms": st["ms"] / max(st["launches"], 1),
@@ -156,9 +264,10 @@ def main():
     d = kst[dom]
     dom_gbs = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "avg_launch_ms": d["ms"] / max(d["launches"], 1),
+                "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic if dom == TRAFFIC_KERNEL else None,
+                "kernel": dom, "avg_launch_ms": d["ms"] / max(d["launches"], 1),
                 "algorithmic_bytes_per_launch": d["bytes"] / max(d["launches"], 1),
+                "traffic_source": traffic_info,
                 "all_kernels": {k: {"ms": v["ms"], "launches": v["launches"],
                                     "gbs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else 0.0}
                                 for k, v in kst.items()}}
@@ -181,8 +290,6 @@ def main():
     aggregate = float(ne_b.sum() / t_batch / 1e9)
     log(f"batched: {t_batch * 1e3:.1f} ms per 64-root search, hmean {value:.2f} GTEPS, "
         f"aggregate {aggregate:.2f} GTEPS, {nl} levels")
-    mode = "batched-64"
-
     out = {
         "metric": METRIC,
         "value": value,
@@ -196,7 +303,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 ids / u64 bitmaps (integer)",
         "data": "synthetic: Graph500 Kronecker (A=.57 B=C=.19), edgefactor 16, seeds (1,2), generated on the device",
-        "config": {"workload": f"{mode} BFS, 64 roots, Kronecker scale {args.scale}, 1x1 block",
+        "config": {"workload": f"batched-64 BFS, 64 roots, Kronecker scale {args.scale}, 1x1 block",
                    "scale": args.scale, "edgefactor": 16, "roots": int(len(roots)),
                    "vertices": int(g.nglobal), "nnz": int(g.nnz)},
         "single_root_hmean_gteps": single_h,
@@ -208,8 +315,143 @@ def main():
         "single_root_kernel": single_kernel,
         "roofline": roofline,
     }
+    m.free()
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(g, args.scale, roots[0], nedge[0])
+    return out
+
+
+def bench_c2(args, g5):
+    g = g5.Graph.kronecker(args.scale)
+    roots = g.select_roots(1 << args.scale, args.roots)
+    log(f"graph: scale {args.scale}, {g.nnz} nonzeros; roots {len(roots)}")
+    b = g5.BFS(g)
+    plain = []
+    for r in roots.tolist():
+        b.run(r)
+        plain.append(b.nedge())
+    for r in roots[:max(args.warmup, 1)].tolist():
+        b.run(r, codec=True)
+    b.profile(True)
+    step_ms, per_root_ms, nedge = [], None, None
+    for i in range(args.warmup + args.steps):
+        g5.lib().g5_device_sync()
+        t = time.perf_counter()
+        ms_r, ne_r = [], []
+        for r in roots.tolist():
+            nl, ms = b.run(r, codec=True)
+            ms_r.append(ms)
+            ne_r.append(b.nedge())
+        g5.lib().g5_device_sync()
+        if i >= args.warmup:
+            step_ms.append((time.perf_counter() - t) * 1e3)
+            per_root_ms = ms_r if per_root_ms is None else [a + c for a, c in zip(per_root_ms, ms_r)]
+        nedge = np.array(ne_r, np.float64)
+    assert np.array_equal(nedge, np.array(plain, np.float64)), "codec in the loop changed a search"
+    per_root_ms = np.array(per_root_ms) / args.steps
+    teps = nedge / (per_root_ms / 1e3)
+    value = hmean(teps) / 1e9
+    st = b.stats()
+    gbs = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
+    log(f"c2: hmean {value:.3f} GTEPS with the codec on every level, mean {per_root_ms.mean():.3f} ms/root")
+    b.free()
+    out = {
+        "metric": METRIC, "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 ids / u64 bitmaps (integer)",
+        "data": "synthetic: Graph500 Kronecker (A=.57 B=C=.19), edgefactor 16, seeds (1,2), generated on the device",
+        "config": {"workload": f"single-root BFS + s4-bp128-d4 encode/decode of every level queue, 64 roots, "
+                               f"Kronecker scale {args.scale}, 1x1 block",
+                   "scale": args.scale, "edgefactor": 16, "roots": int(len(roots)), "nnz": int(g.nnz)},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_expand",
+                     "avg_launch_ms": st["ms"] / max(st["launches"], 1),
+                     "algorithmic_bytes_per_launch": st["bytes"] / max(st["launches"], 1)},
+    }
+    if args.cpu_baseline == "on":
+        out["cpu_baseline"] = cpu_baseline_codec(g, args.scale, roots.tolist(), nedge)
+    return out
+
+
+def bench_c3(args):
+    """The driver runs as a child process; this process touches the GPU only
+    afterwards (for the CPU baseline's graph)."""
+    from gpugraph500_amd import build as gbuild
+    cmd = [gbuild.DRIVER, "-s", str(args.scale), "-R", "4", "-C", "4", "-n", str(args.roots), "-x", "device",
+           "-be", "s4-bp128-d4", "-share", "0", "0"]
+    log("c3: " + " ".join(cmd))
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"g500_hip exited {r.returncode}: {r.stderr[-500:]}")
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    wall = time.perf_counter() - t0
+    share_s = rec["share_expand_s"] + rec["share_column_s"] + rec["share_fold_s"]
+    kx_gbs = rec["share_kexpand_bytes"] / (rec["share_kexpand_ms"] / 1e3) / 1e9 if rec["share_kexpand_ms"] else 0.0
+    log(f"c3: emulated grid {rec['hmean_teps'] / 1e9:.3f} GTEPS; rank (0,0) share {share_s * 1e3:.2f} ms per search")
+    out = {
+        "metric": METRIC, "value": rec["hmean_teps"] / 1e9, "unit": "GTEPS", "n_gpus": 1, "steps": rec["roots"],
+        "warmup": 1, "ms_per_step": rec["mean_time_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 ids / u64 bitmaps (integer)",
+        "data": "synthetic: Graph500 Kronecker (A=.57 B=C=.19), edgefactor 16, seeds (1,2), generated on the device",
+        "config": {"workload": f"4x4 grid emulated on one GPU (16 blocks of 2^24 x 2^24 at scale {args.scale}), "
+                               f"device exchange, s4-bp128-d4 on the fold, rank (0,0)'s share timed",
+                   "scale": args.scale, "grid": [4, 4], "roots": rec["roots"]},
+        "rank_share_ms": {"expand": rec["share_expand_s"] * 1e3, "column_or": rec["share_column_s"] * 1e3,
+                          "fold": rec["share_fold_s"] * 1e3, "total": share_s * 1e3},
+        "exchange_words": {"raw": rec["raw_words"], "coded": rec["coded_words"]},
+        "roofline": {"bound": "hbm", "achieved": kx_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kx_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_expand (rank 0,0)",
+                     "avg_launch_ms": rec["share_kexpand_ms"] / max(rec["share_kexpand_launches"], 1),
+                     "algorithmic_bytes_per_launch": rec["share_kexpand_bytes"] / max(rec["share_kexpand_launches"], 1)},
+        "driver_wall_s": wall,
+    }
+    if args.cpu_baseline == "on":
+        import gpugraph500_amd as g5
+        g5.init(0)
+        g = g5.Graph.kronecker(args.scale)
+        roots = g.select_roots(1 << args.scale, 1)
+        b = g5.BFS(g)
+        b.run(int(roots[0]))
+        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots[0], b.nedge())
+        b.free()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--config", choices=["c4", "c2", "c3"], default="c4")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
+                    help="c4: measure roofline.traffic with rocprofv3 --pmc child passes")
+    ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
+    ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.scale is None:
+        args.scale = DEFAULT_SCALE[args.config]
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the 2D row/column exchange "
+                                   "is emulated on a single device",
+                          "n_gpus": args.gpus}))
+        return 2
+    if args.pmc_probe:
+        return pmc_probe(args)
+    if args.config == "c3":
+        print(json.dumps(bench_c3(args)))
+        return 0
+    traffic, traffic_info = None, None
+    if args.config == "c4" and args.traffic == "auto":
+        traffic, traffic_info = measure_traffic(args)   # before this process initialises the GPU
+
+    import gpugraph500_amd as g5
+
+    g5.init(0)
+    out = bench_c4(args, g5, traffic, traffic_info) if args.config == "c4" else bench_c2(args, g5)
     print(json.dumps(out))
     return 0
 

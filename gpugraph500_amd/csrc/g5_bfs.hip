@@ -328,6 +328,9 @@ extern "C" int g5_bfs_expand(g5_bfs *b, int *any_new)
     G5_HIP(hipGetLastError());
     rc = sync_counters(b);
     if (rc) return rc;
+    // the out bits are this block's newly reached columns
+    rc = account_level(b, b->qlen, b->h_cnt[kOutLen]);
+    if (rc) return rc;
     *any_new = b->h_cnt[kOutLen] > 0;
     return G5_OK;
 }

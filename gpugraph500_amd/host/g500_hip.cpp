@@ -130,9 +130,11 @@ int run(const Opts &o)
     int mismatches = 0;
     double codec_s = 0, pred_s = 0;
     int64_t raw_words = 0, coded_words = 0;
-    // one untimed search first (device buffers, code objects)
+    // one untimed search first (device buffers, code objects), then the
+    // watched rank's expansion kernel is accounted (g5_bfs_profile)
     GridStats st;
     grid.runBFS(roots[0], pred, st, o.r0, o.c0);
+    check(g5_bfs_profile(grid.rank(o.r0, o.c0).handle(), 1), "g5_bfs_profile");
     for (size_t i = 0; i < roots.size(); ++i) {
         grid.runBFS(roots[i], pred, st, o.r0, o.c0);
         const int64_t ne = grid.nedge(pred);
@@ -186,13 +188,18 @@ int run(const Opts &o)
     printf("mean_share_column_time: %.6f\n", mean_sc / k);
     printf("mean_share_fold_time: %.6f\n", mean_sf / k);
     if (o.verify) printf("verify_mismatches: %d\n", mismatches);
+    double kx_ms = 0, kx_bytes = 0;
+    int64_t kx_launches = 0, kx_edges = 0;
+    check(g5_bfs_stats(grid.rank(o.r0, o.c0).handle(), &kx_ms, &kx_launches, &kx_bytes, &kx_edges), "g5_bfs_stats");
     printf("{\"scale\": %d, \"R\": %d, \"C\": %d, \"roots\": %zu, \"exchange\": \"%s\", \"codec\": \"%s\", "
            "\"hmean_teps\": %.6e, \"mean_time_s\": %.6f, \"share\": [%d, %d], \"share_expand_s\": %.6f, "
            "\"share_column_s\": %.6f, \"share_fold_s\": %.6f, \"codec_s\": %.6f, \"pred_s\": %.6f, "
-           "\"raw_words\": %lld, \"coded_words\": %lld, \"verify_mismatches\": %d}\n",
+           "\"raw_words\": %lld, \"coded_words\": %lld, \"verify_mismatches\": %d, \"share_kexpand_ms\": %.6f, "
+           "\"share_kexpand_launches\": %lld, \"share_kexpand_bytes\": %.6e, \"share_kexpand_edges\": %lld}\n",
            o.scale, o.R, o.C, teps.size(), o.exchange.c_str(), o.codec.c_str(), hm, mean_t / k, o.r0, o.c0,
            mean_se / k, mean_sc / k, mean_sf / k, codec_s / k, pred_s / k, (long long)raw_words,
-           (long long)coded_words, o.verify ? mismatches : -1);
+           (long long)coded_words, o.verify ? mismatches : -1, kx_ms, (long long)kx_launches, kx_bytes,
+           (long long)kx_edges);
     gridp.reset();
     if (single) g5_bfs_free(single);
     g5_graph_free(whole);

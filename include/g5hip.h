@@ -146,7 +146,8 @@ int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred);
  * delivers it to non-root ranks (globalbfs.hh:864-905). *ms = device time. */
 enum { G5_RUN_CODEC = 1 };
 int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
-/* Per-kernel accounting of the expansion kernel across g5_bfs_run calls:
+/* Per-kernel accounting of the expansion kernel across g5_bfs_run and
+ * g5_bfs_expand calls:
  * enable (resets the totals), then read total device ms, launches,
  * algorithmic bytes (16 B per frontier vertex, 12 B per frontier edge, 12 B
  * per newly reached vertex) and frontier edges. */
