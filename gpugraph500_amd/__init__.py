@@ -92,6 +92,10 @@ def lib():
     _sig(L, "g5_msbfs_info", I, [P, P, I])
     _sig(L, "g5_msbfs_levels", I, [P, P, I, ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_set_mode", I, [P, I])
+    _sig(L, "g5_msbfs_open", I, [P])
+    _sig(L, "g5_msbfs_join", I, [P, P, P, I])
+    _sig(L, "g5_msbfs_step", I, [P, ctypes.POINTER(c_u64), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_msbfs_leave", I, [P, c_u64, P, P])
     _sig(L, "g5_s4_max_words", c_sz, [c_sz])
     _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
@@ -329,6 +333,27 @@ class MSBFS:
         lev = np.empty(self.graph.n_cols, np.int32)
         check(lib().g5_msbfs_result(self.h, lane, _p(pred), _p(lev)))
         return pred, lev
+
+    # ---- continuous mode ----
+    def open(self):
+        check(lib().g5_msbfs_open(self.h))
+
+    def join(self, roots, lanes):
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        ln = np.ascontiguousarray(lanes, dtype=np.int32)
+        check(lib().g5_msbfs_join(self.h, _p(r), _p(ln), r.size))
+
+    def step(self):
+        live = c_u64()
+        ms = ctypes.c_float()
+        check(lib().g5_msbfs_step(self.h, ctypes.byref(live), ctypes.byref(ms)))
+        return live.value, ms.value
+
+    def leave(self, mask):
+        ne = np.zeros(64, np.int64)
+        ck = np.zeros(64, np.uint64)
+        check(lib().g5_msbfs_leave(self.h, mask, _p(ne), _p(ck)))
+        return ne, ck
 
     def nedge(self):
         out = np.empty(64, np.int64)

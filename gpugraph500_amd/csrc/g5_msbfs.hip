@@ -64,13 +64,15 @@ constexpr int64_t kActiveRatio = 2;
 // Level counters.  Per accounting unit u (the kBins pull bins, then
 // kPushUnit) kUnitSlots accumulated slots at u * kUnitSlots: new (vertex,
 // root) pairs, edges read, vertices worked on, frontier words gathered, and
-// the next frontier's vertices and edges.  Then set-once slots: the level
-// number, the newly reached queue length (push), the frontier queue length
-// and its edge total (push).
+// the next frontier's vertices and edges; then the OR of the next frontier's
+// words (the lanes still searching).  Then set-once slots: the level number,
+// the newly reached queue length (push), the frontier queue length and its
+// edge total (push).
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
-    kAccum = kUnits * kUnitSlots,
+    kLive = kUnits * kUnitSlots,
+    kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3,
     kCntStride = 32
 };
@@ -117,6 +119,15 @@ struct g5_msbfs {
     // per level of the last profiled run (kLevelRec doubles each)
     std::vector<double> lvl;
     int mode = 0;  // 0 direction chosen per level, 1 pull only, 2 push only
+    // level state: the parity of the frontier the next level reads and the
+    // frontier's active vertices / edges (edges < 0: unknown, take push)
+    int p = 0;
+    int64_t act_verts = 0, act_edges = -1;
+    // continuous mode (g5_msbfs_open): lanes in use
+    bool cont = false;
+    uint64_t busy = 0;
+    int64_t *d_join = nullptr;             // roots [0, 64), lanes [64, 128), added frontier [128, 130)
+    unsigned long long *d_sums = nullptr;  // leave: checksums [0, 64), generated-degree sums [64, 128)
 };
 
 namespace {
@@ -610,13 +621,16 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
                                                          const int64_t *__restrict__ rowptr,
                                                          uint64_t *__restrict__ active, int64_t *__restrict__ cnt)
 {
+    __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nv = 0, ne = 0;
+    uint64_t live = 0;
     for (int64_t w0 = (blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6)) * 4; w0 < nwords; w0 += waves * 4) {
         uint64_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = w0 + k < nwords ? next[(w0 + k) * 64 + lane] : 0ull;
+        live |= x[0] | x[1] | x[2] | x[3];
         uint64_t mine = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -630,7 +644,16 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         }
         if (lane < 4 && w0 + lane < nwords) active[w0 + lane] = mine;
     }
-    if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});
+    live = g5_wave_or(live);
+    if (lane == 0) s_live[threadIdx.x >> 6] = live;
+    if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});  // its barrier orders s_live
+    else __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
+        if (t) atomicOr((unsigned long long *)&cnt[(blockIdx.x % kStripes) * kCntStride + kLive], (unsigned long long)t);
+    }
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -638,7 +661,10 @@ __global__ void k_level_end(int64_t *cnt)
 {
     for (int i = threadIdx.x; i < kAccum; i += blockDim.x) {
         int64_t t = 0;
-        for (int st = 0; st < kStripes; ++st) t += cnt[st * kCntStride + i];
+        if (i == kLive)
+            for (int st = 0; st < kStripes; ++st) t |= cnt[st * kCntStride + i];
+        else
+            for (int st = 0; st < kStripes; ++st) t += cnt[st * kCntStride + i];
         cnt[i] = t;
     }
     if (threadIdx.x == 0) cnt[kDepth] += 1;
@@ -788,6 +814,98 @@ __global__ __launch_bounds__(256) void k_ms_nedge(const uint64_t *__restrict__ v
     atomicAdd(&out[lane], acc);
 }
 
+// ---- continuous mode: lanes join and leave between levels ----
+
+// visited[v] = (visited[v] | set) & ~clear
+__global__ void k_lane_masks(uint64_t *__restrict__ visited, int64_t n, uint64_t set, uint64_t clear)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        visited[i] = (visited[i] | set) & ~clear;
+}
+
+// One thread: root jr[i] (original id) starts lane jr[64 + i] in the frontier
+// the next level reads.  A root without neighbours is only marked visited:
+// its search is complete (and vertices outside the degree bins are never
+// rewritten by the pull kernels, so no frontier bit may be left there).
+// added[0..1] = new frontier vertices and their edges.
+__global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__restrict__ old2new,
+                       const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
+                       uint64_t *__restrict__ front, uint64_t *__restrict__ act, int32_t *__restrict__ parent,
+                       int64_t *__restrict__ added)
+{
+    int64_t nv = 0, ne = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t root = jr[i];
+        const int lane = (int)jr[64 + i];
+        const int64_t v = old2new[root];
+        const uint64_t bit = 1ull << lane;
+        const int64_t deg = rowptr[v + 1] - rowptr[v];
+        visited[v] |= bit;
+        parent[v * 64 + lane] = (int32_t)root;
+        if (deg == 0) continue;
+        if (front[v] == 0) {
+            ++nv;
+            ne += deg;
+        }
+        front[v] |= bit;
+        act[v >> 6] |= 1ull << (v & 63);
+    }
+    added[0] = nv;
+    added[1] = ne;
+}
+
+// 64-bit finaliser of SplitMix64 (Steele, Lea, Flood 2014; constants as
+// published by Vigna): a bijective mix for the predecessor-map checksum
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+// Lanes in `mask` leave: per lane the sum of generated degrees over its
+// reached vertices (2 x nedge) and the checksum sum over reached v of
+// mix64(orig(v) << 32 | (pred(v) + 1)) mod 2^64, in original ids; then the
+// lanes are freed (visited set everywhere).  One wave per vertex, lane = root,
+// so each vertex's 256 B parent record is read once, coalesced.
+__global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, const int32_t *__restrict__ parent,
+                                               const uint32_t *__restrict__ new2old, const uint32_t *__restrict__ gdeg,
+                                               int64_t n, uint64_t mask, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s_ck[64], s_dg[64];
+    if (threadIdx.x < 64) {
+        s_ck[threadIdx.x] = 0;
+        s_dg[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int lane = __lane_id();
+    unsigned long long ck = 0, dg = 0;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t v = blockIdx.x * 4 + (threadIdx.x >> 6); v < n; v += waves) {
+        const uint64_t vis = visited[v];
+        const uint64_t reach = vis & mask;
+        if (reach >> lane & 1) {
+            const uint32_t o = new2old[v];
+            const int32_t p = parent[v * 64 + lane];
+            ck += mix64(((uint64_t)o << 32) | (uint32_t)(p + 1));
+            dg += gdeg[v];
+        }
+        if (lane == 0 && (~vis & mask)) visited[v] = vis | mask;
+    }
+    if (mask >> lane & 1) {
+        atomicAdd(&s_ck[lane], ck);
+        atomicAdd(&s_dg[lane], dg);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && (mask >> threadIdx.x & 1)) {
+        if (s_ck[threadIdx.x]) atomicAdd(&out[threadIdx.x], s_ck[threadIdx.x]);
+        if (s_dg[threadIdx.x]) atomicAdd(&out[64 + threadIdx.x], s_dg[threadIdx.x]);
+    }
+}
+
 // act: pull kernels consult the active bitmap; prof: events between the
 // three pull kernels (the uncaptured path)
 int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof = false)
@@ -931,6 +1049,85 @@ int build_relabel(g5_msbfs *m)
     return G5_OK;
 }
 
+// One level at parity m->p: push or pull by the frontier's edges, then the
+// level's counters on the host; sets *new_bits (new (vertex, root) pairs)
+// and *live (OR of the next frontier), and moves the frontier state on.
+int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
+{
+    hipStream_t s = m->s;
+    const int p = m->p;
+    const int64_t act_verts = m->act_verts, act_edges = m->act_edges;
+    const int64_t nnz = m->g->nnz;
+    const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * kPushRatio < nnz));
+    const bool act_check = act_edges < 0 || act_edges * kActiveRatio < nnz;
+    if (push) {
+        int rc = push_level(m, s, p, act_verts, m->prof);
+        if (rc) return rc;
+    } else if (m->prof) {
+        int rc = level_launches(m, s, p, act_check, true);
+        if (rc) return rc;
+    } else {
+        G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
+    }
+    G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kCntStride * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    int64_t new_bits = 0;
+    const int64_t *h = m->h_cnt;
+    int64_t next_verts = 0, next_edges = 0;
+    for (int c = 0; c < kUnits; ++c) {
+        new_bits += h[kUnitSlots * c + kNewBits];
+        next_verts += h[kUnitSlots * c + kNextVerts];
+        next_edges += h[kUnitSlots * c + kNextEdges];
+    }
+    if (m->prof) {
+        double rec[kLevelRec] = {0};
+        rec[8] = (double)new_bits;
+        rec[9] = push ? 1.0 : 0.0;
+        rec[10] = (double)act_verts;
+        rec[11] = (double)act_edges;
+        if (push) {
+            // 16 B row offsets per frontier vertex; per frontier edge and
+            // pass (two passes) 4 B column + 8 B frontier word + 8 B visited
+            // word + 8 B next word; 16 B visited read-modify-write per newly
+            // reached vertex; 4 B parent init + 4 B parent per new (vertex,
+            // root) pair
+            float kms = 0;
+            G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
+            const double E = (double)h[kFqEdges];
+            m->k_ms[kPushUnit] += kms;
+            m->k_launches[kPushUnit] += 1;
+            m->k_bytes[kPushUnit] += 16.0 * h[kFqLen] + 56.0 * E + 16.0 * h[kUnitSlots * kPushUnit + kVisitedVerts] +
+                                     8.0 * h[kUnitSlots * kPushUnit + kNewBits];
+            rec[kPushUnit] = kms;
+            rec[4 + kPushUnit] = E;
+        } else {
+            // per bin: 16 B row offsets + 8 B visited read + 8 B next write
+            // per vertex with an unvisited lane; 4 B column (+ 1 bit of the
+            // active bitmap when checked) per edge read; 8 B per frontier
+            // word gathered; 4 B parent per new (vertex, root) pair
+            for (int k = 0; k < kBins; ++k) {
+                if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
+                float kms = 0;
+                G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
+                m->k_ms[k] += kms;
+                m->k_launches[k] += 1;
+                m->k_bytes[k] += 32.0 * h[kUnitSlots * k + kVisitedVerts] +
+                                 (act_check ? 4.125 : 4.0) * h[kUnitSlots * k + kEdgesRead] +
+                                 8.0 * h[kUnitSlots * k + kGathers] + 4.0 * h[kUnitSlots * k + kNewBits];
+                rec[k] = kms;
+                rec[4 + k] = (double)h[kUnitSlots * k + kEdgesRead];
+            }
+        }
+        m->lvl.insert(m->lvl.end(), rec, rec + kLevelRec);
+    }
+    *new_bits_out = new_bits;
+    *live_out = (uint64_t)h[kLive];
+    m->act_verts = next_verts;
+    m->act_edges = next_edges;
+    m->p ^= 1;
+    return G5_OK;
+}
+
 }  // namespace
 
 extern "C" int g5_msbfs_free(g5_msbfs *m);
@@ -961,6 +1158,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kCntStride * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_join, 132 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_sums, 128 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     for (int k = 0; k <= kBins && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
@@ -980,9 +1179,9 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     for (auto &gp : m->graph)
         for (hipGraphExec_t ge : gp)
             if (ge) (void)hipGraphExecDestroy(ge);
-    void *dev[] = {m->d_fq, m->d_nq, m->d_qoff, m->d_act, m->d_visited, m->d_fa,        m->d_fb,           m->d_parent,
-                   m->d_cnt,     m->d_scratch,   m->d_roots,        m->rl.new2old,     m->rl.old2new,     m->rl.rowptr,
-                   m->rl.cols,   m->rl.gdeg};
+    void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
+                   m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
+                   m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
@@ -1003,6 +1202,8 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         G5_REQUIRE(roots[r] >= 0 && roots[r] < m->n, G5_E_ARG, "g5_msbfs_run: root %lld out of range",
                    (long long)roots[r]);
     hipStream_t s = m->s;
+    m->cont = false;
+    m->busy = 0;
     m->nroots = nroots;
     std::copy(roots, roots + nroots, m->roots);
     int64_t *d_roots = m->d_roots;
@@ -1015,87 +1216,25 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
                        m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
     G5_HIP(hipGetLastError());
-    int p = 0, levels = 1;
+    int levels = 1;
     m->lvl.clear();
     // the first frontier is the set of distinct roots; its edge count is not
     // known on the host, and a handful of roots always takes the push path
-    int64_t act_verts = 0, act_edges = -1;
+    m->p = 0;
+    m->act_edges = -1;
     {
         int64_t tmp[64];
         std::copy(roots, roots + nroots, tmp);
         std::sort(tmp, tmp + nroots);
-        act_verts = std::unique(tmp, tmp + nroots) - tmp;
+        m->act_verts = std::unique(tmp, tmp + nroots) - tmp;
     }
-    const int64_t nnz = m->g->nnz;
     for (;;) {
-        const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * kPushRatio < nnz));
-        const bool act_check = act_edges < 0 || act_edges * kActiveRatio < nnz;
-        if (push) {
-            int rc = push_level(m, s, p, act_verts, m->prof);
-            if (rc) return rc;
-        } else if (m->prof) {
-            int rc = level_launches(m, s, p, act_check, true);
-            if (rc) return rc;
-        } else {
-            G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
-        }
-        G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kCntStride * 8, hipMemcpyDeviceToHost, s));
-        G5_HIP(hipStreamSynchronize(s));
         int64_t new_bits = 0;
-        const int64_t *h = m->h_cnt;
-        int64_t next_verts = 0, next_edges = 0;
-        for (int c = 0; c < kUnits; ++c) {
-            new_bits += h[kUnitSlots * c + kNewBits];
-            next_verts += h[kUnitSlots * c + kNextVerts];
-            next_edges += h[kUnitSlots * c + kNextEdges];
-        }
-        if (m->prof) {
-            double rec[kLevelRec] = {0};
-            rec[8] = (double)new_bits;
-            rec[9] = push ? 1.0 : 0.0;
-            rec[10] = (double)act_verts;
-            rec[11] = (double)act_edges;
-            if (push) {
-                // 16 B row offsets per frontier vertex; per frontier edge and
-                // pass (two passes) 4 B column + 8 B frontier word + 8 B
-                // visited word + 8 B next word; 16 B visited read-modify-write
-                // per newly reached vertex; 4 B parent init + 4 B parent per
-                // new (vertex, root) pair
-                float kms = 0;
-                G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
-                const double E = (double)m->h_cnt[kFqEdges];
-                m->k_ms[kPushUnit] += kms;
-                m->k_launches[kPushUnit] += 1;
-                m->k_bytes[kPushUnit] += 16.0 * m->h_cnt[kFqLen] + 56.0 * E +
-                                         16.0 * h[kUnitSlots * kPushUnit + kVisitedVerts] +
-                                         8.0 * h[kUnitSlots * kPushUnit + kNewBits];
-                rec[kPushUnit] = kms;
-                rec[4 + kPushUnit] = E;
-            } else {
-                // per bin: 16 B row offsets + 8 B visited read + 8 B next
-                // write per vertex with an unvisited lane; 4 B column (+ 1 bit
-                // of the active bitmap when checked) per edge read; 8 B per
-                // frontier word gathered; 4 B parent per new (vertex, root) pair
-                for (int k = 0; k < kBins; ++k) {
-                    if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
-                    float kms = 0;
-                    G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
-                    m->k_ms[k] += kms;
-                    m->k_launches[k] += 1;
-                    m->k_bytes[k] += 32.0 * h[kUnitSlots * k + kVisitedVerts] +
-                                     (act_check ? 4.125 : 4.0) * h[kUnitSlots * k + kEdgesRead] +
-                                     8.0 * h[kUnitSlots * k + kGathers] + 4.0 * h[kUnitSlots * k + kNewBits];
-                    rec[k] = kms;
-                    rec[4 + k] = (double)h[kUnitSlots * k + kEdgesRead];
-                }
-            }
-            m->lvl.insert(m->lvl.end(), rec, rec + kLevelRec);
-        }
+        uint64_t live = 0;
+        int rc = run_level(m, &new_bits, &live);
+        if (rc) return rc;
         if (new_bits == 0) break;
-        act_verts = next_verts;
-        act_edges = next_edges;
         ++levels;
-        p ^= 1;
     }
     G5_HIP(hipEventRecord(m->ev1, s));
     G5_HIP(hipEventSynchronize(m->ev1));
@@ -1104,9 +1243,106 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     return G5_OK;
 }
 
+extern "C" int g5_msbfs_open(g5_msbfs *m)
+{
+    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_open: null handle");
+    hipStream_t s = m->s;
+    // every lane free: free lanes count as visited everywhere, so they never
+    // keep a vertex from the fully-visited early exit
+    hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
+                       m->d_act, m->n, ~0ull);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(s));
+    m->cont = true;
+    m->busy = 0;
+    m->nroots = 0;
+    m->p = 0;
+    m->act_verts = 0;
+    m->act_edges = 0;
+    m->lvl.clear();
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes, int n)
+{
+    G5_REQUIRE(m && n >= 0 && n <= 64 && (n == 0 || (roots && lanes)), G5_E_ARG, "g5_msbfs_join: bad arguments");
+    G5_REQUIRE(m->cont, G5_E_STATE, "g5_msbfs_join: g5_msbfs_open not called");
+    if (n == 0) return G5_OK;
+    uint64_t mask = 0;
+    int64_t h[128];
+    for (int i = 0; i < n; ++i) {
+        G5_REQUIRE(lanes[i] >= 0 && lanes[i] < 64, G5_E_ARG, "g5_msbfs_join: lane %d out of range", lanes[i]);
+        G5_REQUIRE(!((m->busy | mask) >> lanes[i] & 1), G5_E_STATE, "g5_msbfs_join: lane %d is not free", lanes[i]);
+        G5_REQUIRE(roots[i] >= 0 && roots[i] < m->n, G5_E_ARG, "g5_msbfs_join: root %lld out of range",
+                   (long long)roots[i]);
+        mask |= 1ull << lanes[i];
+        h[i] = roots[i];
+        h[64 + i] = lanes[i];
+    }
+    hipStream_t s = m->s;
+    uint64_t *front = m->p == 0 ? m->d_fa : m->d_fb;
+    uint64_t *act = m->d_act + (m->p == 0 ? 0 : m->n / 64);
+    G5_HIP(hipMemcpyAsync(m->d_join, h, sizeof(h), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_lane_masks, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->n, 0ull, mask);
+    hipLaunchKernelGGL(k_join, 1, 1, 0, s, (const int64_t *)m->d_join, n, (const uint32_t *)m->rl.old2new,
+                       (const int64_t *)m->rl.rowptr, m->d_visited, front, act, m->d_parent, m->d_join + 128);
+    G5_HIP(hipGetLastError());
+    int64_t added[2] = {0, 0};
+    G5_HIP(hipMemcpyAsync(added, m->d_join + 128, 16, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    m->busy |= mask;
+    m->act_verts += added[0];
+    m->act_edges += added[1];
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_step(g5_msbfs *m, uint64_t *live, float *ms)
+{
+    G5_REQUIRE(m && live, G5_E_ARG, "g5_msbfs_step: null argument");
+    G5_REQUIRE(m->cont, G5_E_STATE, "g5_msbfs_step: g5_msbfs_open not called");
+    *live = 0;
+    if (ms) *ms = 0;
+    if (!m->busy) return G5_OK;
+    G5_HIP(hipEventRecord(m->ev0, m->s));
+    int64_t new_bits = 0;
+    uint64_t l = 0;
+    int rc = run_level(m, &new_bits, &l);
+    if (rc) return rc;
+    G5_HIP(hipEventRecord(m->ev1, m->s));
+    G5_HIP(hipEventSynchronize(m->ev1));
+    if (ms) G5_HIP(hipEventElapsedTime(ms, m->ev0, m->ev1));
+    *live = l & m->busy;
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint64_t *checksum)
+{
+    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_leave: null handle");
+    G5_REQUIRE(m->cont, G5_E_STATE, "g5_msbfs_leave: g5_msbfs_open not called");
+    G5_REQUIRE((lanes & ~m->busy) == 0, G5_E_STATE, "g5_msbfs_leave: lane not in use");
+    if (!lanes) return G5_OK;
+    hipStream_t s = m->s;
+    G5_HIP(g5::fill_u64((uint64_t *)m->d_sums, 128, 0, s));
+    hipLaunchKernelGGL(k_leave, g5::grid_for((m->n + 3) / 4, 1, 8192), 256, 0, s, m->d_visited,
+                       (const int32_t *)m->d_parent, (const uint32_t *)m->rl.new2old, (const uint32_t *)m->rl.gdeg,
+                       m->n, lanes, m->d_sums);
+    G5_HIP(hipGetLastError());
+    unsigned long long h[128];
+    G5_HIP(hipMemcpyAsync(h, m->d_sums, sizeof(h), hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    for (int r = 0; r < 64; ++r)
+        if (lanes >> r & 1) {
+            if (checksum) checksum[r] = h[r];
+            if (nedge) nedge[r] = (int64_t)(h[64 + r] / 2);
+        }
+    m->busy &= ~lanes;
+    return G5_OK;
+}
+
 extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level)
 {
-    G5_REQUIRE(m && lane >= 0 && lane < m->nroots, G5_E_ARG, "g5_msbfs_result: lane out of range");
+    G5_REQUIRE(m && lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
+               "g5_msbfs_result: lane %d not in use", lane);
     const int64_t n = m->n;
     hipStream_t s = m->s;
     int64_t *dp = nullptr;

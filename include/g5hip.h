@@ -190,6 +190,24 @@ int g5_msbfs_stats(g5_msbfs *m, int unit, double *ms, int64_t *launches, double 
 int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev);
 /* Vertex counts of the three degree bins (units 0..2 above), up to cap. */
 int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
+/* Continuous mode (serving): lanes join and leave between levels, so a root
+ * that arrives while other searches run starts at the next level instead of
+ * waiting for the batch to end.  Each lane's result equals the search of its
+ * root alone.
+ *   g5_msbfs_open   every lane free (ends any batch run on m)
+ *   g5_msbfs_join   roots[i] starts in free lane lanes[i] (n <= 64)
+ *   g5_msbfs_step   one level for all lanes in use; *live = lanes whose
+ *                   frontier is still non-empty -- a lane in use but not in
+ *                   *live has finished; *ms = device time of the level
+ *   g5_msbfs_leave  for each lane in the mask: nedge[lane] (TEPS numerator)
+ *                   and checksum[lane] = sum over reached vertices v of
+ *                   mix64(v << 32 | (pred(v) + 1)) mod 2^64 (original ids;
+ *                   mix64 = SplitMix64's finaliser); then the lanes are free.
+ *                   g5_msbfs_result may be called for a lane before it leaves. */
+int g5_msbfs_open(g5_msbfs *m);
+int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes, int n);
+int g5_msbfs_step(g5_msbfs *m, uint64_t *live, float *ms);
+int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint64_t *checksum);
 
 /* ---- s4-bp128-d4 codec (codecfactory.h:103-105) on the device ----
  * Word stream identical to the reference codec's output for 16-byte aligned
