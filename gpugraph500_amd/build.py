@@ -3,8 +3,8 @@
     python -m gpugraph500_amd.build        # or build() from Python
 
 Every .hip file under csrc/ is compiled for gfx950 and linked into
-gpugraph500_amd/libg5hip.so next to this file.  The C++ driver under host/
-(g500_hip) is built as well.
+gpugraph500_amd/libg5hip.so next to this file.  The C++ host side under host/
+is built as well (drivers g500_hip and g500_serve).
 """
 import os
 import subprocess
@@ -62,20 +62,31 @@ def build(verbose=False, jobs=None):
 
 HOST = os.path.join(HERE, "host")
 DRIVER = os.path.join(HERE, "g500_hip")
-HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), "-I" + HOST]
+SERVE = os.path.join(HERE, "g500_serve")
+HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread", "-I" + os.path.join(ROOT, "include"), "-I" + HOST]
 
 
 def host_sources():
-    return sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    """The host library sources (everything under host/ but the drivers)."""
+    return sorted(os.path.join(HOST, f) for f in os.listdir(HOST)
+                  if f.endswith(".cpp") and not f.startswith("g500_"))
+
+
+def drivers():
+    """host/g500_<name>.cpp -> gpugraph500_amd/g500_<name>"""
+    return {os.path.join(HERE, f[:-4]): os.path.join(HOST, f) for f in sorted(os.listdir(HOST))
+            if f.startswith("g500_") and f.endswith(".cpp")}
 
 
 def build_host():
-    """The C++ host side (HipBFS, the emulated 2D grid, the g500_hip driver):
-    plain g++ against the C ABI of libg5hip.so, found next to the binary."""
-    tmp = DRIVER + ".tmp"
-    _run([os.environ.get("CXX", "g++")] + HOST_FLAGS + host_sources() +
-         ["-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, DRIVER)
+    """The C++ host side (HipBFS, the emulated 2D grid, the serving engine and
+    the g500_* drivers): plain g++ against the C ABI of libg5hip.so, found
+    next to the binaries."""
+    for exe, main in drivers().items():
+        tmp = exe + ".tmp"
+        _run([os.environ.get("CXX", "g++")] + HOST_FLAGS + host_sources() + [main] +
+             ["-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, exe)
     return DRIVER
 
 

@@ -1,0 +1,100 @@
+// g5_serve.hh -- BFS serving over several graphs resident in HBM, with
+// continuous batching on the 64-lane multi-source search.
+//
+// One Engine per resident graph: its own g5_msbfs in continuous mode (own
+// stream) driven by its own host thread, so searches on different graphs
+// overlap on the device.  A query names a graph and 1..8 roots.  Admission is
+// by lane: each lane is a preallocated memory slot of the engine (the
+// int32[n] parent column plus one bit per vertex in the visited and frontier
+// masks), and a query is admitted, FIFO, as soon as enough lanes are free for
+// all of its roots -- between two levels of the searches already running,
+// not when the batch ends.  A root's lane leaves as soon as its own search
+// is complete; its result is (nedge, checksum of the predecessor map), and
+// is identical to the root searched alone.
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "g5hip.h"
+
+namespace g5host {
+
+struct Query {
+    int id = 0;
+    int graph = 0;
+    std::vector<int64_t> roots;
+    double arrival = 0;   // seconds since the serving start (the replay clock)
+    // filled by the engine
+    double admitted = -1, done = -1;
+    std::vector<int64_t> nedge;
+    std::vector<uint64_t> checksum;
+    int remaining = 0;
+};
+
+struct EngineStats {
+    int64_t levels = 0, joins = 0, leaves = 0;
+    double device_ms = 0;  // sum of the levels' device time
+};
+
+class Engine
+{
+public:
+    // owns the graph (built on the device from the Kronecker generator)
+    Engine(int scale, int edgefactor, int lanes);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    void start();                 // launch the engine thread
+    void submit(Query *q);        // hand a query to the engine (thread-safe)
+    void finish();                // no more queries: drain, join the thread, rethrow its error
+    // the whole query alone on an idle engine (no thread), for reference results
+    void run_alone(Query *q);
+
+    int scale() const { return scale_; }
+    int64_t vertices() const { return n_; }
+    const std::vector<int64_t> &keys() const { return keys_; }  // search keys (degree > 0)
+    EngineStats stats() const { return stats_; }
+    double clock() const;         // seconds since the shared epoch
+    void set_epoch(double t0) { epoch_ = t0; }
+
+private:
+    int scale_;
+    int max_lanes_;
+    int64_t n_ = 0;
+    g5_graph *g_ = nullptr;
+    g5_msbfs *m_ = nullptr;
+    std::vector<int64_t> keys_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Query *> pending_;
+    bool stop_ = false;
+    struct Slot {
+        Query *q = nullptr;
+        int idx = 0;
+    };
+    Slot slots_[64];
+    uint64_t busy_ = 0;
+    EngineStats stats_;
+    double epoch_ = 0;
+    std::string error_;  // set when the engine thread stopped on an error
+
+    void loop();
+    void loop_body();
+    // admit from `from` while lanes are free; returns the number of roots joined
+    int admit(std::deque<Query *> &from);
+    // one level plus the departures it causes; returns true while lanes are in use
+    bool advance();
+};
+
+double now_seconds();
+
+}  // namespace g5host

@@ -1,0 +1,68 @@
+"""Serving (g500_serve): several resident graphs, a Poisson query stream,
+continuous batching into the 64-lane search.  Every query's result must be
+the same served in the mix as run alone, and equal to the oracle's search."""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpugraph500_amd import build as gbuild
+import oracle_lib as O
+from test_gpu_continuous import checksum
+
+pytestmark = pytest.mark.gpu
+
+
+def _serve(tmp_path, *args):
+    dump = str(tmp_path / "serve.txt")
+    cmd = [gbuild.SERVE, *args, "-dump", dump]
+    print("case", cmd, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-1500:] + r.stderr[-1500:]
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    rows = []
+    with open(dump) as f:
+        for line in f:
+            t = line.split()
+            qid, scale, arr, done, n = int(t[0]), int(t[1]), float(t[2]), float(t[3]), int(t[4])
+            roots = [(int(t[5 + 3 * k]), int(t[6 + 3 * k]), int(t[7 + 3 * k])) for k in range(n)]
+            rows.append((qid, scale, arr, done, roots))
+    return rec, rows
+
+
+def test_served_equals_alone_under_overlap(tmp_path):
+    """A burst far above the service rate: queries overlap on every engine
+    and join running searches between levels."""
+    rec, rows = _serve(tmp_path, "-g", "10,12,14", "-rate", "50000", "-nq", "300", "-seed", "3", "-compare")
+    assert rec["mismatches"] == 0 and rec["queries"] == 300
+    assert all(done >= arr for _, _, arr, done, _ in rows)
+    # the engines really did run several queries' roots at once
+    assert rec["levels"] < sum(len(r[4]) for r in rows) * 4
+
+
+def test_served_equals_alone_with_few_lanes(tmp_path):
+    """Eight lanes per engine: admission waits for lanes to free up."""
+    rec, rows = _serve(tmp_path, "-g", "11,13", "-rate", "20000", "-nq", "200", "-seed", "5", "-lanes", "8",
+                       "-compare")
+    assert rec["mismatches"] == 0
+
+
+def test_served_results_match_oracle(tmp_path):
+    rec, rows = _serve(tmp_path, "-g", "12", "-rate", "20000", "-nq", "40", "-seed", "7")
+    o = O.build_csr(O.kronecker_edges(12, 16))
+    seen = {}
+    for _, scale, _, _, roots in rows:
+        for root, ne, ck in roots:
+            if root in seen:
+                assert seen[root] == (ne, ck)
+                continue
+            rp, rl, _ = O.bfs_bin(o, root)
+            assert ne == O.nedge(o, rl), root
+            assert ck == checksum(rp), root
+            seen[root] = (ne, ck)
+
+
+def test_alone_mode(tmp_path):
+    rec, rows = _serve(tmp_path, "-g", "10", "-nq", "20", "-alone")
+    assert rec["mode"] == "alone" and len(rows) == 20

@@ -62,12 +62,14 @@ def test_host_unit_cpp(tmp_path):
     assert "all checks passed" in r.stdout
 
 
-def test_host_sources_compile_cleanly(tmp_path):
-    """The host side builds warning-free against the C ABI alone."""
+@pytest.mark.parametrize("name", ["g500_hip", "g500_serve"])
+def test_host_sources_compile_cleanly(tmp_path, name):
+    """The host side and each driver build warning-free against the C ABI alone."""
     lib_dir = os.path.dirname(g5.LIB_PATH)
-    exe = str(tmp_path / "g500_hip")
+    main = os.path.join(gbuild.HOST, name + ".cpp")
+    exe = str(tmp_path / name)
     cmd = [os.environ.get("CXX", "g++")] + gbuild.HOST_FLAGS + ["-Werror"] + gbuild.host_sources() + [
-        "-o", exe, "-L" + lib_dir, "-lg5hip", "-Wl,-rpath," + lib_dir]
+        main, "-o", exe, "-L" + lib_dir, "-lg5hip", "-Wl,-rpath," + lib_dir]
     subprocess.run(cmd, check=True)
 
 
@@ -82,3 +84,6 @@ def test_driver_rejects_bad_options():
     for args in (["-x", "wire"], ["-n", "0"], ["-R", "2", "-C", "2", "-share", "2", "0"], ["-bogus"]):
         r = subprocess.run([gbuild.DRIVER] + args, capture_output=True, text=True)
         assert r.returncode == 1 and "g500_hip:" in r.stderr
+    for args in (["-rate", "0"], ["-maxroots", "9", "-lanes", "8"], ["-lanes", "65"], ["-g", ""], ["-x"]):
+        r = subprocess.run([gbuild.SERVE] + args, capture_output=True, text=True)
+        assert r.returncode == 1 and "g500_serve:" in r.stderr
