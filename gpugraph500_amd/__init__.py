@@ -72,6 +72,7 @@ def lib():
     _sig(L, "g5_select_roots", I, [P, c_i64, I, P, ctypes.POINTER(I)])
     _sig(L, "g5_bfs_profile", I, [P, I])
     _sig(L, "g5_bfs_stats", I, [P, P, P, P, P])
+    _sig(L, "g5_bfs_kernel_stats", I, [P, I, P, P, P, P])
     _sig(L, "g5_bfs_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_bfs_free", I, [P])
     _sig(L, "g5_bfs_start", I, [P, c_i64])
@@ -276,10 +277,14 @@ class BFS:
         check(lib().g5_bfs_result(self.h, _p(pred), _p(lev)))
         return pred, lev
 
-    def run(self, root, codec=False):
+    RUN_CODEC, RUN_TOPDOWN = 1, 2
+    KERNELS = ("k_expand", "k_bottom_up")
+
+    def run(self, root, codec=False, topdown=False):
         nl = c_i32()
         ms = ctypes.c_float()
-        check(lib().g5_bfs_run(self.h, root, 1 if codec else 0, ctypes.byref(nl), ctypes.byref(ms)))
+        flags = (self.RUN_CODEC if codec else 0) | (self.RUN_TOPDOWN if topdown else 0)
+        check(lib().g5_bfs_run(self.h, root, flags, ctypes.byref(nl), ctypes.byref(ms)))
         return nl.value, ms.value
 
     def profile(self, enable=True):
@@ -291,6 +296,17 @@ class BFS:
         check(lib().g5_bfs_stats(self.h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(bytes_),
                                  ctypes.byref(edges)))
         return {"ms": ms.value, "launches": launches.value, "bytes": bytes_.value, "edges": edges.value}
+
+    def kernel_stats(self):
+        """{kernel name: stats} for k_expand and k_bottom_up."""
+        res = {}
+        for k, name in enumerate(self.KERNELS):
+            ms, bytes_ = ctypes.c_double(), ctypes.c_double()
+            launches, edges = c_i64(), c_i64()
+            check(lib().g5_bfs_kernel_stats(self.h, k, ctypes.byref(ms), ctypes.byref(launches),
+                                            ctypes.byref(bytes_), ctypes.byref(edges)))
+            res[name] = {"ms": ms.value, "launches": launches.value, "bytes": bytes_.value, "edges": edges.value}
+        return res
 
     def nedge(self):
         n = c_i64()

@@ -15,6 +15,11 @@
 //   4. out bitmap -> sorted queue (word popcount scan), the next in-queue.
 // Queues are born sorted and unique, so the reference's thrust::sort and host
 // unique_copy (cuda_bfs.cu:380-475) have no counterpart.
+//
+// g5_bfs_run (1x1 block, resident) also runs dense levels bottom-up
+// (k_bottom_up): each unvisited vertex scans its sorted neighbour list and
+// stops at the first frontier vertex, which is again the minimum-id frontier
+// neighbour, so both directions give identical parents and levels.
 #include <algorithm>
 
 #include "g5_codec.hpp"
@@ -28,7 +33,14 @@ constexpr int kExpandBlock = 256;
 constexpr int kExpandTile = 2048;            // edges per block tile
 constexpr int kExpandGrid = 4096;            // persistent grid, grid-stride over tiles
 
-enum Cnt { kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNCnt = 8 };
+// kNewEdges: degree sum of the newly reached vertices (1x1 only); kBuEdges,
+// kBuVerts: neighbours checked and vertices scanned by a bottom-up level
+enum Cnt { kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNewEdges = 4, kBuEdges = 5, kBuVerts = 6, kNCnt = 8 };
+// direction switch of the 1x1 search (Beamer, Asanovic, Patterson, SC'12):
+// top-down -> bottom-up when the frontier's edges exceed the unexplored
+// edges / kAlpha, back when the frontier holds fewer than n / kBeta vertices
+constexpr int64_t kAlpha = 14, kBeta = 24;
+constexpr int kBuUnroll = 4;
 
 }  // namespace
 
@@ -57,6 +69,10 @@ struct g5_bfs {
     hipEvent_t ek0 = nullptr, ek1 = nullptr;
     double exp_ms = 0, exp_bytes = 0;
     int64_t exp_launches = 0, exp_edges = 0;
+    // bottom-up levels of g5_bfs_run: the next-frontier bitmap and accounting
+    uint64_t *d_next = nullptr;
+    double bu_ms = 0, bu_bytes = 0;
+    int64_t bu_launches = 0, bu_edges = 0;
 };
 
 namespace {
@@ -113,11 +129,13 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(
     }
 }
 
-// visited |= out (minus already visited), level := depth for new bits, count.
+// visited |= out (minus already visited), level := depth for new bits, count;
+// with rowptr (1x1 block) also the degree sum of the new vertices.
 __global__ void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visited,
-                         int32_t *__restrict__ level, int64_t wc, int32_t depth, int64_t *__restrict__ cnt)
+                         int32_t *__restrict__ level, int64_t wc, int32_t depth, int64_t *__restrict__ cnt,
+                         const int64_t *__restrict__ rowptr)
 {
-    int64_t c = 0;
+    int64_t c = 0, ce = 0;
     for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < wc;
          w += (int64_t)gridDim.x * blockDim.x) {
         uint64_t o = out[w];
@@ -131,11 +149,74 @@ __global__ void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visi
         while (b) {
             const int i = __ffsll((long long)b) - 1;
             level[w * 64 + i] = depth;
+            if (rowptr) ce += rowptr[w * 64 + i + 1] - rowptr[w * 64 + i];
             b &= b - 1;
         }
     }
     c = g5_wave_sum(c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)&cnt[kNew], (unsigned long long)c);
+    ce = g5_wave_sum(ce);
+    if ((threadIdx.x & 63) == 0 && c) {
+        atomicAdd((unsigned long long *)&cnt[kNew], (unsigned long long)c);
+        if (ce) atomicAdd((unsigned long long *)&cnt[kNewEdges], (unsigned long long)ce);
+    }
+}
+
+// Bottom-up level of the 1x1 search: every unvisited vertex scans its
+// neighbours in ascending id order and stops at the first one whose frontier
+// bit is set -- its minimum-id frontier neighbour, the parent CPUBFS_bin's
+// ascending scan keeps (cpu_2d/cpubfs_bin.cpp:150-173).  One wavefront per
+// 64-vertex bitmap word (lane = vertex), so the next-frontier word is the
+// wave's ballot and needs no atomics.  kBuUnroll frontier bits are gathered
+// before any is tested, to keep several loads in flight per lane.
+__global__ __launch_bounds__(256) void k_bottom_up(const int64_t *__restrict__ rowptr,
+                                                   const uint32_t *__restrict__ cols,
+                                                   const uint64_t *__restrict__ visited,
+                                                   const uint64_t *__restrict__ front, uint64_t *__restrict__ next,
+                                                   uint32_t *__restrict__ parent, int64_t wc, int64_t *__restrict__ cnt)
+{
+    __shared__ int64_t s_e[4], s_v[4];
+    const int lane = __lane_id();
+    int64_t nedges = 0, nverts = 0;
+    for (int64_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < wc; w += (int64_t)gridDim.x * 4) {
+        const uint64_t vis = visited[w];
+        bool hit = false;
+        if (!(vis >> lane & 1)) {
+            const int64_t v = w * 64 + lane;
+            const int64_t b = rowptr[v], e = rowptr[v + 1];
+            int64_t j = b;
+            while (j < e && !hit) {
+                uint32_t u[kBuUnroll];
+                uint64_t fw[kBuUnroll];
+#pragma unroll
+                for (int k = 0; k < kBuUnroll; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
+#pragma unroll
+                for (int k = 0; k < kBuUnroll; ++k) fw[k] = j + k < e ? front[u[k] >> 6] : 0ull;
+#pragma unroll
+                for (int k = 0; k < kBuUnroll; ++k)
+                    if (!hit && (fw[k] >> (u[k] & 63) & 1)) {
+                        hit = true;
+                        parent[v] = u[k];
+                    }
+                j += kBuUnroll;
+            }
+            nedges += min(j, e) - b;
+            ++nverts;
+        }
+        const uint64_t bits = __ballot(hit);
+        if (lane == 0) next[w] = bits;
+    }
+    nedges = g5_wave_sum(nedges);
+    nverts = g5_wave_sum(nverts);
+    if (lane == 0) {
+        s_e[threadIdx.x >> 6] = nedges;
+        s_v[threadIdx.x >> 6] = nverts;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int64_t te = s_e[0] + s_e[1] + s_e[2] + s_e[3], tv = s_v[0] + s_v[1] + s_v[2] + s_v[3];
+        if (te) atomicAdd((unsigned long long *)&cnt[kBuEdges], (unsigned long long)te);
+        if (tv) atomicAdd((unsigned long long *)&cnt[kBuVerts], (unsigned long long)tv);
+    }
 }
 
 __global__ void k_count_bits(const uint64_t *__restrict__ bits, int64_t n, int64_t *__restrict__ dst)
@@ -148,13 +229,14 @@ __global__ void k_count_bits(const uint64_t *__restrict__ bits, int64_t n, int64
     if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
 }
 
-__global__ void k_start(int64_t root_row, int64_t root_col, uint32_t *queue, uint64_t *visited,
+__global__ void k_start(int64_t root_row, int64_t root_col, uint32_t *queue, uint64_t *visited, uint64_t *out,
                         int32_t *level, uint32_t *parent, int64_t *cnt)
 {
     cnt[kQLen] = root_row >= 0 ? 1 : 0;
     if (root_row >= 0) queue[0] = (uint32_t)root_row;
     if (root_col >= 0) {
         visited[root_col >> 6] = 1ull << (root_col & 63);
+        out[root_col >> 6] = 1ull << (root_col & 63);  // the frontier of a bottom-up first level
         level[root_col] = 0;
         parent[root_col] = kRootMark;
     }
@@ -225,6 +307,23 @@ int account_level(g5_bfs *b, int64_t qlen, int64_t n_new)
     return G5_OK;
 }
 
+// After a bottom-up level: k_bottom_up time and its algorithmic bytes, 16 B
+// row-offset pair per unvisited vertex scanned, 4 B column + 8 B frontier
+// word per neighbour checked, 4 B parent per newly reached vertex, 16 B
+// (visited read + next write) per 64-vertex word.
+int account_bottom_up(g5_bfs *b, int64_t n_new)
+{
+    if (!b->prof) return G5_OK;
+    float ms = 0;
+    G5_HIP(hipEventElapsedTime(&ms, b->ek0, b->ek1));
+    const int64_t E = b->h_cnt[kBuEdges], V = b->h_cnt[kBuVerts];
+    b->bu_ms += ms;
+    b->bu_launches += 1;
+    b->bu_edges += E;
+    b->bu_bytes += 16.0 * V + 12.0 * E + 4.0 * n_new + 16.0 * b->wc;
+    return G5_OK;
+}
+
 int sync_counters(g5_bfs *b)
 {
     G5_HIP(hipMemcpyAsync(b->h_cnt, b->d_cnt, kNCnt * 8, hipMemcpyDeviceToHost, b->s));
@@ -244,7 +343,7 @@ int reset_search(g5_bfs *b, int64_t root)
     G5_HIP(g5::fill_u32(b->d_parent, g->n_cols, kNoParent, s));
     G5_HIP(g5::fill_i32(b->d_level, g->n_cols, -1, s));
     G5_HIP(g5::fill_i64(b->d_cnt, kNCnt, 0, s));
-    hipLaunchKernelGGL(k_start, 1, 1, 0, s, root_row, root_col, b->d_queue, b->d_visited, b->d_level,
+    hipLaunchKernelGGL(k_start, 1, 1, 0, s, root_row, root_col, b->d_queue, b->d_visited, b->d_out, b->d_level,
                        b->d_parent, b->d_cnt);
     G5_HIP(hipGetLastError());
     b->root = root;
@@ -271,6 +370,7 @@ extern "C" int g5_bfs_create(g5_graph *g, g5_bfs **out)
     const size_t scr = g5::scan_scratch_words(nmax + 1) + g5::scan_scratch_words(b->wc + 1) + 64;
     if (e == hipSuccess) e = hipMalloc(&b->d_visited, (size_t)std::max<int64_t>(b->wc, 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&b->d_out, (size_t)std::max<int64_t>(b->wc, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_next, (size_t)std::max<int64_t>(b->wc, 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&b->d_parent, (size_t)std::max<int64_t>(g->n_cols, 1) * 4);
     if (e == hipSuccess) e = hipMalloc(&b->d_level, (size_t)std::max<int64_t>(g->n_cols, 1) * 4);
     if (e == hipSuccess) e = hipMalloc(&b->d_queue, (size_t)std::max<int64_t>(nmax, 1) * 4);
@@ -292,7 +392,7 @@ extern "C" int g5_bfs_free(g5_bfs *b)
 {
     if (!b) return G5_OK;
     if (b->s) (void)hipStreamSynchronize(b->s);
-    void *dev[] = {b->d_visited, b->d_out, b->d_parent, b->d_level, b->d_queue,
+    void *dev[] = {b->d_visited, b->d_out, b->d_next, b->d_parent, b->d_level, b->d_queue,
                    b->d_outq,    b->d_qoff, b->d_scratch, b->d_cnt, b->d_words};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -371,7 +471,8 @@ extern "C" int g5_bfs_set_outqueue(g5_bfs *b, const uint32_t *q, int64_t n)
     }
     hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
     hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
-                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt,
+                       (const int64_t *)nullptr);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(b->s));
     return G5_OK;
@@ -440,7 +541,8 @@ extern "C" int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits)
         G5_HIP(hipMemcpyAsync(b->d_out, d_bits, (size_t)b->wc * 8, hipMemcpyDeviceToDevice, b->s));
     hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
     hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
-                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+                       b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt,
+                       (const int64_t *)nullptr);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(b->s));
     return G5_OK;
@@ -493,41 +595,66 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
                "g5_bfs_run: the resident loop needs a diagonal (1x1) block");
     G5_REQUIRE(root >= 0 && root < g->n_rows, G5_E_ARG, "g5_bfs_run: root %lld out of range", (long long)root);
     const bool codec = (flags & G5_RUN_CODEC) != 0;
+    const bool topdown_only = (flags & G5_RUN_TOPDOWN) != 0;
     if (codec && !b->d_words) {
         b->words_cap = g5_s4_max_words((size_t)g->n_cols);
         G5_HIP(hipMalloc(&b->d_words, b->words_cap * 4));
         int rc = b->codec.reserve((int64_t)g->n_cols);
         if (rc) return rc;
     }
-    G5_HIP(hipStreamSynchronize(b->s));
-    G5_HIP(hipEventRecord(b->ev0, b->s));
+    hipStream_t s = b->s;
+    G5_HIP(hipStreamSynchronize(s));
+    G5_HIP(hipEventRecord(b->ev0, s));
     int rc = reset_search(b, root);
     if (rc) return rc;
+    const int64_t n = g->n_rows;
+    int64_t m_u = g->nnz;  // edges of the vertices not yet reached (the root's own are not subtracted)
+    bool bu = false;       // the first level is always top-down
     for (;;) {
-        rc = expand_level(b);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
-        hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out,
-                           b->d_visited, b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt);
+        G5_HIP(g5::fill_i64(b->d_cnt + kNew, kBuVerts - kNew + 1, 0, s));
+        uint64_t *fresh = b->d_out;
+        if (bu) {
+            if (b->prof) G5_HIP(hipEventRecord(b->ek0, s));
+            hipLaunchKernelGGL(k_bottom_up, g5::grid_for(b->wc, 4, 16384), 256, 0, s, (const int64_t *)g->d_rowptr,
+                               (const uint32_t *)g->d_cols, (const uint64_t *)b->d_visited,
+                               (const uint64_t *)b->d_out, b->d_next, b->d_parent, b->wc, b->d_cnt);
+            G5_HIP(hipGetLastError());
+            if (b->prof) G5_HIP(hipEventRecord(b->ek1, s));
+            fresh = b->d_next;
+        } else {
+            rc = expand_level(b);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, s, fresh, b->d_visited, b->d_level,
+                           b->wc, (int32_t)(b->depth + 1), b->d_cnt, (const int64_t *)g->d_rowptr);
         G5_HIP(hipGetLastError());
-        G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
+        if (bu) std::swap(b->d_out, b->d_next);
+        G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, s));
         rc = sync_counters(b);
         if (rc) return rc;
-        rc = account_level(b, b->qlen, b->h_cnt[kNew]);
+        const int64_t n_new = b->h_cnt[kNew];
+        rc = bu ? account_bottom_up(b, n_new) : account_level(b, b->qlen, n_new);
         if (rc) return rc;
-        if (b->h_cnt[kNew] == 0) break;
+        if (n_new == 0) break;
         b->qlen = b->h_cnt[kQLen];
         if (codec) {
             size_t nw = 0, nd = 0;
-            rc = b->codec.encode(b->d_queue, b->qlen, b->d_words, b->words_cap, &nw, b->s);
+            rc = b->codec.encode(b->d_queue, b->qlen, b->d_words, b->words_cap, &nw, s);
             if (rc) return rc;
-            rc = b->codec.decode(b->d_words, nw, b->d_queue, (size_t)g->n_rows, &nd, b->s);
+            rc = b->codec.decode(b->d_words, nw, b->d_queue, (size_t)g->n_rows, &nd, s);
             if (rc) return rc;
             G5_REQUIRE((int64_t)nd == b->qlen, G5_E_STREAM, "g5_bfs_run: codec round trip lost values");
         }
+        // the next level's direction
+        const int64_t m_f = b->h_cnt[kNewEdges];
+        m_u -= m_f;
+        if (!topdown_only) {
+            if (!bu && m_f > m_u / kAlpha) bu = true;
+            else if (bu && n_new < n / kBeta) bu = false;
+        }
         b->depth++;
     }
-    G5_HIP(hipEventRecord(b->ev1, b->s));
+    G5_HIP(hipEventRecord(b->ev1, s));
     G5_HIP(hipEventSynchronize(b->ev1));
     if (ms) G5_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
     if (nlevels) *nlevels = b->depth + 1;
@@ -540,6 +667,20 @@ extern "C" int g5_bfs_profile(g5_bfs *b, int enable)
     b->prof = enable != 0;
     b->exp_ms = b->exp_bytes = 0;
     b->exp_launches = b->exp_edges = 0;
+    b->bu_ms = b->bu_bytes = 0;
+    b->bu_launches = b->bu_edges = 0;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_kernel_stats(g5_bfs *b, int kernel, double *ms, int64_t *launches, double *bytes,
+                                   int64_t *edges)
+{
+    G5_REQUIRE(b && (kernel == 0 || kernel == 1), G5_E_ARG, "g5_bfs_kernel_stats: kernel must be 0 or 1");
+    if (kernel == 0) return g5_bfs_stats(b, ms, launches, bytes, edges);
+    if (ms) *ms = b->bu_ms;
+    if (launches) *launches = b->bu_launches;
+    if (bytes) *bytes = b->bu_bytes;
+    if (edges) *edges = b->bu_edges;
     return G5_OK;
 }
 

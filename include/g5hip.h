@@ -141,10 +141,16 @@ int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred);
 /* The whole 1x1 runBFS (globalbfs.hh:548-1263) resident in HBM: no host
- * round trip per level.  flags: G5_RUN_CODEC routes every level's queue
- * through s4-bp128-d4 encode -> decode on the device, as the row broadcast
- * delivers it to non-root ranks (globalbfs.hh:864-905). *ms = device time. */
-enum { G5_RUN_CODEC = 1 };
+ * round trip per level.  Dense levels run bottom-up (every unvisited vertex
+ * stops at its first -- minimum-id -- frontier neighbour), chosen by the
+ * direction switch of Beamer et al. (SC'12: bottom-up once the frontier's
+ * edges exceed the unexplored edges / 14, top-down again below n / 24
+ * frontier vertices); results are identical in both directions.  flags:
+ * G5_RUN_CODEC routes every level's queue through s4-bp128-d4 encode ->
+ * decode on the device, as the row broadcast delivers it to non-root ranks
+ * (globalbfs.hh:864-905); G5_RUN_TOPDOWN keeps every level top-down.
+ * *ms = device time. */
+enum { G5_RUN_CODEC = 1, G5_RUN_TOPDOWN = 2 };
 int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
 /* Per-kernel accounting of the expansion kernel across g5_bfs_run and
  * g5_bfs_expand calls:
@@ -153,6 +159,10 @@ int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
  * per newly reached vertex) and frontier edges. */
 int g5_bfs_profile(g5_bfs *b, int enable);
 int g5_bfs_stats(g5_bfs *b, double *ms, int64_t *launches, double *bytes, int64_t *edges);
+/* The same per kernel: 0 = k_expand (as g5_bfs_stats), 1 = k_bottom_up (16 B
+ * per unvisited vertex scanned, 12 B per neighbour checked, 4 B per newly
+ * reached vertex, 16 B per 64-vertex word; edges = neighbours checked). */
+int g5_bfs_kernel_stats(g5_bfs *b, int kernel, double *ms, int64_t *launches, double *bytes, int64_t *edges);
 /* TEPS numerator of the last search: generated undirected edges with a
  * reached endpoint (validate/validate.h:439-485). */
 int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);

@@ -40,6 +40,28 @@ def test_resident_all_roots_bit_exact(scale):
     assert bad == 0 and np.array_equal(depth, lev)
 
 
+@pytest.mark.parametrize("scale", [14, 18])
+def test_directions_identical(scale):
+    """Direction-optimised (bottom-up dense levels) and top-down-only searches
+    give the same levels and parents, equal to the oracle's; the bottom-up
+    kernel really ran."""
+    e, ref, g = kron(scale)
+    roots = O.select_roots(ref, 1 << scale)[:16]
+    b = g5.BFS(g)
+    b.profile(True)
+    for r in roots.tolist():
+        nl_a, _ = b.run(r)
+        pa, la = b.result()
+        nl_b, _ = b.run(r, topdown=True)
+        pb, lb = b.result()
+        opred, olev, onl = O.bfs_bin(ref, r)
+        assert nl_a == nl_b == onl
+        assert np.array_equal(la, olev) and np.array_equal(lb, olev), f"levels differ, root {r}"
+        assert np.array_equal(pa, opred) and np.array_equal(pb, opred), f"parents differ, root {r}"
+    ks = b.kernel_stats()
+    assert ks["k_bottom_up"]["launches"] > 0 and ks["k_expand"]["launches"] > 0
+
+
 def test_hook_path_matches_oracle_levels_and_queues():
     e, ref, g = kron(12)
     roots = O.select_roots(ref, 1 << 12)[:6]
