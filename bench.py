@@ -248,9 +248,16 @@ def bench_c4(args, g5, traffic, traffic_info):
 
     # ---- batched multi-source search over the same roots ----
     m = g5.MSBFS(g)
-    log(f"msbfs degree bins (>2048, 33..2048, 1..32): {m.bins()}")
     for _ in range(max(args.warmup, 1)):
         m.run(roots)
+    tuned = None
+    if args.autotune == "on":
+        t0 = time.perf_counter()
+        best_ms, trials, params = m.autotune(roots)
+        tuned = {"params": params, "candidates_run": trials, "best_ms": best_ms,
+                 "seconds": time.perf_counter() - t0}
+        log(f"autotune: {params} after {trials} candidates, {best_ms:.2f} ms")
+    log(f"msbfs degree bins (workgroup, wavefront, thread per vertex): {m.bins()}")
     m.profile(True)            # one run as direct launches with per-kernel events
     m.run(roots)
     kst = m.stats()
@@ -314,6 +321,7 @@ def bench_c4(args, g5, traffic, traffic_info):
         "single_root_mean_ms": float(np.mean(single_ms)),
         "single_root_kernel": single_kernel,
         "roofline": roofline,
+        "autotune": tuned,
     }
     m.free()
     if args.cpu_baseline == "on":
@@ -432,6 +440,8 @@ def main():
     ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
                     help="c4: measure roofline.traffic with rocprofv3 --pmc child passes")
     ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="c4: pick the batched search's shape parameters before timing")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.scale is None:

@@ -94,6 +94,9 @@ def lib():
     _sig(L, "g5_msbfs_levels", I, [P, P, I, ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_set_mode", I, [P, I])
     _sig(L, "g5_msbfs_open", I, [P])
+    _sig(L, "g5_msbfs_tune", I, [P, I, c_i64])
+    _sig(L, "g5_msbfs_get_tune", I, [P, I, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_msbfs_autotune", I, [P, P, I, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_join", I, [P, P, P, I])
     _sig(L, "g5_msbfs_step", I, [P, ctypes.POINTER(c_u64), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_msbfs_leave", I, [P, c_u64, P, P])
@@ -349,6 +352,36 @@ class MSBFS:
         lev = np.empty(self.graph.n_cols, np.int32)
         check(lib().g5_msbfs_result(self.h, lane, _p(pred), _p(lev)))
         return pred, lev
+
+    # ---- shape parameters ----
+    TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3}
+
+    def tune(self, **params):
+        """Set shape parameters; the bin bounds are applied in the order that
+        keeps thread_max <= wave_max at every step."""
+        keys = list(params)
+        if "thread_max" in params and "wave_max" in params:
+            cur_wave = self.get_tune()["wave_max"]
+            first = "wave_max" if params["thread_max"] > cur_wave else "thread_max"
+            keys = [first] + [k for k in keys if k != first]
+        for k in keys:
+            check(lib().g5_msbfs_tune(self.h, self.TUNE_KEYS[k], int(params[k])))
+
+    def get_tune(self):
+        out = {}
+        for k, key in self.TUNE_KEYS.items():
+            v = c_i64()
+            check(lib().g5_msbfs_get_tune(self.h, key, ctypes.byref(v)))
+            out[k] = v.value
+        return out
+
+    def autotune(self, roots, passes=1):
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        best = ctypes.c_double()
+        trials = ctypes.c_int()
+        check(lib().g5_msbfs_autotune(self.h, _p(r), r.size, passes, ctypes.byref(best), ctypes.byref(trials)))
+        self.nroots = r.size
+        return best.value, trials.value, self.get_tune()
 
     # ---- continuous mode ----
     def open(self):

@@ -46,20 +46,23 @@
 
 namespace {
 
-constexpr int kMediumMax = 2048;
-// degree bins in launch order: one workgroup per vertex (> 2048), one
-// wavefront per vertex (33..2048), one thread per vertex (1..32)
+// degree bins in launch order: one workgroup per vertex (degree above
+// wave_max), one wavefront per vertex (thread_max < degree <= wave_max), one
+// thread per vertex (1..thread_max); the bounds are tunable (Tune)
 constexpr int kBins = 3;
-constexpr int kBinMaxDeg[kBins] = {1 << 30, 2048, 32};
 constexpr int kPushUnit = kBins;  // accounting unit of push levels
 constexpr int kUnits = kBins + 1;
 constexpr int kBlock = 256;
 constexpr int kLevelRec = 12;  // per level: ms[4], edges[4], new pairs, pushed, frontier vertices, edges
-// push when the frontier's edges are fewer than nnz / kPushRatio
-constexpr int64_t kPushRatio = 16;
-// pull levels consult the active bitmap when the frontier's edges are fewer
-// than nnz / kActiveRatio (denser frontiers make nearly every check a hit)
-constexpr int64_t kActiveRatio = 2;
+// Shape parameters (g5_msbfs_tune / g5_msbfs_autotune).  Any values give the
+// same results; they only move work between kernels.
+struct Tune {
+    int64_t thread_max = 32;   // degree bound of the thread-per-vertex bin
+    int64_t wave_max = 2048;   // degree bound of the wavefront-per-vertex bin
+    int64_t push_ratio = 16;   // push when the frontier's edges < nnz / push_ratio
+    int64_t active_ratio = 2;  // pull levels consult the active bitmap when the
+                               // frontier's edges < nnz / active_ratio
+};
 
 // Level counters.  Per accounting unit u (the kBins pull bins, then
 // kPushUnit) kUnitSlots accumulated slots at u * kUnitSlots: new (vertex,
@@ -119,6 +122,7 @@ struct g5_msbfs {
     // per level of the last profiled run (kLevelRec doubles each)
     std::vector<double> lvl;
     int mode = 0;  // 0 direction chosen per level, 1 pull only, 2 push only
+    Tune tune;
     // level state: the parity of the frontier the next level reads and the
     // frontier's active vertices / edges (edges < 0: unknown, take push)
     int p = 0;
@@ -699,17 +703,25 @@ __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const 
 }
 
 // Degree-order keys: (2^32-1 - degree) << 32 | id, so an ascending sort
-// gives descending degree, ties by id; also counts the degree bins.
-__global__ void k_degree_keys(const int64_t *__restrict__ rowptr, int64_t n, uint64_t *__restrict__ keys,
-                              unsigned long long *__restrict__ bins)
+// gives descending degree, ties by id.
+__global__ void k_degree_keys(const int64_t *__restrict__ rowptr, int64_t n, uint64_t *__restrict__ keys)
 {
-    unsigned long long c[kBins] = {0, 0, 0};
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
-         v += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = rowptr[v + 1] - rowptr[v];
         keys[v] = ((uint64_t)(0xFFFFFFFFu - (uint32_t)d) << 32) | (uint64_t)v;
-        for (int k = 0; k < kBins; ++k)
-            if (d > 0 && d <= kBinMaxDeg[k] && (k + 1 == kBins || d > kBinMaxDeg[k + 1])) ++c[k];
+    }
+}
+
+// vertices with degree > wave_max, in (thread_max, wave_max], in (0, thread_max]
+__global__ void k_bin_counts(const int64_t *__restrict__ rowptr, int64_t n, int64_t thread_max, int64_t wave_max,
+                             unsigned long long *__restrict__ bins)
+{
+    unsigned long long c[kBins] = {0, 0, 0};
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = rowptr[v + 1] - rowptr[v];
+        if (d > wave_max) ++c[0];
+        else if (d > thread_max) ++c[1];
+        else if (d > 0) ++c[2];
     }
     for (int k = 0; k < kBins; ++k) {
         const unsigned long long t = (unsigned long long)g5_wave_sum((int64_t)c[k]);
@@ -1002,27 +1014,23 @@ int build_relabel(g5_msbfs *m)
     Relabeled &R = m->rl;
     const int64_t n = m->n;
     uint64_t *keys = nullptr, *tmp = nullptr;
-    unsigned long long *bins = nullptr;
     int64_t *ndeg = nullptr;
     auto cleanup = [&]() {
         if (keys) (void)hipFree(keys);
         if (tmp) (void)hipFree(tmp);
-        if (bins) (void)hipFree(bins);
         if (ndeg) (void)hipFree(ndeg);
     };
     hipError_t e = hipMalloc(&keys, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&tmp, (size_t)n * 8);
-    if (e == hipSuccess) e = hipMalloc(&bins, kBins * 8);
     if (e == hipSuccess) e = hipMalloc(&ndeg, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.new2old, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.old2new, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.gdeg, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
-    if (e == hipSuccess) e = hipMemsetAsync(bins, 0, kBins * 8, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_degree_keys, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)g->d_rowptr, n,
-                           keys, bins);
+                           keys);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = g5::sort_u64(keys, tmp, n, 64, s);
@@ -1039,14 +1047,39 @@ int build_relabel(g5_msbfs *m)
                            (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols);
         e = hipGetLastError();
     }
-    unsigned long long hb[kBins] = {0, 0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(hb, bins, kBins * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     if (e != hipSuccess) return g5::fail_hip(e, "msbfs relabel", __FILE__, __LINE__);
+    return G5_OK;
+}
+
+// The degree bins of the relabelled order for the current Tune bounds
+// (contiguous id ranges, degrees descending).
+int compute_bins(g5_msbfs *m)
+{
+    hipStream_t s = m->s;
+    Relabeled &R = m->rl;
+    unsigned long long *d = (unsigned long long *)m->d_sums;  // transient scratch, free between calls
+    unsigned long long hb[kBins] = {0, 0, 0};
+    G5_HIP(g5::fill_u64((uint64_t *)d, kBins, 0, s));
+    hipLaunchKernelGGL(k_bin_counts, g5::grid_for(m->n, 256, 16384), 256, 0, s, (const int64_t *)R.rowptr, m->n,
+                       m->tune.thread_max, m->tune.wave_max, d);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(hb, d, kBins * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
     R.bin_first[0] = 0;
     for (int k = 0; k < kBins; ++k) R.bin_first[k + 1] = R.bin_first[k] + (int64_t)hb[k];
     return G5_OK;
+}
+
+void destroy_graphs(g5_msbfs *m)
+{
+    for (auto &gp : m->graph)
+        for (hipGraphExec_t &ge : gp)
+            if (ge) {
+                (void)hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
 }
 
 // One level at parity m->p: push or pull by the frontier's edges, then the
@@ -1058,8 +1091,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     const int p = m->p;
     const int64_t act_verts = m->act_verts, act_edges = m->act_edges;
     const int64_t nnz = m->g->nnz;
-    const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * kPushRatio < nnz));
-    const bool act_check = act_edges < 0 || act_edges * kActiveRatio < nnz;
+    const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * m->tune.push_ratio < nnz));
+    const bool act_check = act_edges < 0 || act_edges * m->tune.active_ratio < nnz;
     if (push) {
         int rc = push_level(m, s, p, act_verts, m->prof);
         if (rc) return rc;
@@ -1165,6 +1198,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     for (int k = 0; k <= kBins && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_msbfs_create allocation", __FILE__, __LINE__));
     int rc = build_relabel(m);
+    if (rc) return fail(rc);
+    rc = compute_bins(m);
     if (rc) return fail(rc);
     rc = capture_graphs(m);
     if (rc) return fail(rc);
@@ -1456,5 +1491,118 @@ extern "C" int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev)
     *nlev = n;
     if (out)
         for (int i = 0; i < std::min(n, cap) * kLevelRec; ++i) out[i] = m->lvl[i];
+    return G5_OK;
+}
+
+namespace {
+
+int64_t *tune_slot(g5_msbfs *m, int key)
+{
+    switch (key) {
+    case G5_TUNE_THREAD_MAX: return &m->tune.thread_max;
+    case G5_TUNE_WAVE_MAX: return &m->tune.wave_max;
+    case G5_TUNE_PUSH_RATIO: return &m->tune.push_ratio;
+    case G5_TUNE_ACTIVE_RATIO: return &m->tune.active_ratio;
+    default: return nullptr;
+    }
+}
+
+// Installs t: new degree bins and re-captured level graphs when the bounds move.
+int apply_tune(g5_msbfs *m, const Tune &t)
+{
+    const bool rebin = t.thread_max != m->tune.thread_max || t.wave_max != m->tune.wave_max;
+    m->tune = t;
+    if (!rebin) return G5_OK;
+    G5_HIP(hipStreamSynchronize(m->s));
+    int rc = compute_bins(m);
+    if (rc) return rc;
+    destroy_graphs(m);
+    return capture_graphs(m);
+}
+
+}  // namespace
+
+extern "C" int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value)
+{
+    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_tune: null handle");
+    int64_t *slot = tune_slot(m, key);
+    G5_REQUIRE(slot, G5_E_ARG, "g5_msbfs_tune: unknown key %d", key);
+    Tune t = m->tune;
+    switch (key) {
+    case G5_TUNE_THREAD_MAX: t.thread_max = value; break;
+    case G5_TUNE_WAVE_MAX: t.wave_max = value; break;
+    case G5_TUNE_PUSH_RATIO: t.push_ratio = value; break;
+    default: t.active_ratio = value; break;
+    }
+    G5_REQUIRE(t.thread_max >= 0 && t.wave_max >= t.thread_max && t.wave_max <= (int64_t)1 << 40, G5_E_ARG,
+               "g5_msbfs_tune: need 0 <= thread_max <= wave_max");
+    G5_REQUIRE(t.push_ratio >= 1 && t.active_ratio >= 1, G5_E_ARG, "g5_msbfs_tune: ratios must be >= 1");
+    return apply_tune(m, t);
+}
+
+extern "C" int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value)
+{
+    G5_REQUIRE(m && value, G5_E_ARG, "g5_msbfs_get_tune: null argument");
+    int64_t *slot = tune_slot(m, key);
+    G5_REQUIRE(slot, G5_E_ARG, "g5_msbfs_get_tune: unknown key %d", key);
+    *value = *slot;
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, int passes, double *best_ms,
+                                 int *trials)
+{
+    G5_REQUIRE(m && roots && nroots >= 1 && nroots <= 64 && passes >= 1 && passes <= 4, G5_E_ARG,
+               "g5_msbfs_autotune: bad arguments");
+    G5_REQUIRE(!(m->cont && m->busy), G5_E_STATE, "g5_msbfs_autotune: lanes in use");
+    static const int64_t cand[4][6] = {{8, 16, 32, 64, 128, 0},
+                                       {512, 1024, 2048, 4096, 8192, 16384},
+                                       {4, 8, 16, 32, 64, 0},
+                                       {1, 2, 4, 8, 0, 0}};
+    const bool prof = m->prof;
+    m->prof = false;
+    int n_trials = 0;
+    // a configuration's time: the better of two searches
+    auto measure = [&](const Tune &t, double *out) -> int {
+        int rc = apply_tune(m, t);
+        if (rc) return rc;
+        double best = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            int32_t nl = 0;
+            float ms = 0;
+            rc = g5_msbfs_run(m, roots, nroots, &nl, &ms);
+            if (rc) return rc;
+            best = std::min(best, (double)ms);
+        }
+        ++n_trials;
+        *out = best;
+        return G5_OK;
+    };
+    Tune best = m->tune;
+    double best_t = 0;
+    int rc = measure(best, &best_t);
+    for (int pass = 0; pass < passes && !rc; ++pass)
+        for (int key = 0; key < 4 && !rc; ++key)
+            for (int i = 0; i < 6 && !rc; ++i) {
+                const int64_t v = cand[key][i];
+                if (v == 0) continue;
+                Tune t = best;
+                int64_t *f = key == 0 ? &t.thread_max : key == 1 ? &t.wave_max : key == 2 ? &t.push_ratio
+                                                                                          : &t.active_ratio;
+                if (*f == v) continue;
+                *f = v;
+                if (t.wave_max < t.thread_max) continue;
+                double ms = 0;
+                rc = measure(t, &ms);
+                if (!rc && ms < best_t * 0.99) {
+                    best = t;
+                    best_t = ms;
+                }
+            }
+    if (!rc) rc = apply_tune(m, best);
+    m->prof = prof;
+    if (rc) return rc;
+    if (best_ms) *best_ms = best_t;
+    if (trials) *trials = n_trials;
     return G5_OK;
 }

@@ -214,6 +214,19 @@ int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
  *                   mix64(v << 32 | (pred(v) + 1)) mod 2^64 (original ids;
  *                   mix64 = SplitMix64's finaliser); then the lanes are free.
  *                   g5_msbfs_result may be called for a lane before it leaves. */
+/* Shape parameters of the batched search (results never depend on them):
+ * G5_TUNE_THREAD_MAX / G5_TUNE_WAVE_MAX bound the degree bins of the
+ * thread-per-vertex and wavefront-per-vertex pull kernels (default 32, 2048;
+ * higher degrees take a workgroup per vertex); G5_TUNE_PUSH_RATIO pushes a
+ * level whose frontier edges are below nnz / ratio (16); G5_TUNE_ACTIVE_RATIO
+ * makes pull levels consult the active bitmap below nnz / ratio edges (2).
+ * g5_msbfs_autotune searches these by coordinate descent over candidate
+ * values (passes rounds, two timed batch searches of roots per candidate),
+ * keeps the fastest and reports its time and the number of candidates run. */
+enum { G5_TUNE_THREAD_MAX = 0, G5_TUNE_WAVE_MAX = 1, G5_TUNE_PUSH_RATIO = 2, G5_TUNE_ACTIVE_RATIO = 3 };
+int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value);
+int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value);
+int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, int passes, double *best_ms, int *trials);
 int g5_msbfs_open(g5_msbfs *m);
 int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes, int n);
 int g5_msbfs_step(g5_msbfs *m, uint64_t *live, float *ms);

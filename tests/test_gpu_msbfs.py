@@ -113,3 +113,54 @@ def test_bad_roots_rejected():
         m.run(np.arange(65))
     with pytest.raises(g5.G5Error):
         m.run([ref.n_rows])
+
+
+def _check_all_lanes(m, ref, roots):
+    ne = m.nedge()
+    for lane, r in enumerate(roots.tolist()):
+        pred, lev = m.result(lane)
+        opred, olev, _ = O.bfs_bin(ref, r)
+        assert np.array_equal(lev, olev), f"levels differ, lane {lane} root {r}"
+        assert np.array_equal(pred, opred), f"parents differ, lane {lane} root {r}"
+        assert ne[lane] == O.nedge(ref, olev)
+
+
+@pytest.mark.parametrize("shape", [
+    dict(thread_max=0, wave_max=0),              # every vertex in the workgroup kernel
+    dict(thread_max=1 << 20, wave_max=1 << 20),  # every vertex in the thread kernel
+    dict(thread_max=0, wave_max=1 << 20),        # every vertex in the wavefront kernel
+    dict(thread_max=4, wave_max=64, push_ratio=1, active_ratio=1),
+    dict(push_ratio=1000, active_ratio=1000),
+])
+def test_shape_parameters_do_not_change_results(shape):
+    e, ref, g = kron(14)
+    roots = O.select_roots(ref, 1 << 14)
+    m = g5.MSBFS(g)
+    m.tune(**shape)
+    got = m.get_tune()
+    for k, v in shape.items():
+        assert got[k] == v
+    m.run(roots)
+    _check_all_lanes(m, ref, roots)
+
+
+def test_autotune_keeps_results():
+    e, ref, g = kron(13)
+    roots = O.select_roots(ref, 1 << 13)
+    m = g5.MSBFS(g)
+    best, trials, params = m.autotune(roots)
+    assert best > 0 and trials > 10
+    assert params == m.get_tune()
+    m.run(roots)
+    _check_all_lanes(m, ref, roots)
+
+
+def test_tune_rejects_bad_values():
+    e, ref, g = kron(10)
+    m = g5.MSBFS(g)
+    for bad in (dict(thread_max=-1), dict(thread_max=100, wave_max=50), dict(push_ratio=0), dict(active_ratio=0)):
+        m2 = g5.MSBFS(g)
+        with pytest.raises(g5.G5Error):
+            m2.tune(**bad)
+    with pytest.raises(g5.G5Error):
+        g5.check(g5.lib().g5_msbfs_tune(m.h, 9, 1))
