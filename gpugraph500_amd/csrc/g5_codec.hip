@@ -18,9 +18,14 @@
 // exclusive scan of 4*b gives every miniblock's position; a second pass packs
 // (each lane assembles whole output words from the deltas staged in LDS, so
 // stores are coalesced and need no atomics); one wave writes the varint tail.
-// Decode: one workgroup walks the group headers through LDS-staged chunks
-// (their positions depend on all earlier widths), then one wave per
-// miniblock unpacks; the 4-lane running sums across miniblocks come from an
+// Decode: the group headers' positions depend on all earlier widths.  Short
+// streams (<= 64 groups) walk them in one workgroup through LDS-staged
+// chunks; longer ones find them in parallel: every word index 4 + 4i is a
+// candidate group start (groups are 4 + 4 * sum(widths) words), its next
+// start follows from its own 16 widths, 5 rounds of pointer doubling give
+// the 32-step jump, one thread walks the anchors (every 32nd start) and one
+// thread per anchor fills its 32 starts.  Then one wave per miniblock
+// unpacks; the 4-lane running sums across miniblocks come from an
 // exclusive scan of per-miniblock (reset, sum) records, where b = 32
 // miniblocks reset the running value to their raw last four values.
 #include "g5_codec.hpp"
@@ -326,6 +331,106 @@ __global__ __launch_bounds__(256) void k_carry_apply(uint32_t *__restrict__ c, i
     if (i < n) store_carry(c, i, ex);
 }
 
+// ---- parallel group-header search (streams of more than kSerialGroups groups) ----
+constexpr int64_t kSerialGroups = 64;
+constexpr int kJumpLog = 5;  // anchors every 2^5 groups
+constexpr int32_t kBadWidth = -1, kTruncated = -2;
+
+// nxt[i] = index of the group start after candidate start p = 4 + 4i, or
+// kBadWidth / kTruncated if a group header at p is impossible
+__global__ void k_dec_next(const uint32_t *__restrict__ in, int64_t nwords, int64_t ncand, int32_t *__restrict__ nxt)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncand; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = 4 + 4 * i;
+        int32_t r = kTruncated;
+        if (p + 4 <= nwords) {
+            int64_t size = 4;
+            bool ok = true;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t x = in[p + w];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t b = (x >> (8 * k)) & 0xFF;
+                    ok = ok && b <= 32;
+                    size += 4 * b;
+                }
+            }
+            if (!ok) r = kBadWidth;
+            else if (p + size <= nwords) r = (int32_t)((p + size - 4) / 4);
+        }
+        nxt[i] = r;
+    }
+}
+
+// one doubling round: out[i] = in[in[i]] (negative entries propagate)
+__global__ void k_dec_jump(const int32_t *__restrict__ in, int64_t ncand, int32_t *__restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncand; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = in[i];
+        out[i] = (a < 0 || a >= ncand) ? (a < 0 ? a : kTruncated) : in[a];
+    }
+}
+
+__global__ void k_dec_set_len(int64_t *meta, int64_t L, int64_t ngroups)
+{
+    meta[kMetaL] = L;
+    meta[kMetaGroups] = ngroups;
+}
+
+// one thread: anchor[k] = start index of group 32k, k = 0..(ngroups-1)/32; the
+// jumps between anchors never cross the last group, whose unused widths are
+// not read
+__global__ void k_dec_anchors(const int32_t *__restrict__ jmp, int64_t nanch, int32_t *__restrict__ anchor,
+                              int64_t *__restrict__ meta)
+{
+    int32_t a = 0;
+    for (int64_t k = 0; k < nanch; ++k) {
+        anchor[k] = a;
+        if (k + 1 == nanch) break;
+        a = jmp[a];
+        if (a < 0) {
+            meta[kMetaErr] = a == kBadWidth ? 3 : 4;
+            return;
+        }
+    }
+}
+
+// thread k: group starts 32k .. 32k+31; the last group's own widths (only
+// the miniblocks it holds) give the tail position
+__global__ void k_dec_fill(const uint32_t *__restrict__ in, int64_t nwords, int64_t nblk,
+                           const int32_t *__restrict__ nxt, const int32_t *__restrict__ anchor, int64_t nanch,
+                           int64_t ngroups, int64_t ncand, int64_t *__restrict__ grp, int64_t *__restrict__ meta)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nanch || meta[kMetaErr]) return;
+    int32_t a = anchor[k];
+    for (int j = 0; j < (1 << kJumpLog); ++j) {
+        const int64_t g = (k << kJumpLog) + j;
+        if (g >= ngroups) break;
+        const int64_t p = 4 + 4 * (int64_t)a;
+        grp[g] = p;
+        if (g == ngroups - 1) {
+            int64_t err = p + 4 > nwords ? 4 : 0, size = 4;
+            const int64_t cnt = min((int64_t)16, nblk - 16 * g);
+            for (int64_t b = 0; b < cnt && !err; ++b) {
+                const uint32_t w = (in[p + (b >> 2)] >> (24 - 8 * (b & 3))) & 0xFF;
+                if (w > 32) err = 3;
+                size += 4 * w;
+            }
+            if (!err && p + size > nwords) err = 4;
+            if (err) atomicMax((unsigned long long *)&meta[kMetaErr], (unsigned long long)err);
+            else meta[kMetaTailPos] = p + size;
+            break;
+        }
+        a = a < ncand ? nxt[a] : kTruncated;
+        if (a < 0) {
+            atomicMax((unsigned long long *)&meta[kMetaErr], (unsigned long long)(a == kBadWidth ? 3 : 4));
+            return;
+        }
+    }
+}
+
 int64_t carry_scratch_records(int64_t n)
 {
     int64_t w = 0;
@@ -440,6 +545,11 @@ int S4Codec::reserve(int64_t n)
     if (e == hipSuccess) e = hipMalloc(&d_carry, (size_t)nblk * 20);
     if (e == hipSuccess) e = hipMalloc(&d_carry_scan, (size_t)(carry_scratch_records(nblk) + 2) * 20);
     if (e == hipSuccess) e = hipMalloc(&d_grp, (size_t)ngrp * 8);
+    ncand_cap = (int64_t)g5_s4_max_words((size_t)n) / 4 + 2;
+    if (e == hipSuccess) e = hipMalloc(&d_nxt, (size_t)ncand_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_jmp, (size_t)ncand_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_jtmp, (size_t)ncand_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_anchor, (size_t)(ngrp / 32 + 2) * 4);
     if (e == hipSuccess) e = hipMalloc(&d_meta, 16 * 8);
     if (e == hipSuccess) e = hipHostMalloc(&h_meta, 16 * 8, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -452,12 +562,14 @@ int S4Codec::reserve(int64_t n)
 
 void S4Codec::release()
 {
-    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta};
+    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta, d_nxt, d_jmp, d_jtmp, d_anchor};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (h_meta) (void)hipHostFree(h_meta);
     d_words = d_scan = d_off = d_grp = d_meta = nullptr;
     d_carry = d_carry_scan = nullptr;
+    d_nxt = d_jmp = d_jtmp = d_anchor = nullptr;
+    ncand_cap = 0;
     h_meta = nullptr;
     cap = 0;
 }
@@ -495,8 +607,34 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
     int rc = reserve((int64_t)cap_values);
     if (rc) return rc;
     G5_HIP(fill_i64(d_meta, 16, 0, s));
-    hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, (int64_t)nwords, (int64_t)cap_values, d_grp, d_meta);
-    G5_HIP(hipGetLastError());
+    // the stream header decides between the serial and the parallel header search
+    uint32_t hdr[4];
+    G5_HIP(hipMemcpyAsync(hdr, d_in, 16, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    const int64_t L0 = hdr[0], ngroups0 = (L0 / 128 + 15) / 16;
+    const bool header_ok = hdr[1] == kCookie && hdr[2] == kCookie && hdr[3] == kCookie && L0 % 128 == 0 &&
+                           L0 <= (int64_t)cap_values;
+    const int64_t ncand = ((int64_t)nwords - 4) / 4 + 1;
+    if (!header_ok || ngroups0 <= kSerialGroups || ncand > ncand_cap) {
+        hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, (int64_t)nwords, (int64_t)cap_values, d_grp, d_meta);
+        G5_HIP(hipGetLastError());
+    } else {
+        const int64_t nanch = ((ngroups0 - 1) >> kJumpLog) + 1;
+        const unsigned grid = (unsigned)grid_for(ncand, 256, 16384);
+        hipLaunchKernelGGL(k_dec_next, grid, 256, 0, s, d_in, (int64_t)nwords, ncand, d_nxt);
+        const int32_t *src = d_nxt;
+        int32_t *bufs[2] = {d_jmp, d_jtmp};
+        for (int r = 0; r < kJumpLog; ++r) {
+            hipLaunchKernelGGL(k_dec_jump, grid, 256, 0, s, src, ncand, bufs[r & 1]);
+            src = bufs[r & 1];
+        }
+        hipLaunchKernelGGL(k_dec_anchors, 1, 1, 0, s, src, nanch, d_anchor, d_meta);
+        hipLaunchKernelGGL(k_dec_fill, (unsigned)((nanch + 255) / 256), 256, 0, s, d_in, (int64_t)nwords,
+                           L0 / 128, (const int32_t *)d_nxt,
+                           (const int32_t *)d_anchor, nanch, ngroups0, ncand, d_grp, d_meta);
+        hipLaunchKernelGGL(k_dec_set_len, 1, 1, 0, s, d_meta, L0, ngroups0);
+        G5_HIP(hipGetLastError());
+    }
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_STREAM, "s4 decode: malformed stream (code %lld)",

@@ -14,6 +14,10 @@ struct S4Codec {
     uint32_t *d_carry = nullptr;   // decode: per-miniblock carry records (5 words)
     uint32_t *d_carry_scan = nullptr;
     int64_t *d_grp = nullptr;      // decode: group header positions
+    int32_t *d_nxt = nullptr;      // decode: next group start per candidate start (4 + 4i)
+    int32_t *d_jmp = nullptr, *d_jtmp = nullptr;  // decode: 32-step jumps (pointer doubling)
+    int32_t *d_anchor = nullptr;   // decode: every 32nd group start
+    int64_t ncand_cap = 0;
     int64_t *d_meta = nullptr;     // small device record
     int64_t *h_meta = nullptr;     // pinned mirror
 

@@ -57,3 +57,49 @@ def test_malformed_streams_rejected():
         g5.s4_decode(w[:6], len(x))  # truncated
     with pytest.raises(g5.G5Error):
         g5.s4_decode(w, 10)  # output capacity too small
+
+
+def _group_starts(w):
+    """Word index of every group header of a well-formed stream (test helper:
+    each header's 16 big-endian width bytes size its group)."""
+    nblk = int(w[0]) // 128
+    pos, out = 4, []
+    for g in range((nblk + 15) // 16):
+        out.append(pos)
+        cnt = min(16, nblk - 16 * g)
+        pos += 4 + 4 * sum((int(w[pos + (j >> 2)]) >> (24 - 8 * (j & 3))) & 0xFF for j in range(cnt))
+    return out, pos
+
+
+def test_malformed_long_streams_rejected():
+    """Streams of more than 64 groups take the parallel header search: a bad
+    width in any group, a truncation and a short output buffer still fail."""
+    rng = np.random.default_rng(7)
+    n = 400000
+    x = np.sort(rng.choice(1 << 26, size=n, replace=False)).astype(np.uint32)
+    w = g5.s4_encode(x)
+    starts, tail = _group_starts(w)
+    assert len(starts) > 64 and tail <= len(w)
+    for g in (0, 31, 32, 100, len(starts) - 1):
+        bad = w.copy()
+        bad[starts[g]] = (bad[starts[g]] & 0x00FFFFFF) | (40 << 24)  # width 40 > 32
+        with pytest.raises(g5.G5Error):
+            g5.s4_decode(bad, n)
+    for cut in (len(w) // 2, starts[-1] + 2, 4 + 4 * 70):
+        with pytest.raises(g5.G5Error):
+            g5.s4_decode(w[:cut], n)
+    with pytest.raises(g5.G5Error):
+        g5.s4_decode(w, n - 1)
+    assert np.array_equal(g5.s4_decode(w, n), x)  # the codec state is still good
+
+
+@pytest.mark.parametrize("nblk", [64 * 16, 64 * 16 + 1, 65 * 16, 96 * 16 + 5, 32 * 16 * 7])
+def test_group_count_boundaries(nblk):
+    """Group counts either side of the serial/parallel switch and of the
+    32-group anchor spacing, with a varint tail."""
+    rng = np.random.default_rng(nblk)
+    n = nblk * 128 + 77
+    x = np.sort(rng.choice(1 << 28, size=n, replace=False)).astype(np.uint32)
+    w = g5.s4_encode(x)
+    assert np.array_equal(w, O.s4_encode(x))
+    assert np.array_equal(g5.s4_decode(w, n), x)
