@@ -87,6 +87,13 @@ def lib():
     _sig(L, "g5_msbfs_free", I, [P])
     _sig(L, "g5_msbfs_run", I, [P, P, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_msbfs_result", I, [P, I, P, P])
+    _sig(L, "g5_msbfs_pred_device", I, [P, I, P])
+    _sig(L, "g5_validator_create", I, [P, ctypes.POINTER(c_vp)])
+    _sig(L, "g5_validator_free", I, [P])
+    _sig(L, "g5_validate", I, [P, c_i64, c_i64, P, ctypes.POINTER(I), ctypes.POINTER(c_i64), P,
+                               ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_validate_device", I, [P, c_i64, c_i64, P, ctypes.POINTER(I), ctypes.POINTER(c_i64), P,
+                                      ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_msbfs_nedge", I, [P, P])
     _sig(L, "g5_msbfs_profile", I, [P, I])
     _sig(L, "g5_msbfs_stats", I, [P, I, P, P, P])
@@ -280,6 +287,10 @@ class BFS:
         check(lib().g5_bfs_result(self.h, _p(pred), _p(lev)))
         return pred, lev
 
+    def pred_device(self, out):
+        """The predecessor map (global ids) into the DeviceArray `out` (int64[n_cols])."""
+        check(lib().g5_bfs_pred_device(self.h, out.ptr))
+
     RUN_CODEC, RUN_TOPDOWN = 1, 2
     KERNELS = ("k_expand", "k_bottom_up")
 
@@ -352,6 +363,10 @@ class MSBFS:
         lev = np.empty(self.graph.n_cols, np.int32)
         check(lib().g5_msbfs_result(self.h, lane, _p(pred), _p(lev)))
         return pred, lev
+
+    def pred_device(self, lane, out):
+        """Lane `lane`'s predecessor map into the DeviceArray `out` (int64[n])."""
+        check(lib().g5_msbfs_pred_device(self.h, lane, out.ptr))
 
     # ---- shape parameters ----
     TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3}
@@ -489,3 +504,44 @@ def vreduce_min_pred(preds):
     o = DeviceArray(n, np.int64)
     check(lib().g5_vreduce_min_pred(d.ptr, k, n, o.ptr))
     return o.to_host()
+
+
+BAD_FLAGS = {"range": 1, "root": 2, "self_parent": 4, "depth_map": 8, "edge_level": 16, "parent_edge": 32}
+
+
+class Validator:
+    """Graph500 result validation on the device (validate/validate.h:249-491)
+    for a whole (1x1) graph.  validate() returns (bad, nedge, depth, ms):
+    bad = 0 for a valid predecessor map, else the OR of BAD_FLAGS."""
+
+    def __init__(self, graph):
+        self.graph = graph
+        self.h = c_vp()
+        check(lib().g5_validator_create(graph.h, ctypes.byref(self.h)))
+
+    def _call(self, fn, root, pred_ptr, nglobal, want_depth):
+        bad, nedge, ms = ctypes.c_int(), c_i64(), ctypes.c_float()
+        depth = np.empty(self.graph.n_rows, np.int32) if want_depth else None
+        check(fn(self.h, root, self.graph.nglobal if nglobal is None else nglobal, pred_ptr, ctypes.byref(bad),
+                 ctypes.byref(nedge), _p(depth), ctypes.byref(ms)))
+        return bad.value, nedge.value, depth, ms.value
+
+    def validate(self, root, pred, nglobal=None, want_depth=False):
+        pred = np.ascontiguousarray(pred, dtype=np.int64)
+        if pred.size != self.graph.n_rows:
+            raise ValueError("pred must hold n_rows entries")
+        return self._call(lib().g5_validate, root, _p(pred), nglobal, want_depth)
+
+    def validate_device(self, root, d_pred, nglobal=None, want_depth=False):
+        return self._call(lib().g5_validate_device, root, d_pred.ptr, nglobal, want_depth)
+
+    def free(self):
+        if self.h:
+            lib().g5_validator_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1430,6 +1430,18 @@ extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *le
     return G5_OK;
 }
 
+extern "C" int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred)
+{
+    G5_REQUIRE(m && d_pred, G5_E_ARG, "g5_msbfs_pred_device: null argument");
+    G5_REQUIRE(lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
+               "g5_msbfs_pred_device: lane %d not in use", lane);
+    hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s, (const uint32_t *)m->rl.old2new,
+                       (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, m->n, lane, d_pred);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(m->s));
+    return G5_OK;
+}
+
 extern "C" int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge)
 {
     G5_REQUIRE(m && nedge, G5_E_ARG, "g5_msbfs_nedge: null argument");

@@ -5,7 +5,7 @@
 //
 //   g500_hip [-s scale] [-e edgefactor] [-R rows] [-C cols] [-n roots]
 //            [-x host|device] [-be codec] [-btr threshold] [-share r c]
-//            [-verify] [-dump file] [-dumpk k]
+//            [-verify] [-validate] [-dump file] [-dumpk k]
 //
 //   -x host     the reference's hook protocol (host queues, every message
 //               through the Compression schema); device: bitmaps in HBM
@@ -15,6 +15,10 @@
 //               row's fold) is timed separately (default 0 0)
 //   -verify     compare every search with the 1 x 1 search (g5_bfs_run on the
 //               undivided graph): predecessors must be identical
+//   -validate   check every search's predecessors on the device as the
+//               reference's validate_bfs_result does (main.cpp:511-533,
+//               validate/validate.h:249-491); its nedge must equal the TEPS
+//               numerator used
 //   -dump f     write the first k searches' predecessors: int64 count, then
 //               per search int64 root, int64 n, int64 pred[n]
 // Output: per-search lines, then the reference's statistics names
@@ -45,7 +49,7 @@ void check(int rc, const char *what)
 struct Opts {
     int scale = 16, edgefactor = 16, R = 1, C = 1, roots = 64, r0 = 0, c0 = 0, dumpk = 4, threshold = 64;
     std::string exchange = "device", codec, dump;
-    bool verify = false;
+    bool verify = false, validate = false;
 };
 
 Opts parse(int argc, char **argv)
@@ -69,6 +73,7 @@ Opts parse(int argc, char **argv)
             o.r0 = std::stoi(next());
             o.c0 = std::stoi(next());
         } else if (a == "-verify") o.verify = true;
+        else if (a == "-validate") o.validate = true;
         else if (a == "-dump") o.dump = next();
         else if (a == "-dumpk") o.dumpk = std::stoi(next());
         else throw std::invalid_argument("unknown option " + a);
@@ -101,6 +106,13 @@ int run(const Opts &o)
     roots.resize(found);
     g5_bfs *single = nullptr;
     if (o.verify) check(g5_bfs_create(whole, &single), "g5_bfs_create");
+    g5_validator *validator = nullptr;
+    int64_t whole_rows = 0, whole_nglobal = 0;
+    {
+        int64_t nc = 0, nnz = 0, gen = 0;
+        check(g5_graph_info(whole, &whole_rows, &nc, &nnz, &whole_nglobal, &gen), "g5_graph_info");
+    }
+    if (o.validate) check(g5_validator_create(whole, &validator), "g5_validator_create");
 
     Compression<int64_t, uint32_t> *schema = nullptr;
     bool device_codec = false;
@@ -126,8 +138,9 @@ int run(const Opts &o)
         fwrite(&k, 8, 1, dump);
     }
     std::vector<double> teps, share_expand, share_column, share_fold, totals;
-    std::vector<int64_t> pred, ref;
-    int mismatches = 0;
+    std::vector<int64_t> pred, ref, vpred;
+    std::vector<double> valid_time;
+    int mismatches = 0, invalid = 0;
     double codec_s = 0, pred_s = 0;
     int64_t raw_words = 0, coded_words = 0;
     // one untimed search first (device buffers, code objects), then the
@@ -161,6 +174,23 @@ int run(const Opts &o)
             for (int64_t v = n; v < (int64_t)pred.size() && same; ++v) same = pred[v] == -1;
             if (!same) ++mismatches;
         }
+        if (o.validate) {
+            // the grid's map may be padded past the graph's rows: those entries must be empty
+            vpred.assign(pred.begin(), pred.begin() + std::min<int64_t>(whole_rows, (int64_t)pred.size()));
+            vpred.resize(whole_rows, -1);
+            bool pad_empty = true;
+            for (size_t v = vpred.size(); v < pred.size(); ++v) pad_empty = pad_empty && pred[v] == -1;
+            int bad = 0;
+            int64_t vne = 0;
+            float vms = 0;
+            check(g5_validate(validator, roots[i], whole_nglobal, vpred.data(), &bad, &vne, nullptr, &vms),
+                  "g5_validate");
+            const bool ok = bad == 0 && pad_empty && vne == ne;
+            if (!ok) ++invalid;
+            valid_time.push_back(vms * 1e-3);
+            printf("Validation of iteration %zu finished in %fs\n", i, vms * 1e-3);
+            printf("Result: %s %lld Edge(s) processed (flags %d)\n", ok ? "Valid" : "Invalid", (long long)vne, bad);
+        }
         if (dump && (int64_t)i < o.dumpk) {
             const int64_t root = roots[i], n = (int64_t)pred.size();
             fwrite(&root, 8, 1, dump);
@@ -188,6 +218,12 @@ int run(const Opts &o)
     printf("mean_share_column_time: %.6f\n", mean_sc / k);
     printf("mean_share_fold_time: %.6f\n", mean_sf / k);
     if (o.verify) printf("verify_mismatches: %d\n", mismatches);
+    if (o.validate) {
+        double vt = 0;
+        for (double t : valid_time) vt += t;
+        printf("Validation: %s\n", invalid ? "failed!" : "passed");
+        printf("mean_validation_time: %.6f\n", valid_time.empty() ? 0.0 : vt / valid_time.size());
+    }
     double kx_ms = 0, kx_bytes = 0;
     int64_t kx_launches = 0, kx_edges = 0;
     check(g5_bfs_stats(grid.rank(o.r0, o.c0).handle(), &kx_ms, &kx_launches, &kx_bytes, &kx_edges), "g5_bfs_stats");
@@ -195,16 +231,18 @@ int run(const Opts &o)
            "\"hmean_teps\": %.6e, \"mean_time_s\": %.6f, \"share\": [%d, %d], \"share_expand_s\": %.6f, "
            "\"share_column_s\": %.6f, \"share_fold_s\": %.6f, \"codec_s\": %.6f, \"pred_s\": %.6f, "
            "\"raw_words\": %lld, \"coded_words\": %lld, \"verify_mismatches\": %d, \"share_kexpand_ms\": %.6f, "
-           "\"share_kexpand_launches\": %lld, \"share_kexpand_bytes\": %.6e, \"share_kexpand_edges\": %lld}\n",
+           "\"share_kexpand_launches\": %lld, \"share_kexpand_bytes\": %.6e, \"share_kexpand_edges\": %lld, "
+           "\"validate_failures\": %d}\n",
            o.scale, o.R, o.C, teps.size(), o.exchange.c_str(), o.codec.c_str(), hm, mean_t / k, o.r0, o.c0,
            mean_se / k, mean_sc / k, mean_sf / k, codec_s / k, pred_s / k, (long long)raw_words,
            (long long)coded_words, o.verify ? mismatches : -1, kx_ms, (long long)kx_launches, kx_bytes,
-           (long long)kx_edges);
+           (long long)kx_edges, o.validate ? invalid : -1);
     gridp.reset();
+    if (validator) g5_validator_free(validator);
     if (single) g5_bfs_free(single);
     g5_graph_free(whole);
     for (g5_graph *g : blocks) g5_graph_free(g);
-    return (o.verify && mismatches) ? 3 : 0;
+    return ((o.verify && mismatches) || (o.validate && invalid)) ? 3 : 0;
 }
 
 }  // namespace

@@ -40,6 +40,7 @@ enum {
 typedef struct g5_graph g5_graph;
 typedef struct g5_bfs g5_bfs;
 typedef struct g5_msbfs g5_msbfs;
+typedef struct g5_validator g5_validator;
 
 /* ---- library ---- */
 const char *g5_last_error(void);
@@ -181,6 +182,9 @@ int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32_t *nlevels
  * (-1 unreached, root -> root) and level int32[n] (-1 unreached); either may
  * be NULL.  Allocates device scratch; G5_E_STATE if the tree has a cycle. */
 int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level);
+/* The same pred int64[n] written to device memory d_pred (no allocation), for
+ * g5_validate_device without a host round trip. */
+int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred);
 /* TEPS numerators nedge[nroots] of the last run (validate/validate.h:439-485). */
 int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
 /* Direction per level: 0 (default) push while the frontier's edges are
@@ -248,6 +252,30 @@ int g5_vreduce_or(const uint64_t *d_maps, int k, int64_t n_words, uint64_t *d_ou
 /* d_preds: k contiguous int64[n] predecessor arrays (-1 absent) -> d_out =
  * minimum present id (the canonical rule replacing rank precedence). */
 int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out);
+
+/* ---- result validation (validate_bfs_result, validate/validate.h:249-491) ----
+ * For a whole (1x1) graph.  *bad is 0 for a valid predecessor map, else the
+ * OR of the failed checks below; a range failure stops the other checks, as
+ * in the reference.  *nedge = generated undirected edges whose source has a
+ * parent (validate.h:474-477,486), depth int32[n] (-1 unreached; may be
+ * NULL), *ms = device time (may be NULL).  The graph must outlive the
+ * validator. */
+enum {
+    G5_BAD_RANGE = 1,         /* a parent or the root out of range (validate.h:69-105,258-264) */
+    G5_BAD_ROOT = 2,          /* the root is not its own parent (:285-295) */
+    G5_BAD_SELF_PARENT = 4,   /* another vertex is its own parent (:300-342) */
+    G5_BAD_DEPTH_MAP = 8,     /* inconsistent depth map (:200-222); a set-once map never is */
+    G5_BAD_EDGE_LEVEL = 16,   /* an edge joins depths more than one apart (:425-433) */
+    G5_BAD_PARENT_EDGE = 32   /* a vertex has no edge to its parent (:436-471) */
+};
+int g5_validator_create(const g5_graph *g, g5_validator **out);
+int g5_validator_free(g5_validator *v);
+/* pred int64[n] in host memory */
+int g5_validate(g5_validator *v, int64_t root, int64_t nglobalverts, const int64_t *pred, int *bad,
+                int64_t *nedge, int32_t *depth, float *ms);
+/* pred int64[n] in device memory (g5_bfs_pred_device, g5_msbfs_pred_device) */
+int g5_validate_device(g5_validator *v, int64_t root, int64_t nglobalverts, const int64_t *d_pred, int *bad,
+                       int64_t *nedge, int32_t *depth, float *ms);
 
 #ifdef __cplusplus
 }
