@@ -18,6 +18,10 @@ roots (cpu_2d/main.cpp:944-950), in GTEPS.
     One rank's share of a 4 x 4 grid: the g500_hip driver's emulated grid
     (device exchange, codec on the fold), rank (0, 0)'s share timed.
 
+Every measured search is then validated on the device (g5_validate_device:
+the checks of validate/validate.h:249-491 with its TEPS numerator); a result
+that fails, or whose numerator differs, aborts the run.
+
 roofline.traffic: HBM bytes per launch of the dominant kernel from two
 rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run as child processes before
 this process touches the GPU, each scaled by what the counter reads for a
@@ -231,10 +235,18 @@ def bench_c4(args, g5, traffic, traffic_info):
         b.run(r)
     b.profile(True)
     single_ms, nedge = [], []
+    val = g5.Validator(g) if args.validate == "on" else None
+    d_pred = g5.DeviceArray(g.n_rows, np.int64) if val else None
+    val_ms, val_fail = [], 0
     for r in roots.tolist():
         nl, ms = b.run(r)
         single_ms.append(ms)
         nedge.append(b.nedge())
+        if val:  # outside the search's own device-time window
+            b.pred_device(d_pred)
+            bad, vne, _, vms = val.validate_device(r, d_pred)
+            val_fail += int(bad != 0 or vne != nedge[-1])
+            val_ms.append(vms)
     kst1 = b.kernel_stats()
     nedge = np.array(nedge, np.float64)
     single_teps = nedge / (np.array(single_ms) / 1e3)
@@ -291,6 +303,20 @@ def bench_c4(args, g5, traffic, traffic_info):
             dev_ms.append(ms)
     ne_b = m.nedge().astype(np.float64)
     assert np.array_equal(ne_b, nedge), "batched and single-root TEPS numerators differ"
+    validation = None
+    if val:
+        for lane, r in enumerate(roots.tolist()):
+            m.pred_device(lane, d_pred)
+            bad, vne, _, vms = val.validate_device(r, d_pred)
+            val_fail += int(bad != 0 or vne != ne_b[lane])
+            val_ms.append(vms)
+        validation = {"searches": len(val_ms), "failures": val_fail, "mean_ms": float(np.mean(val_ms)),
+                      "what": "every single-root and batched result, device validator (validate.h:249-491)"}
+        log(f"validation: {len(val_ms)} results, {val_fail} failures, {np.mean(val_ms):.1f} ms each")
+        val.free()
+        d_pred.free()
+        if val_fail:
+            raise SystemExit(f"validation failed for {val_fail} of {len(val_ms)} results")
     t_batch = float(np.mean(dev_ms)) / 1e3
     batched_teps = ne_b / (t_batch / len(roots))
     value = hmean(batched_teps) / 1e9
@@ -322,6 +348,7 @@ def bench_c4(args, g5, traffic, traffic_info):
         "single_root_kernel": single_kernel,
         "roofline": roofline,
         "autotune": tuned,
+        "validation": validation,
     }
     m.free()
     if args.cpu_baseline == "on":
@@ -440,6 +467,8 @@ def main():
     ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
                     help="c4: measure roofline.traffic with rocprofv3 --pmc child passes")
     ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
+    ap.add_argument("--validate", choices=["on", "off"], default="on",
+                    help="validate every measured search on the device (c4)")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="c4: pick the batched search's shape parameters before timing")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)

@@ -63,6 +63,21 @@ __device__ __forceinline__ void wave_flags(uint32_t f, unsigned long long *acc)
     if (f && g5_lane() == 0) atomicOr(acc + kFlags, (unsigned long long)f);
 }
 
+// the workgroup's total of x added to *dst with one atomic (every thread of
+// the 256-thread workgroup calls it; same-address atomics serialise, so one
+// per workgroup rather than per wave)
+__device__ __forceinline__ void block_sum_add(int64_t x, unsigned long long *dst)
+{
+    __shared__ int64_t s_part[4];
+    x = g5_wave_sum(x);
+    if (g5_lane() == 0) s_part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int64_t t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+        if (t) atomicAdd(dst, (unsigned long long)t);
+    }
+}
+
 // ranges, root and self-parent checks, the jump words, and sum(gdeg) over
 // vertices with a parent
 __global__ void k_val_init(const int64_t *__restrict__ pred, int64_t n, int64_t root, int64_t nglob,
@@ -71,28 +86,22 @@ __global__ void k_val_init(const int64_t *__restrict__ pred, int64_t n, int64_t 
 {
     uint32_t f = 0;
     int64_t sum = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n_pad = (n + 63) / 64 * 64;  // whole waves reach the reductions
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_pad; v += stride) {
-        if (v < n) {
-            const int64_t p = pred[v];
-            if (p < -1 || p >= nglob || p >= n) f |= G5_BAD_RANGE;
-            uint64_t w;
-            if (v == root) {
-                if (p != root) f |= G5_BAD_ROOT;
-                w = pack(0, (uint64_t)root);
-            } else {
-                if (p == v) f |= G5_BAD_SELF_PARENT;
-                w = (p < 0 || p >= n) ? kNone : pack(1, (uint64_t)p);
-            }
-            J[v] = w;
-            if (p != -1) sum += gdeg[v];
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = pred[v];
+        if (p < -1 || p >= nglob || p >= n) f |= G5_BAD_RANGE;
+        uint64_t w;
+        if (v == root) {
+            if (p != root) f |= G5_BAD_ROOT;
+            w = pack(0, (uint64_t)root);
+        } else {
+            if (p == v) f |= G5_BAD_SELF_PARENT;
+            w = (p < 0 || p >= n) ? kNone : pack(1, (uint64_t)p);
         }
-        const int64_t s = g5_wave_sum(sum);
-        if (s && g5_lane() == 0) atomicAdd(acc + kEdgeSum, (unsigned long long)s);
-        sum = 0;
+        J[v] = w;
+        if (p != -1) sum += gdeg[v];
     }
     wave_flags(f, acc);
+    block_sum_add(sum, acc + kEdgeSum);
 }
 
 // one doubling round: J'(v) = J(anc(v)) after J(v), distances added and
@@ -157,16 +166,11 @@ __device__ __forceinline__ bool far_apart(D a, D b)
 __global__ void k_val_count_pieces(const int64_t *__restrict__ rowptr, int64_t n, unsigned long long *cnt)
 {
     int64_t c = 0;
-    const int64_t n_pad = (n + 63) / 64 * 64;
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_pad; v += (int64_t)gridDim.x * blockDim.x) {
-        if (v < n) {
-            const int64_t deg = rowptr[v + 1] - rowptr[v];
-            if (deg > kLight) c += (deg + kPiece - 1) / kPiece;
-        }
-        const int64_t s = g5_wave_sum(c);
-        if (s && g5_lane() == 0) atomicAdd(cnt, (unsigned long long)s);
-        c = 0;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t deg = rowptr[v + 1] - rowptr[v];
+        if (deg > kLight) c += (deg + kPiece - 1) / kPiece;
     }
+    block_sum_add(c, cnt);
 }
 
 // thread per row: light rows are checked whole; heavy rows get a binary
@@ -177,11 +181,14 @@ __global__ void k_val_rows(const int64_t *__restrict__ rowptr, const uint32_t *_
                            uint64_t *__restrict__ pieces, int64_t piece_cap, unsigned long long *__restrict__ acc)
 {
     uint32_t f = 0;
+    // n is a multiple of 64, so every lane of a wave runs the same iterations
+    // and the wave scan below sees all 64 lanes
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         const int64_t p = pred[v];
         const bool need_parent = p != -1 && p != v;
         const D dv = depth[v];
+        int64_t np = 0;  // pieces of a heavy row
         if (e - b <= kLight) {
             bool found = false;
             for (int64_t k = b; k < e; ++k) {
@@ -200,10 +207,17 @@ __global__ void k_val_rows(const int64_t *__restrict__ rowptr, const uint32_t *_
                 }
                 if (lo == e || (int64_t)cols[lo] != p) f |= G5_BAD_PARENT_EDGE;
             }
-            const int64_t np = (e - b + kPiece - 1) / kPiece;
-            const int64_t base = (int64_t)atomicAdd(acc + kPieceCount, (unsigned long long)np);
-            for (int64_t j = 0; j < np && base + j < piece_cap; ++j) pieces[base + j] = (uint64_t)v << 32 | (uint64_t)j;
+            np = (e - b + kPiece - 1) / kPiece;
         }
+        // the wave's pieces take one contiguous range of the list
+        const int64_t incl = g5_wave_incl_scan(np);
+        const int64_t total = __shfl(incl, 63, 64);
+        int64_t base = 0;
+        if (total) {
+            if (g5_lane() == 63) base = (int64_t)atomicAdd(acc + kPieceCount, (unsigned long long)total);
+            base = __shfl(base, 63, 64) + incl - np;
+        }
+        for (int64_t j = 0; j < np && base + j < piece_cap; ++j) pieces[base + j] = (uint64_t)v << 32 | (uint64_t)j;
     }
     wave_flags(f, acc);
 }
@@ -348,6 +362,7 @@ extern "C" int g5_validator_create(const g5_graph *g, g5_validator **out)
                "g5_validator_create: needs a whole (1x1) graph");
     G5_REQUIRE(g->n_rows < (int64_t)0xFFFFFFFF, G5_E_ARG, "g5_validator_create: %lld vertices exceed 32-bit ids",
                (long long)g->n_rows);
+    G5_REQUIRE(g->n_rows % 64 == 0, G5_E_ARG, "g5_validator_create: rows must be a multiple of 64");
     G5_HIP(hipSetDevice(g->device));
     g5_validator *v = new g5_validator;
     v->g = g;
