@@ -25,7 +25,10 @@ that fails, or whose numerator differs, aborts the run.
 roofline.traffic: HBM bytes per launch of the dominant kernel from two
 rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run as child processes before
 this process touches the GPU, each scaled by what the counter reads for a
-plain copy kernel of known size; null when rocprofv3 is unavailable.
+plain copy kernel of known size; null when rocprofv3 is unavailable.  A
+first child (no profiler) autotunes the batched search's shapes; the counter
+passes and the timed run all use those shapes, so the kernels measured are
+the kernels timed.
 
     python bench.py [--config c4|c2|c3] [--steps K] [--warmup W] [--scale S]
 """
@@ -49,7 +52,7 @@ sys.path.insert(0, ROOT)
 METRIC = "GTEPS hmean, batched-64 vs single-root, scale 26, 1 MI355X; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 COPY_WORDS = 1 << 26          # copy-kernel calibration: 64 Mi uint64 words read and written
-TRAFFIC_KERNEL = "k_pull_wave"
+PULL_KERNELS = ("k_pull_block", "k_pull_wave", "k_pull_thread")
 DEFAULT_SCALE = {"c4": 26, "c2": 22, "c3": 26}
 
 
@@ -146,14 +149,30 @@ def cpu_baseline_codec(g, scale, roots, nedge_gpu, budget_s=12.0):
 
 # ---------------------------------------------------------------- PMC traffic
 
-def pmc_probe(args):
-    """Run under rocprofv3 --pmc by measure_traffic: two batched searches and
-    the copy calibration kernel."""
+def tune_probe(args):
+    """Child process (no profiler): autotune the batched search's shapes and
+    print them as one JSON line."""
     import gpugraph500_amd as g5
     g5.init(0)
     g = g5.Graph.kronecker(args.scale)
     roots = g.select_roots(1 << args.scale, args.roots)
     m = g5.MSBFS(g)
+    m.run(roots)
+    best_ms, trials, params = m.autotune(roots)
+    print(json.dumps({"params": params, "candidates_run": trials, "best_ms": best_ms}), flush=True)
+    return 0
+
+
+def pmc_probe(args):
+    """Run under rocprofv3 --pmc by measure_traffic: two batched searches with
+    the given shapes and the copy calibration kernel."""
+    import gpugraph500_amd as g5
+    g5.init(0)
+    g = g5.Graph.kronecker(args.scale)
+    roots = g.select_roots(1 << args.scale, args.roots)
+    m = g5.MSBFS(g)
+    if args.tune_json:
+        m.tune(**json.loads(args.tune_json))
     m.run(roots)
     m.run(roots)
     m.free()
@@ -167,11 +186,13 @@ def pmc_probe(args):
     return 0
 
 
-def _pmc_pass(exe, counter, args):
+def _pmc_pass(exe, counter, args, tune_json):
+    """{pull kernel: mean counter per dispatch}, the copy kernel's mean and the
+    dispatches sampled."""
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        cmd = [exe, "--pmc", counter, "--kernel-include-regex", f"{TRAFFIC_KERNEL}|k_or", "-d", d, "-o", "p",
+        cmd = [exe, "--pmc", counter, "--kernel-include-regex", "k_pull|k_or", "-d", d, "-o", "p",
                "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-probe",
-               "--scale", str(args.scale), "--roots", str(args.roots)]
+               "--scale", str(args.scale), "--roots", str(args.roots), "--tune-json", tune_json]
         env = dict(os.environ, TMPDIR="/tmp")
         r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            timeout=300)
@@ -180,48 +201,70 @@ def _pmc_pass(exe, counter, args):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if not files:
             raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file")
-        kern, copy = [], []
+        kern, copy = {k: [] for k in PULL_KERNELS}, []
         with open(files[0]) as f:
             for row in csv.DictReader(f):
                 if row["Counter_Name"] != counter:
                     continue
                 name, v = row["Kernel_Name"], float(row["Counter_Value"])
-                if TRAFFIC_KERNEL in name:
-                    kern.append(v)
-                elif "k_or" in name:
+                for k in PULL_KERNELS:
+                    if k in name:
+                        kern[k].append(v)
+                if "k_or" in name:
                     copy.append(v)
-        if not kern or not copy:
+        if not copy or not any(kern.values()):
             raise RuntimeError(f"rocprofv3 --pmc {counter}: kernels not found")
-        return float(np.mean(kern)), float(np.mean(copy)), len(kern)
+        return ({k: float(np.mean(v)) for k, v in kern.items() if v}, float(np.mean(copy)),
+                {k: len(v) for k, v in kern.items()})
+
+
+def run_tune_probe(args):
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--tune-probe", "--scale", str(args.scale),
+                        "--roots", str(args.roots)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=600,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"tune probe exited {r.returncode}: {r.stderr[-400:]!r}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def measure_traffic(args):
-    """HBM bytes per launch of TRAFFIC_KERNEL from FETCH_SIZE and WRITE_SIZE
-    (KiB per dispatch), each scaled so that the copy kernel reads and writes
-    8 * COPY_WORDS bytes."""
+    """Shapes from a tune probe, then HBM bytes per launch of every pull
+    kernel with those shapes from FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled so that the copy kernel reads and writes
+    8 * COPY_WORDS bytes.  Returns ({kernel: bytes}, info, tuned)."""
+    t0 = time.perf_counter()
+    try:
+        tuned = run_tune_probe(args) if args.autotune == "on" else None
+    except (RuntimeError, subprocess.TimeoutExpired, OSError, ValueError) as e:
+        return None, {"error": str(e)[:300]}, None
+    if tuned:
+        tuned["seconds"] = time.perf_counter() - t0
+        log(f"tune probe: {tuned['params']} after {tuned['candidates_run']} candidates, {tuned['best_ms']:.2f} ms")
     exe = shutil.which("rocprofv3")
     if not exe:
-        return None, {"error": "rocprofv3 not found"}
+        return None, {"error": "rocprofv3 not found"}, tuned
+    tune_json = json.dumps(tuned["params"]) if tuned else ""
     try:
-        t0 = time.perf_counter()
-        f_k, f_c, n_f = _pmc_pass(exe, "FETCH_SIZE", args)
-        w_k, w_c, n_w = _pmc_pass(exe, "WRITE_SIZE", args)
+        t1 = time.perf_counter()
+        f_k, f_c, n_f = _pmc_pass(exe, "FETCH_SIZE", args, tune_json)
+        w_k, w_c, n_w = _pmc_pass(exe, "WRITE_SIZE", args, tune_json)
     except (RuntimeError, subprocess.TimeoutExpired, OSError) as e:
-        return None, {"error": str(e)[:300]}
+        return None, {"error": str(e)[:300]}, tuned
     copy_bytes = 8.0 * COPY_WORDS
     f_scale = copy_bytes / (f_c * 1024.0)
     w_scale = copy_bytes / (w_c * 1024.0)
-    traffic = f_k * 1024.0 * f_scale + w_k * 1024.0 * w_scale
-    info = {"kernel": TRAFFIC_KERNEL, "fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k,
-            "fetch_scale": f_scale, "write_scale": w_scale, "launches_sampled": n_f,
-            "seconds": time.perf_counter() - t0}
-    log(f"traffic: {traffic / 1e9:.2f} GB per {TRAFFIC_KERNEL} launch (fetch x{f_scale:.2f}, write x{w_scale:.2f})")
-    return traffic, info
+    traffic = {k: f_k[k] * 1024.0 * f_scale + w_k.get(k, 0.0) * 1024.0 * w_scale for k in f_k}
+    info = {"fetch_kib_per_launch": f_k, "write_kib_per_launch": w_k, "fetch_scale": f_scale,
+            "write_scale": w_scale, "launches_sampled": n_f, "shapes": tuned["params"] if tuned else "default",
+            "seconds": time.perf_counter() - t1}
+    log("traffic per launch: " + ", ".join(f"{k} {v / 1e9:.2f} GB" for k, v in traffic.items()) +
+        f" (fetch x{f_scale:.2f}, write x{w_scale:.2f})")
+    return traffic, info, tuned
 
 
 # ---------------------------------------------------------------- configs
 
-def bench_c4(args, g5, traffic, traffic_info):
+def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     t0 = time.perf_counter()
     g = g5.Graph.kronecker(args.scale)
     log(f"graph: scale {args.scale}, {g.nglobal} vertices, {g.nnz} nonzeros, "
@@ -263,11 +306,15 @@ def bench_c4(args, g5, traffic, traffic_info):
     for _ in range(max(args.warmup, 1)):
         m.run(roots)
     tuned = None
-    if args.autotune == "on":
+    if probe_tuned:  # the shapes the counter passes measured
+        m.tune(**probe_tuned["params"])
+        tuned = dict(probe_tuned, source="tune probe (child process, before the counter passes)")
+        log(f"shapes from the tune probe: {probe_tuned['params']}")
+    elif args.autotune == "on":
         t0 = time.perf_counter()
         best_ms, trials, params = m.autotune(roots)
         tuned = {"params": params, "candidates_run": trials, "best_ms": best_ms,
-                 "seconds": time.perf_counter() - t0}
+                 "seconds": time.perf_counter() - t0, "source": "this process"}
         log(f"autotune: {params} after {trials} candidates, {best_ms:.2f} ms")
     log(f"msbfs degree bins (workgroup, wavefront, thread per vertex): {m.bins()}")
     m.profile(True)            # one run as direct launches with per-kernel events
@@ -283,10 +330,11 @@ def bench_c4(args, g5, traffic, traffic_info):
     d = kst[dom]
     dom_gbs = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic if dom == TRAFFIC_KERNEL else None,
+                "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic.get(dom) if traffic else None,
                 "kernel": dom, "avg_launch_ms": d["ms"] / max(d["launches"], 1),
                 "algorithmic_bytes_per_launch": d["bytes"] / max(d["launches"], 1),
                 "traffic_source": traffic_info,
+                "traffic_all_kernels": traffic,
                 "all_kernels": {k: {"ms": v["ms"], "launches": v["launches"],
                                     "gbs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else 0.0}
                                 for k, v in kst.items()}}
@@ -472,6 +520,8 @@ def main():
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="c4: pick the batched search's shape parameters before timing")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tune-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tune-json", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.scale is None:
         args.scale = DEFAULT_SCALE[args.config]
@@ -482,17 +532,20 @@ def main():
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
+    if args.tune_probe:
+        return tune_probe(args)
     if args.config == "c3":
         print(json.dumps(bench_c3(args)))
         return 0
-    traffic, traffic_info = None, None
+    traffic, traffic_info, probe_tuned = None, None, None
     if args.config == "c4" and args.traffic == "auto":
-        traffic, traffic_info = measure_traffic(args)   # before this process initialises the GPU
+        # child processes, before this process initialises the GPU
+        traffic, traffic_info, probe_tuned = measure_traffic(args)
 
     import gpugraph500_amd as g5
 
     g5.init(0)
-    out = bench_c4(args, g5, traffic, traffic_info) if args.config == "c4" else bench_c2(args, g5)
+    out = bench_c4(args, g5, traffic, traffic_info, probe_tuned) if args.config == "c4" else bench_c2(args, g5)
     print(json.dumps(out))
     return 0
 

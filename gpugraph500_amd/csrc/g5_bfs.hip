@@ -130,32 +130,48 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(
 }
 
 // visited |= out (minus already visited), level := depth for new bits, count;
-// with rowptr (1x1 block) also the degree sum of the new vertices.
-__global__ void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visited,
-                         int32_t *__restrict__ level, int64_t wc, int32_t depth, int64_t *__restrict__ cnt,
-                         const int64_t *__restrict__ rowptr)
+// with rowptr (1x1 block) also the degree sum of the new vertices.  One
+// wavefront per 64 consecutive words: lane j commits word w0 + j, then, word
+// by word over the words with new bits, lane i takes vertex 64 w + i, so a
+// word's level stores and degree reads are one coalesced access each rather
+// than 64 scattered ones from a single thread.
+__global__ __launch_bounds__(256) void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visited,
+                                                int32_t *__restrict__ level, int64_t wc, int32_t depth,
+                                                int64_t *__restrict__ cnt, const int64_t *__restrict__ rowptr)
 {
+    const int lane = __lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     int64_t c = 0, ce = 0;
-    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < wc;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t o = out[w];
-        if (!o) continue;
-        const uint64_t vis = visited[w];
-        const uint64_t nw = o & ~vis;
-        if (nw != o) out[w] = nw;
-        visited[w] = vis | nw;
-        c += __popcll(nw);
-        uint64_t b = nw;
-        while (b) {
-            const int i = __ffsll((long long)b) - 1;
-            level[w * 64 + i] = depth;
-            if (rowptr) ce += rowptr[w * 64 + i + 1] - rowptr[w * 64 + i];
-            b &= b - 1;
+    for (int64_t w0 = (blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; w0 < wc;
+         w0 += nwaves * 64) {
+        const int64_t w = w0 + lane;
+        uint64_t nw = 0;
+        if (w < wc) {
+            const uint64_t o = out[w];
+            if (o) {
+                const uint64_t vis = visited[w];
+                nw = o & ~vis;
+                if (nw != o) out[w] = nw;
+                if (nw) visited[w] = vis | nw;
+                c += __popcll(nw);
+            }
+        }
+        uint64_t todo = __ballot(nw != 0);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint64_t bits = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(nw >> 32), j) << 32) |
+                                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)nw, j);
+            if (bits >> lane & 1) {
+                const int64_t v = (w0 + j) * 64 + lane;
+                level[v] = depth;
+                if (rowptr) ce += rowptr[v + 1] - rowptr[v];
+            }
         }
     }
     c = g5_wave_sum(c);
     ce = g5_wave_sum(ce);
-    if ((threadIdx.x & 63) == 0 && c) {
+    if (lane == 0 && c) {
         atomicAdd((unsigned long long *)&cnt[kNew], (unsigned long long)c);
         if (ce) atomicAdd((unsigned long long *)&cnt[kNewEdges], (unsigned long long)ce);
     }
