@@ -131,10 +131,20 @@ __global__ __launch_bounds__(kExpandBlock) void k_expand(
 
 // visited |= out (minus already visited), level := depth for new bits, count;
 // with rowptr (1x1 block) also the degree sum of the new vertices.  One
-// wavefront per 64 consecutive words: lane j commits word w0 + j, then, word
-// by word over the words with new bits, lane i takes vertex 64 w + i, so a
-// word's level stores and degree reads are one coalesced access each rather
-// than 64 scattered ones from a single thread.
+// wavefront per 64 consecutive words: lane j commits word w0 + j; then the
+// wave walks the words with new bits 8 at a time, lane i taking vertex
+// 64 w + i, so a word's level stores and offset reads are coalesced, and the
+// 8 words' offset loads are all issued before any is used (a vertex's degree
+// is the next lane's offset minus its own; lane 63 loads the next word's
+// first offset).
+constexpr int kCommitGroup = 8;
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t x, int l)
+{
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+}
+
 __global__ __launch_bounds__(256) void k_commit(uint64_t *__restrict__ out, uint64_t *__restrict__ visited,
                                                 int32_t *__restrict__ level, int64_t wc, int32_t depth,
                                                 int64_t *__restrict__ cnt, const int64_t *__restrict__ rowptr)
@@ -156,16 +166,25 @@ __global__ __launch_bounds__(256) void k_commit(uint64_t *__restrict__ out, uint
                 c += __popcll(nw);
             }
         }
-        uint64_t todo = __ballot(nw != 0);
-        while (todo) {
-            const int j = __ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            const uint64_t bits = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(nw >> 32), j) << 32) |
-                                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)nw, j);
-            if (bits >> lane & 1) {
-                const int64_t v = (w0 + j) * 64 + lane;
-                level[v] = depth;
-                if (rowptr) ce += rowptr[v + 1] - rowptr[v];
+        const uint64_t todo = __ballot(nw != 0);
+        for (int j0 = 0; j0 < 64; j0 += kCommitGroup) {
+            if (((todo >> j0) & ((1ull << kCommitGroup) - 1)) == 0) continue;
+            uint64_t bits[kCommitGroup];
+            int64_t lo[kCommitGroup], hi[kCommitGroup];
+#pragma unroll
+            for (int k = 0; k < kCommitGroup; ++k) {
+                bits[k] = bcast64(nw, j0 + k);  // zero for words past wc
+                const int64_t v = (w0 + j0 + k) * 64 + lane;
+                const bool any = bits[k] != 0;  // wave-uniform
+                lo[k] = (rowptr && any) ? rowptr[v] : 0;
+                hi[k] = (rowptr && any && lane == 63) ? rowptr[v + 1] : 0;
+                if (bits[k] >> lane & 1) level[v] = depth;
+            }
+#pragma unroll
+            for (int k = 0; k < kCommitGroup; ++k) {
+                const int64_t nxt = __shfl_down(lo[k], 1, 64);
+                const int64_t up = lane == 63 ? hi[k] : nxt;
+                if (bits[k] >> lane & 1) ce += up - lo[k];
             }
         }
     }
