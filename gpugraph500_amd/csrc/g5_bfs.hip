@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void k_commit(uint64_t *__restrict__ out, uint
             }
         }
     }
-    c = g5_wave_sum(c);
-    ce = g5_wave_sum(ce);
-    if (lane == 0 && c) {
+    c = g5_block_sum256(c);
+    ce = g5_block_sum256(ce);
+    if (threadIdx.x == 0 && c) {
         atomicAdd((unsigned long long *)&cnt[kNew], (unsigned long long)c);
         if (ce) atomicAdd((unsigned long long *)&cnt[kNewEdges], (unsigned long long)ce);
     }
@@ -209,9 +209,8 @@ __global__ __launch_bounds__(256) void k_bottom_up(const int64_t *__restrict__ r
                                                    const uint64_t *__restrict__ front, uint64_t *__restrict__ next,
                                                    uint32_t *__restrict__ parent, int64_t wc, int64_t *__restrict__ cnt)
 {
-    __shared__ int64_t s_e[4], s_v[4];
     const int lane = __lane_id();
-    int64_t nedges = 0, nverts = 0;
+    int64_t nedges = 0, nverts = 0, newdeg = 0;
     for (int64_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < wc; w += (int64_t)gridDim.x * 4) {
         const uint64_t vis = visited[w];
         bool hit = false;
@@ -236,21 +235,18 @@ __global__ __launch_bounds__(256) void k_bottom_up(const int64_t *__restrict__ r
             }
             nedges += min(j, e) - b;
             ++nverts;
+            if (hit) newdeg += e - b;
         }
         const uint64_t bits = __ballot(hit);
         if (lane == 0) next[w] = bits;
     }
-    nedges = g5_wave_sum(nedges);
-    nverts = g5_wave_sum(nverts);
-    if (lane == 0) {
-        s_e[threadIdx.x >> 6] = nedges;
-        s_v[threadIdx.x >> 6] = nverts;
-    }
-    __syncthreads();
+    nedges = g5_block_sum256(nedges);
+    nverts = g5_block_sum256(nverts);
+    newdeg = g5_block_sum256(newdeg);
     if (threadIdx.x == 0) {
-        const int64_t te = s_e[0] + s_e[1] + s_e[2] + s_e[3], tv = s_v[0] + s_v[1] + s_v[2] + s_v[3];
-        if (te) atomicAdd((unsigned long long *)&cnt[kBuEdges], (unsigned long long)te);
-        if (tv) atomicAdd((unsigned long long *)&cnt[kBuVerts], (unsigned long long)tv);
+        if (nedges) atomicAdd((unsigned long long *)&cnt[kBuEdges], (unsigned long long)nedges);
+        if (nverts) atomicAdd((unsigned long long *)&cnt[kBuVerts], (unsigned long long)nverts);
+        if (newdeg) atomicAdd((unsigned long long *)&cnt[kNewEdges], (unsigned long long)newdeg);
     }
 }
 
@@ -260,8 +256,8 @@ __global__ void k_count_bits(const uint64_t *__restrict__ bits, int64_t n, int64
     for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n;
          w += (int64_t)gridDim.x * blockDim.x)
         c += __popcll(bits[w]);
-    c = g5_wave_sum(c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
+    c = g5_block_sum256(c);
+    if (threadIdx.x == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
 }
 
 __global__ void k_start(int64_t root_row, int64_t root_col, uint32_t *queue, uint64_t *visited, uint64_t *out,
@@ -294,8 +290,8 @@ __global__ void k_nedge(const uint32_t *__restrict__ gdeg, const int32_t *__rest
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         if (level[i] >= 0) c += gdeg[i];
-    c = g5_wave_sum(c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
+    c = g5_block_sum256(c);
+    if (threadIdx.x == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
 }
 
 __global__ void k_set_cnt(int64_t *cnt, int idx, int64_t v) { cnt[idx] = v; }
@@ -505,7 +501,7 @@ extern "C" int g5_bfs_set_outqueue(g5_bfs *b, const uint32_t *q, int64_t n)
         }
     }
     hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
-    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
+    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 2048), 256, 0, b->s, b->d_out, b->d_visited,
                        b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt,
                        (const int64_t *)nullptr);
     G5_HIP(hipGetLastError());
@@ -575,7 +571,7 @@ extern "C" int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits)
     if (d_bits && d_bits != b->d_out && b->wc)
         G5_HIP(hipMemcpyAsync(b->d_out, d_bits, (size_t)b->wc * 8, hipMemcpyDeviceToDevice, b->s));
     hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNew, (int64_t)0);
-    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, b->s, b->d_out, b->d_visited,
+    hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 2048), 256, 0, b->s, b->d_out, b->d_visited,
                        b->d_level, b->wc, (int32_t)(b->depth + 1), b->d_cnt,
                        (const int64_t *)nullptr);
     G5_HIP(hipGetLastError());
@@ -650,7 +646,7 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
         uint64_t *fresh = b->d_out;
         if (bu) {
             if (b->prof) G5_HIP(hipEventRecord(b->ek0, s));
-            hipLaunchKernelGGL(k_bottom_up, g5::grid_for(b->wc, 4, 16384), 256, 0, s, (const int64_t *)g->d_rowptr,
+            hipLaunchKernelGGL(k_bottom_up, g5::grid_for(b->wc, 4, 4096), 256, 0, s, (const int64_t *)g->d_rowptr,
                                (const uint32_t *)g->d_cols, (const uint64_t *)b->d_visited,
                                (const uint64_t *)b->d_out, b->d_next, b->d_parent, b->wc, b->d_cnt);
             G5_HIP(hipGetLastError());
@@ -660,18 +656,31 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
             rc = expand_level(b);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 4096), 256, 0, s, fresh, b->d_visited, b->d_level,
-                           b->wc, (int32_t)(b->depth + 1), b->d_cnt, (const int64_t *)g->d_rowptr);
+        // a bottom-up level already summed its new vertices' degrees
+        hipLaunchKernelGGL(k_commit, g5::grid_for(b->wc, 256, 2048), 256, 0, s, fresh, b->d_visited, b->d_level,
+                           b->wc, (int32_t)(b->depth + 1), b->d_cnt,
+                           bu ? (const int64_t *)nullptr : (const int64_t *)g->d_rowptr);
         G5_HIP(hipGetLastError());
         if (bu) std::swap(b->d_out, b->d_next);
-        G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, s));
         rc = sync_counters(b);
         if (rc) return rc;
         const int64_t n_new = b->h_cnt[kNew];
         rc = bu ? account_bottom_up(b, n_new) : account_level(b, b->qlen, n_new);
         if (rc) return rc;
         if (n_new == 0) break;
-        b->qlen = b->h_cnt[kQLen];
+        // the next level's direction
+        const int64_t m_f = b->h_cnt[kNewEdges];
+        m_u -= m_f;
+        bool bu_next = bu;
+        if (!topdown_only) {
+            if (!bu && m_f > m_u / kAlpha) bu_next = true;
+            else if (bu && n_new < n / kBeta) bu_next = false;
+        }
+        // the out-bitmap holds exactly the new vertices; a queue of them is
+        // needed by a top-down level and by the codec
+        b->qlen = n_new;
+        if (!bu_next || codec)
+            G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, s));
         if (codec) {
             size_t nw = 0, nd = 0;
             rc = b->codec.encode(b->d_queue, b->qlen, b->d_words, b->words_cap, &nw, s);
@@ -680,13 +689,7 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
             if (rc) return rc;
             G5_REQUIRE((int64_t)nd == b->qlen, G5_E_STREAM, "g5_bfs_run: codec round trip lost values");
         }
-        // the next level's direction
-        const int64_t m_f = b->h_cnt[kNewEdges];
-        m_u -= m_f;
-        if (!topdown_only) {
-            if (!bu && m_f > m_u / kAlpha) bu = true;
-            else if (bu && n_new < n / kBeta) bu = false;
-        }
+        bu = bu_next;
         b->depth++;
     }
     G5_HIP(hipEventRecord(b->ev1, s));

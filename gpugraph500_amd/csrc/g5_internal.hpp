@@ -113,3 +113,18 @@ __device__ __forceinline__ int64_t g5_wave_sum(int64_t v)
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
 }
+
+// Sum of v over a 256-thread workgroup, returned to every thread (every
+// thread must call it).  Kernels add it to a global counter from one thread:
+// same-address atomics serialise in one L2 channel, so one per workgroup
+// rather than one per wave.
+__device__ __forceinline__ int64_t g5_block_sum256(int64_t v)
+{
+    __shared__ int64_t s_part[4];
+    v = g5_wave_sum(v);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const int64_t t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    __syncthreads();  // s_part is free again for the next call
+    return t;
+}
