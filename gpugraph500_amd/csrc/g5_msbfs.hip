@@ -724,8 +724,8 @@ __global__ void k_bin_counts(const int64_t *__restrict__ rowptr, int64_t n, int6
         else if (d > 0) ++c[2];
     }
     for (int k = 0; k < kBins; ++k) {
-        const unsigned long long t = (unsigned long long)g5_wave_sum((int64_t)c[k]);
-        if (__lane_id() == 0 && t) atomicAdd(&bins[k], t);
+        const unsigned long long t = (unsigned long long)g5_block_sum256((int64_t)c[k]);
+        if (threadIdx.x == 0 && t) atomicAdd(&bins[k], t);
     }
 }
 
@@ -823,7 +823,14 @@ __global__ __launch_bounds__(256) void k_ms_nedge(const uint64_t *__restrict__ v
         const uint32_t d = gdeg[v];
         if (d && (visited[v] >> lane & 1)) acc += d;
     }
-    atomicAdd(&out[lane], acc);
+    // the workgroup's 4 waves fold in LDS, then one atomic per lane slot
+    __shared__ unsigned long long s_acc[4][64];
+    s_acc[threadIdx.x >> 6][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const unsigned long long t = s_acc[0][lane] + s_acc[1][lane] + s_acc[2][lane] + s_acc[3][lane];
+        if (t) atomicAdd(&out[lane], t);
+    }
 }
 
 // ---- continuous mode: lanes join and leave between levels ----
