@@ -888,8 +888,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x)
 // Lanes in `mask` leave: per lane the sum of generated degrees over its
 // reached vertices (2 x nedge) and the checksum sum over reached v of
 // mix64(orig(v) << 32 | (pred(v) + 1)) mod 2^64, in original ids; then the
-// lanes are freed (visited set everywhere).  One wave per vertex, lane = root,
-// so each vertex's 256 B parent record is read once, coalesced.
+// lanes are freed (visited set everywhere).  A wave takes 64 consecutive
+// vertices (lane = vertex: visited words, ids and degrees in coalesced
+// loads), then reads the parent records of the reached ones 8 vertices at a
+// time with lane = root, all 8 loads issued before any is used.
+constexpr int kLeaveGroup = 8;
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t x, int l)
+{
+    return ((uint64_t)lane_read((uint32_t)(x >> 32), l) << 32) | lane_read((uint32_t)x, l);
+}
+
 __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, const int32_t *__restrict__ parent,
                                                const uint32_t *__restrict__ new2old, const uint32_t *__restrict__ gdeg,
                                                int64_t n, uint64_t mask, unsigned long long *__restrict__ out)
@@ -901,18 +910,40 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
     }
     __syncthreads();
     const int lane = __lane_id();
-    unsigned long long ck = 0, dg = 0;
-    const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t v = blockIdx.x * 4 + (threadIdx.x >> 6); v < n; v += waves) {
-        const uint64_t vis = visited[v];
-        const uint64_t reach = vis & mask;
-        if (reach >> lane & 1) {
-            const uint32_t o = new2old[v];
-            const int32_t p = parent[v * 64 + lane];
-            ck += mix64(((uint64_t)o << 32) | (uint32_t)(p + 1));
-            dg += gdeg[v];
+    unsigned long long ck = 0, dg = 0;  // root `lane`'s sums
+    for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int64_t v = base + lane;
+        uint64_t reach = 0;
+        uint32_t o = 0, d = 0;
+        if (v < n) {
+            const uint64_t vis = visited[v];
+            reach = vis & mask;
+            if (reach) {
+                o = new2old[v];
+                d = gdeg[v];
+            }
+            if (~vis & mask) visited[v] = vis | mask;
         }
-        if (lane == 0 && (~vis & mask)) visited[v] = vis | mask;
+        const uint64_t todo = __ballot(reach != 0);
+        for (int j0 = 0; j0 < 64; j0 += kLeaveGroup) {
+            if (((todo >> j0) & ((1ull << kLeaveGroup) - 1)) == 0) continue;
+            uint64_t rj[kLeaveGroup];
+            int32_t pj[kLeaveGroup];
+#pragma unroll
+            for (int k = 0; k < kLeaveGroup; ++k) {
+                rj[k] = bcast64(reach, j0 + k);
+                pj[k] = (rj[k] >> lane & 1) ? parent[(base + j0 + k) * 64 + lane] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kLeaveGroup; ++k) {
+                const uint32_t oj = lane_read(o, j0 + k), dj = lane_read(d, j0 + k);
+                if (rj[k] >> lane & 1) {
+                    ck += mix64(((uint64_t)oj << 32) | (uint32_t)(pj[k] + 1));
+                    dg += dj;
+                }
+            }
+        }
     }
     if (mask >> lane & 1) {
         atomicAdd(&s_ck[lane], ck);
@@ -1365,7 +1396,7 @@ extern "C" int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint6
     if (!lanes) return G5_OK;
     hipStream_t s = m->s;
     G5_HIP(g5::fill_u64((uint64_t *)m->d_sums, 128, 0, s));
-    hipLaunchKernelGGL(k_leave, g5::grid_for((m->n + 3) / 4, 1, 8192), 256, 0, s, m->d_visited,
+    hipLaunchKernelGGL(k_leave, g5::grid_for(m->n, kBlock, 4096), kBlock, 0, s, m->d_visited,
                        (const int32_t *)m->d_parent, (const uint32_t *)m->rl.new2old, (const uint32_t *)m->rl.gdeg,
                        m->n, lanes, m->d_sums);
     G5_HIP(hipGetLastError());
