@@ -3,11 +3,14 @@
 // queries in real time.
 //
 //   g500_serve [-g 18,20,22,24,25,26] [-e 16] [-rate q/s] [-nq queries] [-seed s]
-//              [-maxroots k] [-lanes l] [-alone] [-compare] [-dump file]
+//              [-maxroots k] [-lanes l] [-window w] [-alone] [-compare] [-dump file]
 //
 //   -rate      mean arrival rate; inter-arrival times are exponential
 //   -maxroots  roots per query drawn uniformly from 1..k (default 8)
 //   -lanes     lanes (memory slots) per graph engine, at most 64
+//   -window    new roots join a running cohort only while all its lanes are
+//              within their first w levels (default 1; -1: whenever lanes
+//              are free)
 //   -alone     every query by itself on an idle engine, one after another
 //   -compare   after the served run, run every query alone and compare each
 //              root's (nedge, checksum): results must be identical
@@ -34,7 +37,7 @@ namespace {
 
 struct Opts {
     std::vector<int> scales{18, 20, 22, 24, 25, 26};
-    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64;
+    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64, window = 1;
     double rate = 1000;
     uint64_t seed = 1;
     bool alone = false, compare = false;
@@ -61,6 +64,7 @@ Opts parse(int argc, char **argv)
         else if (a == "-seed") o.seed = std::stoull(next());
         else if (a == "-maxroots") o.maxroots = std::stoi(next());
         else if (a == "-lanes") o.lanes = std::stoi(next());
+        else if (a == "-window") o.window = std::stoi(next());
         else if (a == "-alone") o.alone = true;
         else if (a == "-compare") o.compare = true;
         else if (a == "-dump") o.dump = next();
@@ -90,7 +94,7 @@ int run(const Opts &o)
 {
     check(g5_init(0), "g5_init");
     std::vector<std::unique_ptr<Engine>> eng;
-    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes));
+    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window));
 
     // the query stream: seeded, independent of the serving run
     std::mt19937_64 rng(o.seed);
@@ -175,11 +179,11 @@ int run(const Opts &o)
     }
     std::string scales;
     for (size_t i = 0; i < o.scales.size(); ++i) scales += (i ? "," : "") + std::to_string(o.scales[i]);
-    printf("{\"mode\": \"%s\", \"graphs\": [%s], \"queries\": %d, \"rate_qps\": %.1f, \"lanes\": %d, "
+    printf("{\"mode\": \"%s\", \"graphs\": [%s], \"queries\": %d, \"rate_qps\": %.1f, \"lanes\": %d, \"window\": %d, "
            "\"wall_s\": %.4f, \"achieved_qps\": %.1f, \"gteps\": %.4f, \"latency_ms\": {\"p50\": %.3f, "
            "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"device_ms\": %.3f, "
            "\"mismatches\": %d, \"engines\": {%s}}\n",
-           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, wall, o.nq / wall,
+           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, wall, o.nq / wall,
            nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100), (long long)levels, dev_ms,
            mismatches, per.c_str());
     return mismatches > 0 ? 3 : 0;

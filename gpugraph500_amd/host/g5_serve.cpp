@@ -25,7 +25,7 @@ double now_seconds()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-Engine::Engine(int scale, int edgefactor, int lanes) : scale_(scale), max_lanes_(lanes)
+Engine::Engine(int scale, int edgefactor, int lanes, int window) : scale_(scale), max_lanes_(lanes), window_(window)
 {
     if (lanes < 1 || lanes > 64) throw std::invalid_argument("Engine: lanes must be in [1, 64]");
     check(g5_graph_kronecker(scale, edgefactor, 1, 2, &g_), "g5_graph_kronecker");
@@ -75,8 +75,17 @@ void Engine::finish()
     if (!error_.empty()) throw std::runtime_error("engine (scale " + std::to_string(scale_) + "): " + error_);
 }
 
+bool Engine::in_window() const
+{
+    if (window_ < 0) return true;
+    for (int lane = 0; lane < 64; ++lane)
+        if ((busy_ >> lane & 1) && level_ - slots_[lane].joined > window_) return false;
+    return true;
+}
+
 int Engine::admit(std::deque<Query *> &from)
 {
+    if (!in_window()) return 0;  // wait for the running cohort
     std::vector<int64_t> roots;
     std::vector<int> lanes;
     const uint64_t allowed = max_lanes_ == 64 ? ~0ull : ((1ull << max_lanes_) - 1);
@@ -95,6 +104,7 @@ int Engine::admit(std::deque<Query *> &from)
             busy_ |= 1ull << lane;
             slots_[lane].q = q;
             slots_[lane].idx = i;
+            slots_[lane].joined = level_;
             roots.push_back(q->roots[i]);
             lanes.push_back(lane);
         }
@@ -113,6 +123,7 @@ bool Engine::advance()
     float ms = 0;
     check(g5_msbfs_step(m_, &live, &ms), "g5_msbfs_step");
     stats_.levels += 1;
+    level_ += 1;
     stats_.device_ms += ms;
     const uint64_t finished = busy_ & ~live;
     if (finished) {

@@ -8,9 +8,14 @@
 // int32[n] parent column plus one bit per vertex in the visited and frontier
 // masks), and a query is admitted, FIFO, as soon as enough lanes are free for
 // all of its roots -- between two levels of the searches already running,
-// not when the batch ends.  A root's lane leaves as soon as its own search
-// is complete; its result is (nedge, checksum of the predecessor map), and
-// is identical to the root searched alone.
+// not when the batch ends -- provided every running lane is still within the
+// join window (its first `window` levels).  Lanes that join later than that
+// run out of step with the others: every level then holds some lane in its
+// dense middle levels, which defeats the pull kernels' early exit for all of
+// them, so outside the window new roots wait for the running cohort to
+// drain (window < 0: join whenever lanes are free).  A root's lane leaves as
+// soon as its own search is complete; its result is (nedge, checksum of the
+// predecessor map), and is identical to the root searched alone.
 #pragma once
 
 #include <condition_variable>
@@ -47,7 +52,7 @@ class Engine
 {
 public:
     // owns the graph (built on the device from the Kronecker generator)
-    Engine(int scale, int edgefactor, int lanes);
+    Engine(int scale, int edgefactor, int lanes, int window = 1);
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -68,6 +73,8 @@ public:
 private:
     int scale_;
     int max_lanes_;
+    int window_;
+    int64_t level_ = 0;  // levels run by this engine
     int64_t n_ = 0;
     g5_graph *g_ = nullptr;
     g5_msbfs *m_ = nullptr;
@@ -80,6 +87,7 @@ private:
     struct Slot {
         Query *q = nullptr;
         int idx = 0;
+        int64_t joined = 0;  // level_ when the lane joined
     };
     Slot slots_[64];
     uint64_t busy_ = 0;
@@ -89,8 +97,10 @@ private:
 
     void loop();
     void loop_body();
-    // admit from `from` while lanes are free; returns the number of roots joined
+    // admit from `from` while lanes are free and the running lanes are within
+    // the join window; returns the number of roots joined
     int admit(std::deque<Query *> &from);
+    bool in_window() const;
     // one level plus the departures it causes; returns true while lanes are in use
     bool advance();
 };

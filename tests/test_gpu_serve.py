@@ -66,3 +66,12 @@ def test_served_results_match_oracle(tmp_path):
 def test_alone_mode(tmp_path):
     rec, rows = _serve(tmp_path, "-g", "10", "-nq", "20", "-alone")
     assert rec["mode"] == "alone" and len(rows) == 20
+
+
+@pytest.mark.parametrize("window", ["-1", "0", "3"])
+def test_join_window_does_not_change_results(tmp_path, window):
+    """Joining whenever lanes are free (-1), only into an idle engine (0),
+    or within a cohort's first 3 levels: the same results."""
+    rec, rows = _serve(tmp_path, "-g", "11,13", "-rate", "50000", "-nq", "150", "-seed", "9", "-window", window,
+                       "-compare")
+    assert rec["mismatches"] == 0 and rec["window"] == int(window)
