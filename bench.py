@@ -19,8 +19,9 @@ roots (cpu_2d/main.cpp:944-950), in GTEPS.
     (device exchange, codec on the fold), rank (0, 0)'s share timed.
 
 Every measured search is then validated on the device (g5_validate_device:
-the checks of validate/validate.h:249-491 with its TEPS numerator); a result
-that fails, or whose numerator differs, aborts the run.
+the checks of validate/validate.h:249-491 with its TEPS numerator), and four
+batched lanes' predecessor maps are compared bit for bit with the single-root
+search of the same root; a failure aborts the run.
 
 roofline.traffic: HBM bytes per launch of the dominant kernel from two
 rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run as child processes before
@@ -361,10 +362,27 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         validation = {"searches": len(val_ms), "failures": val_fail, "mean_ms": float(np.mean(val_ms)),
                       "what": "every single-root and batched result, device validator (validate.h:249-491)"}
         log(f"validation: {len(val_ms)} results, {val_fail} failures, {np.mean(val_ms):.1f} ms each")
+        # bit-exact agreement of batched lanes with the single-root search
+        # (predecessor maps, minimum-id parents) on four lanes
+        b2 = g5.BFS(g)
+        d_single = g5.DeviceArray(g.n_rows, np.int64)
+        same_lanes = []
+        for lane in sorted({0, len(roots) // 3, (2 * len(roots)) // 3, len(roots) - 1}):
+            b2.run(int(roots[lane]))
+            b2.pred_device(d_single)
+            m.pred_device(lane, d_pred)
+            same_lanes.append((lane, bool(np.array_equal(d_single.to_host(), d_pred.to_host()))))
+        b2.free()
+        d_single.free()
+        validation["batched_equals_single_root"] = {str(k): v for k, v in same_lanes}
+        log(f"batched == single-root predecessor maps on lanes {[k for k, _ in same_lanes]}: "
+            f"{all(v for _, v in same_lanes)}")
         val.free()
         d_pred.free()
         if val_fail:
             raise SystemExit(f"validation failed for {val_fail} of {len(val_ms)} results")
+        if not all(v for _, v in same_lanes):
+            raise SystemExit("batched and single-root predecessor maps differ")
     t_batch = float(np.mean(dev_ms)) / 1e3
     batched_teps = ne_b / (t_batch / len(roots))
     value = hmean(batched_teps) / 1e9

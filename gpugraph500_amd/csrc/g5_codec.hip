@@ -19,7 +19,7 @@
 // (each lane assembles whole output words from the deltas staged in LDS, so
 // stores are coalesced and need no atomics); one wave writes the varint tail.
 // Decode: the group headers' positions depend on all earlier widths.  Short
-// streams (<= 64 groups) walk them in one workgroup through LDS-staged
+// streams (<= 4096 words) walk them in one workgroup through LDS-staged
 // chunks; longer ones find them in parallel: every word index 4 + 4i is a
 // candidate group start (groups are 4 + 4 * sum(widths) words), its next
 // start follows from its own 16 widths, 5 rounds of pointer doubling give
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in
 
 // ------------------------------------------------------------------ decode
 
-enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 4 };
+enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 4, kMetaAnchors = 5 };
 
 constexpr int kWalkChunk = 8192;
 
@@ -331,8 +331,10 @@ __global__ __launch_bounds__(256) void k_carry_apply(uint32_t *__restrict__ c, i
     if (i < n) store_carry(c, i, ex);
 }
 
-// ---- parallel group-header search (streams of more than kSerialGroups groups) ----
-constexpr int64_t kSerialGroups = 64;
+// ---- parallel group-header search (streams of more than kSerialWords words) ----
+// The choice needs only the word count, so the host reads nothing of the
+// stream before launching; the header's checks run on the device.
+constexpr int64_t kSerialWords = 4096;
 constexpr int kJumpLog = 5;  // anchors every 2^5 groups
 constexpr int32_t kBadWidth = -1, kTruncated = -2;
 
@@ -372,18 +374,31 @@ __global__ void k_dec_jump(const int32_t *__restrict__ in, int64_t ncand, int32_
     }
 }
 
-__global__ void k_dec_set_len(int64_t *meta, int64_t L, int64_t ngroups)
-{
-    meta[kMetaL] = L;
-    meta[kMetaGroups] = ngroups;
-}
-
-// one thread: anchor[k] = start index of group 32k, k = 0..(ngroups-1)/32; the
+// one thread: the stream header's checks (as the serial walk makes them),
+// then anchor[k] = start index of group 32k, k = 0..(ngroups-1)/32; the
 // jumps between anchors never cross the last group, whose unused widths are
-// not read
-__global__ void k_dec_anchors(const int32_t *__restrict__ jmp, int64_t nanch, int32_t *__restrict__ anchor,
+// not read.  meta gets L, the group and anchor counts.
+__global__ void k_dec_anchors(const uint32_t *__restrict__ in, int64_t cap, int64_t ncand,
+                              const int32_t *__restrict__ jmp, int32_t *__restrict__ anchor,
                               int64_t *__restrict__ meta)
 {
+    const int64_t L = in[0], ngroups = (L / 128 + 15) / 16;
+    meta[kMetaL] = L;
+    meta[kMetaGroups] = ngroups;
+    meta[kMetaAnchors] = 0;
+    int64_t err = 0;
+    if (in[1] != kCookie || in[2] != kCookie || in[3] != kCookie) err = 1;
+    if (L % 128 != 0 || L > cap) err = 2;
+    if (!err && ngroups > ncand) err = 4;  // more groups than possible starts: truncated
+    if (err) {
+        meta[kMetaErr] = err;
+        return;
+    }
+    if (ngroups == 0) {
+        meta[kMetaTailPos] = 4;
+        return;
+    }
+    const int64_t nanch = ((ngroups - 1) >> kJumpLog) + 1;
     int32_t a = 0;
     for (int64_t k = 0; k < nanch; ++k) {
         anchor[k] = a;
@@ -394,16 +409,18 @@ __global__ void k_dec_anchors(const int32_t *__restrict__ jmp, int64_t nanch, in
             return;
         }
     }
+    meta[kMetaAnchors] = nanch;
 }
 
 // thread k: group starts 32k .. 32k+31; the last group's own widths (only
 // the miniblocks it holds) give the tail position
-__global__ void k_dec_fill(const uint32_t *__restrict__ in, int64_t nwords, int64_t nblk,
-                           const int32_t *__restrict__ nxt, const int32_t *__restrict__ anchor, int64_t nanch,
-                           int64_t ngroups, int64_t ncand, int64_t *__restrict__ grp, int64_t *__restrict__ meta)
+__global__ void k_dec_fill(const uint32_t *__restrict__ in, int64_t nwords, const int32_t *__restrict__ nxt,
+                           const int32_t *__restrict__ anchor, int64_t ncand, int64_t *__restrict__ grp,
+                           int64_t *__restrict__ meta)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= nanch || meta[kMetaErr]) return;
+    if (k >= meta[kMetaAnchors] || meta[kMetaErr]) return;
+    const int64_t ngroups = meta[kMetaGroups], nblk = meta[kMetaL] / 128;
     int32_t a = anchor[k];
     for (int j = 0; j < (1 << kJumpLog); ++j) {
         const int64_t g = (k << kJumpLog) + j;
@@ -607,19 +624,13 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
     int rc = reserve((int64_t)cap_values);
     if (rc) return rc;
     G5_HIP(fill_i64(d_meta, 16, 0, s));
-    // the stream header decides between the serial and the parallel header search
-    uint32_t hdr[4];
-    G5_HIP(hipMemcpyAsync(hdr, d_in, 16, hipMemcpyDeviceToHost, s));
-    G5_HIP(hipStreamSynchronize(s));
-    const int64_t L0 = hdr[0], ngroups0 = (L0 / 128 + 15) / 16;
-    const bool header_ok = hdr[1] == kCookie && hdr[2] == kCookie && hdr[3] == kCookie && L0 % 128 == 0 &&
-                           L0 <= (int64_t)cap_values;
     const int64_t ncand = ((int64_t)nwords - 4) / 4 + 1;
-    if (!header_ok || ngroups0 <= kSerialGroups || ncand > ncand_cap) {
+    if ((int64_t)nwords <= kSerialWords || ncand > ncand_cap) {
         hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, (int64_t)nwords, (int64_t)cap_values, d_grp, d_meta);
         G5_HIP(hipGetLastError());
     } else {
-        const int64_t nanch = ((ngroups0 - 1) >> kJumpLog) + 1;
+        // a stream holds at most ncand groups (k_dec_anchors rejects more)
+        const int64_t nanch_max = ((ncand - 1) >> kJumpLog) + 1;
         const unsigned grid = (unsigned)grid_for(ncand, 256, 16384);
         hipLaunchKernelGGL(k_dec_next, grid, 256, 0, s, d_in, (int64_t)nwords, ncand, d_nxt);
         const int32_t *src = d_nxt;
@@ -628,11 +639,9 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
             hipLaunchKernelGGL(k_dec_jump, grid, 256, 0, s, src, ncand, bufs[r & 1]);
             src = bufs[r & 1];
         }
-        hipLaunchKernelGGL(k_dec_anchors, 1, 1, 0, s, src, nanch, d_anchor, d_meta);
-        hipLaunchKernelGGL(k_dec_fill, (unsigned)((nanch + 255) / 256), 256, 0, s, d_in, (int64_t)nwords,
-                           L0 / 128, (const int32_t *)d_nxt,
-                           (const int32_t *)d_anchor, nanch, ngroups0, ncand, d_grp, d_meta);
-        hipLaunchKernelGGL(k_dec_set_len, 1, 1, 0, s, d_meta, L0, ngroups0);
+        hipLaunchKernelGGL(k_dec_anchors, 1, 1, 0, s, d_in, (int64_t)cap_values, ncand, src, d_anchor, d_meta);
+        hipLaunchKernelGGL(k_dec_fill, (unsigned)((nanch_max + 255) / 256), 256, 0, s, d_in, (int64_t)nwords,
+                           (const int32_t *)d_nxt, (const int32_t *)d_anchor, ncand, d_grp, d_meta);
         G5_HIP(hipGetLastError());
     }
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));

@@ -90,6 +90,17 @@ def test_malformed_long_streams_rejected():
             g5.s4_decode(w[:cut], n)
     with pytest.raises(g5.G5Error):
         g5.s4_decode(w, n - 1)
+    with pytest.raises(g5.G5Error):
+        g5.s4_decode(w, 1000)  # the header's L exceeds the capacity
+    bad = w.copy()
+    bad[2] = 7  # cookie
+    with pytest.raises(g5.G5Error):
+        g5.s4_decode(bad, n)
+    bad = w.copy()
+    big = ((len(w) - 4) // 4 + 11) * 2048  # more groups than the stream has 4-word starts
+    bad[0] = np.uint32(big)
+    with pytest.raises(g5.G5Error):
+        g5.s4_decode(bad, big)
     assert np.array_equal(g5.s4_decode(w, n), x)  # the codec state is still good
 
 
