@@ -73,6 +73,9 @@ struct g5_bfs {
     uint64_t *d_next = nullptr;
     double bu_ms = 0, bu_bytes = 0;
     int64_t bu_launches = 0, bu_edges = 0;
+    // g5_bfs_run: vertices without edges (a diagonal block), visited from
+    // the start so that no bottom-up level reads their offsets
+    uint64_t *d_isolated = nullptr;
 };
 
 namespace {
@@ -266,7 +269,7 @@ __global__ void k_start(int64_t root_row, int64_t root_col, uint32_t *queue, uin
     cnt[kQLen] = root_row >= 0 ? 1 : 0;
     if (root_row >= 0) queue[0] = (uint32_t)root_row;
     if (root_col >= 0) {
-        visited[root_col >> 6] = 1ull << (root_col & 63);
+        visited[root_col >> 6] |= 1ull << (root_col & 63);
         out[root_col >> 6] = 1ull << (root_col & 63);  // the frontier of a bottom-up first level
         level[root_col] = 0;
         parent[root_col] = kRootMark;
@@ -362,14 +365,38 @@ int sync_counters(g5_bfs *b)
     return G5_OK;
 }
 
-int reset_search(g5_bfs *b, int64_t root)
+// bit v of word v / 64: row v of the (diagonal) block has no neighbours
+__global__ void k_isolated(const int64_t *__restrict__ rowptr, int64_t wc, uint64_t *__restrict__ bits)
+{
+    const int lane = __lane_id();
+    for (int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; w < wc;
+         w += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const int64_t v = w * 64 + lane;
+        const uint64_t word = __ballot(rowptr[v + 1] == rowptr[v]);
+        if (lane == 0) bits[w] = word;
+    }
+}
+
+// isolated: start with the block's edgeless vertices visited (g5_bfs_run on a
+// diagonal block: they are never reached, so results do not change)
+int reset_search(g5_bfs *b, int64_t root, bool isolated = false)
 {
     g5_graph *g = b->g;
     hipStream_t s = b->s;
     const int64_t rr = root - g->row_start, rc = root - g->col_start;
     const int64_t root_row = (root >= 0 && rr >= 0 && rr < g->n_rows) ? rr : -1;
     const int64_t root_col = (root >= 0 && rc >= 0 && rc < g->n_cols) ? rc : -1;
-    G5_HIP(g5::fill_u64(b->d_visited, b->wc, 0, s));
+    if (isolated) {
+        if (!b->d_isolated) {
+            G5_HIP(hipMalloc(&b->d_isolated, (size_t)(b->wc ? b->wc : 1) * 8));
+            hipLaunchKernelGGL(k_isolated, g5::grid_for(b->wc * 64, 256, 4096), 256, 0, s,
+                               (const int64_t *)g->d_rowptr, b->wc, b->d_isolated);
+            G5_HIP(hipGetLastError());
+        }
+        G5_HIP(hipMemcpyAsync(b->d_visited, b->d_isolated, (size_t)b->wc * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        G5_HIP(g5::fill_u64(b->d_visited, b->wc, 0, s));
+    }
     G5_HIP(g5::fill_u64(b->d_out, b->wc, 0, s));
     G5_HIP(g5::fill_u32(b->d_parent, g->n_cols, kNoParent, s));
     G5_HIP(g5::fill_i32(b->d_level, g->n_cols, -1, s));
@@ -423,8 +450,8 @@ extern "C" int g5_bfs_free(g5_bfs *b)
 {
     if (!b) return G5_OK;
     if (b->s) (void)hipStreamSynchronize(b->s);
-    void *dev[] = {b->d_visited, b->d_out, b->d_next, b->d_parent, b->d_level, b->d_queue,
-                   b->d_outq,    b->d_qoff, b->d_scratch, b->d_cnt, b->d_words};
+    void *dev[] = {b->d_visited, b->d_out,     b->d_next, b->d_parent, b->d_level, b->d_queue,
+                   b->d_outq,    b->d_qoff,    b->d_scratch, b->d_cnt, b->d_words, b->d_isolated};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->h_cnt) (void)hipHostFree(b->h_cnt);
@@ -636,7 +663,7 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
     hipStream_t s = b->s;
     G5_HIP(hipStreamSynchronize(s));
     G5_HIP(hipEventRecord(b->ev0, s));
-    int rc = reset_search(b, root);
+    int rc = reset_search(b, root, true);
     if (rc) return rc;
     const int64_t n = g->n_rows;
     int64_t m_u = g->nnz;  // edges of the vertices not yet reached (the root's own are not subtracted)
