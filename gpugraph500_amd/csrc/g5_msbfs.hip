@@ -206,7 +206,7 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // is clear before gathering their frontier word (pays only when few sources
 // are active).
 constexpr int kHits = 4;
-constexpr int kThreadUnroll = 4;
+constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4 (scale 26)
 
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
