@@ -40,7 +40,11 @@ enum Cnt { kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNewEdges = 4, kBuEdges
 // top-down -> bottom-up when the frontier's edges exceed the unexplored
 // edges / kAlpha, back when the frontier holds fewer than n / kBeta vertices
 constexpr int64_t kAlpha = 14, kBeta = 24;
-constexpr int kBuUnroll = 4;
+// frontier bits gathered ahead per lane in the bottom-up scan: 4 on graphs
+// below 2^25 vertices, 2 above, where each gather past the first hit more
+// likely misses the caches (measured at scales 22, 24 and 26; 1 and 8 were
+// slower everywhere)
+constexpr int64_t kBuLarge = (int64_t)1 << 25;
 
 }  // namespace
 
@@ -206,6 +210,7 @@ __global__ __launch_bounds__(256) void k_commit(uint64_t *__restrict__ out, uint
 // 64-vertex bitmap word (lane = vertex), so the next-frontier word is the
 // wave's ballot and needs no atomics.  kBuUnroll frontier bits are gathered
 // before any is tested, to keep several loads in flight per lane.
+template <int kBuUnroll>
 __global__ __launch_bounds__(256) void k_bottom_up(const int64_t *__restrict__ rowptr,
                                                    const uint32_t *__restrict__ cols,
                                                    const uint64_t *__restrict__ visited,
@@ -673,7 +678,8 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
         uint64_t *fresh = b->d_out;
         if (bu) {
             if (b->prof) G5_HIP(hipEventRecord(b->ek0, s));
-            hipLaunchKernelGGL(k_bottom_up, g5::grid_for(b->wc, 4, 4096), 256, 0, s, (const int64_t *)g->d_rowptr,
+            hipLaunchKernelGGL(g->n_rows < kBuLarge ? k_bottom_up<4> : k_bottom_up<2>, g5::grid_for(b->wc, 4, 4096),
+                               256, 0, s, (const int64_t *)g->d_rowptr,
                                (const uint32_t *)g->d_cols, (const uint64_t *)b->d_visited,
                                (const uint64_t *)b->d_out, b->d_next, b->d_parent, b->wc, b->d_cnt);
             G5_HIP(hipGetLastError());
