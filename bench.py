@@ -94,23 +94,33 @@ def _host_csr(g):
     return ref
 
 
-def cpu_baseline(g, scale, root, nedge_gpu):
-    """The oracle's CPUBFS_bin restatement (single thread) on this host, one
-    root of the same graph: the bounded sample of the workload."""
+def cpu_baseline(g, scale, roots, nedge_gpu, budget_s=10.0):
+    """The oracle's CPUBFS_bin restatement (single thread) on this host: whole
+    searches of the same graph's roots, in order, until about budget_s of
+    CPU work (at least one) -- the bounded sample of the workload; TEPS is
+    the harmonic mean over the searches done."""
     O = _oracle()
     ref = _host_csr(g)
-    t0 = time.perf_counter()
-    pred, lev, nl = O.bfs_bin(ref, int(root))
-    t = time.perf_counter() - t0
-    ne = O.nedge(ref, lev)
+    teps, done, t_all, same = [], 0, 0.0, True
+    for k, root in enumerate(roots):
+        t0 = time.perf_counter()
+        pred, lev, nl = O.bfs_bin(ref, int(root))
+        t = time.perf_counter() - t0
+        ne = O.nedge(ref, lev)
+        same = same and int(ne) == int(nedge_gpu[k])
+        teps.append(ne / t)
+        done += 1
+        t_all += t
+        if t_all >= budget_s:
+            break
     return {
-        "value": ne / t / 1e9,
+        "value": hmean(teps) / 1e9,
         "unit": "GTEPS",
         "cores": 1,
         "kind": "port",
-        "sample": f"1 root ({int(root)}) of the scale-{scale} graph, full search, {t:.1f} s, "
+        "sample": f"{done} of {len(roots)} roots of the scale-{scale} graph, whole searches, {t_all:.1f} s, "
                   f"single thread on {cpu_model()} (nproc {os.cpu_count()})",
-        "nedge_matches_gpu": int(ne) == int(nedge_gpu),
+        "nedge_matches_gpu": same,
     }
 
 
@@ -418,7 +428,7 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     }
     m.free()
     if args.cpu_baseline == "on":
-        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots[0], nedge[0])
+        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots, nedge)
     return out
 
 
@@ -513,10 +523,13 @@ def bench_c3(args):
         import gpugraph500_amd as g5
         g5.init(0)
         g = g5.Graph.kronecker(args.scale)
-        roots = g.select_roots(1 << args.scale, 1)
+        roots = g.select_roots(1 << args.scale, 2)
         b = g5.BFS(g)
-        b.run(int(roots[0]))
-        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots[0], b.nedge())
+        nedge = []
+        for r in roots.tolist():
+            b.run(int(r))
+            nedge.append(b.nedge())
+        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots, nedge)
         b.free()
     return out
 
