@@ -169,3 +169,24 @@ def test_rejects_a_2d_block():
     blocks = g5.Graph.kronecker_2d(10, 2, 2)
     with pytest.raises(g5.G5Error):
         g5.Validator(blocks[1])
+
+
+def test_edgeless_graph():
+    """No edges at all: every search reaches only its root, in every path."""
+    rowptr = np.zeros(65, np.int64)
+    g = g5.Graph.upload(rowptr, np.zeros(0, np.uint32), 64)
+    b = g5.BFS(g)
+    nl, _ = b.run(5)
+    pred, lev = b.result()
+    want = np.full(64, -1, np.int64)
+    want[5] = 5
+    assert nl == 1 and np.array_equal(pred, want) and lev[5] == 0 and (lev >= 0).sum() == 1
+    assert b.nedge() == 0
+    m = g5.MSBFS(g)
+    m.run(np.array([0, 7, 7], np.int64))
+    for lane, r in enumerate((0, 7, 7)):
+        p, lv = m.result(lane)
+        assert p[r] == r and (p >= 0).sum() == 1 and lv[r] == 0
+    val = g5.Validator(g)
+    bad, ne, depth, _ = val.validate(5, pred, nglobal=64, want_depth=True)
+    assert bad == 0 and ne == 0 and depth[5] == 0 and (depth >= 0).sum() == 1
