@@ -19,7 +19,9 @@
 // g5_bfs_run (1x1 block, resident) also runs dense levels bottom-up
 // (k_bottom_up): each unvisited vertex scans its sorted neighbour list and
 // stops at the first frontier vertex, which is again the minimum-id frontier
-// neighbour, so both directions give identical parents and levels.
+// neighbour, so both directions give identical parents and levels.  It
+// reads only the counters from the host per level, and builds the sorted
+// queue of step 4 only for a top-down level or the codec.
 #include <algorithm>
 
 #include "g5_codec.hpp"

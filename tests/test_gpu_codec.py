@@ -72,7 +72,7 @@ def _group_starts(w):
 
 
 def test_malformed_long_streams_rejected():
-    """Streams of more than 64 groups take the parallel header search: a bad
+    """Streams of more than 4096 words take the parallel header search: a bad
     width in any group, a truncation and a short output buffer still fail."""
     rng = np.random.default_rng(7)
     n = 400000
@@ -106,11 +106,31 @@ def test_malformed_long_streams_rejected():
 
 @pytest.mark.parametrize("nblk", [64 * 16, 64 * 16 + 1, 65 * 16, 96 * 16 + 5, 32 * 16 * 7])
 def test_group_count_boundaries(nblk):
-    """Group counts either side of the serial/parallel switch and of the
-    32-group anchor spacing, with a varint tail."""
+    """Group counts around the 32-group anchor spacing (all past the 4096-word
+    serial/parallel switch), with a varint tail."""
     rng = np.random.default_rng(nblk)
     n = nblk * 128 + 77
     x = np.sort(rng.choice(1 << 28, size=n, replace=False)).astype(np.uint32)
     w = g5.s4_encode(x)
     assert np.array_equal(w, O.s4_encode(x))
     assert np.array_equal(g5.s4_decode(w, n), x)
+
+
+def test_serial_parallel_switch():
+    """Streams just under and just over 4096 words (the switch between the
+    serial and the parallel header search) decode identically."""
+    rng = np.random.default_rng(41)
+    pool = np.sort(rng.choice(1 << 24, size=60000, replace=False)).astype(np.uint32)
+    under = over = None
+    for n in range(2000, 60000, 500):
+        if len(O.s4_encode(pool[:n])) <= 4096:
+            under = n
+        else:
+            over = n
+            break
+    assert under and over
+    for n in (under, over):
+        x = pool[:n]
+        w = g5.s4_encode(x)
+        assert np.array_equal(w, O.s4_encode(x))
+        assert np.array_equal(g5.s4_decode(w, n), x)
