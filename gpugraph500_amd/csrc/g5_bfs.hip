@@ -82,6 +82,10 @@ struct g5_bfs {
     // g5_bfs_run: vertices without edges (a diagonal block), visited from
     // the start so that no bottom-up level reads their offsets
     uint64_t *d_isolated = nullptr;
+    // hook path: the row frontier as a bitmap (g5_bfs_set_in_bitmap), for
+    // g5_bfs_expand_bottom_up
+    uint64_t *d_inbits = nullptr;
+    bool inbits_valid = false;
 };
 
 namespace {
@@ -414,6 +418,7 @@ int reset_search(g5_bfs *b, int64_t root, bool isolated = false)
     b->root = root;
     b->depth = 0;
     b->qlen = root_row >= 0 ? 1 : 0;
+    b->inbits_valid = false;
     return G5_OK;
 }
 
@@ -458,7 +463,7 @@ extern "C" int g5_bfs_free(g5_bfs *b)
     if (!b) return G5_OK;
     if (b->s) (void)hipStreamSynchronize(b->s);
     void *dev[] = {b->d_visited, b->d_out,     b->d_next, b->d_parent, b->d_level, b->d_queue,
-                   b->d_outq,    b->d_qoff,    b->d_scratch, b->d_cnt, b->d_words, b->d_isolated};
+                   b->d_outq,    b->d_qoff,    b->d_scratch, b->d_cnt, b->d_words, b->d_isolated, b->d_inbits};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->h_cnt) (void)hipHostFree(b->h_cnt);
@@ -566,6 +571,7 @@ extern "C" int g5_bfs_set_inqueue(g5_bfs *b, const uint32_t *q, int64_t n)
     }
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(b->s));
+    b->inbits_valid = false;
     b->depth++;
     return G5_OK;
 }
@@ -618,10 +624,45 @@ extern "C" int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits)
     G5_REQUIRE(b && d_bits, G5_E_ARG, "g5_bfs_set_in_bitmap: null argument");
     G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_set_in_bitmap: g5_bfs_start not called");
     G5_HIP(g5::bitmap_to_queue(d_bits, b->wr, b->d_queue, b->d_cnt + kQLen, b->d_scratch, b->s));
+    if (!b->d_inbits) G5_HIP(hipMalloc(&b->d_inbits, (size_t)(b->wr ? b->wr : 1) * 8));
+    if (b->wr) G5_HIP(hipMemcpyAsync(b->d_inbits, d_bits, (size_t)b->wr * 8, hipMemcpyDeviceToDevice, b->s));
     int rc = sync_counters(b);
     if (rc) return rc;
     b->qlen = b->h_cnt[kQLen];
+    b->inbits_valid = true;
     b->depth++;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_expand_bottom_up(g5_bfs *b, const g5_graph *gt, int *any_new)
+{
+    G5_REQUIRE(b && gt && any_new, G5_E_ARG, "g5_bfs_expand_bottom_up: null argument");
+    G5_REQUIRE(b->depth >= 0 && b->inbits_valid, G5_E_STATE,
+               "g5_bfs_expand_bottom_up: needs a row frontier from g5_bfs_set_in_bitmap");
+    const g5_graph *g = b->g;
+    G5_REQUIRE(gt->n_rows == g->n_cols && gt->n_cols == g->n_rows && gt->row_start == g->col_start &&
+                   gt->col_start == g->row_start,
+               G5_E_ARG, "g5_bfs_expand_bottom_up: gt is not the transposed block");
+    hipStream_t s = b->s;
+    G5_HIP(g5::fill_i64(b->d_cnt + kNew, kBuVerts - kNew + 1, 0, s));
+    if (b->prof) G5_HIP(hipEventRecord(b->ek0, s));
+    // rows of gt = this block's columns, its columns = this block's rows: the
+    // kernel's frontier bits are the row frontier, its parents local row ids
+    hipLaunchKernelGGL(gt->n_rows < kBuLarge ? k_bottom_up<4> : k_bottom_up<2>, g5::grid_for(b->wc, 4, 4096), 256, 0,
+                       s, (const int64_t *)gt->d_rowptr, (const uint32_t *)gt->d_cols,
+                       (const uint64_t *)b->d_visited, (const uint64_t *)b->d_inbits, b->d_out, b->d_parent, b->wc,
+                       b->d_cnt);
+    G5_HIP(hipGetLastError());
+    if (b->prof) G5_HIP(hipEventRecord(b->ek1, s));
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, s, b->d_cnt, (int)kOutLen, (int64_t)0);
+    hipLaunchKernelGGL(k_count_bits, g5::grid_for(b->wc, 256, 2048), 256, 0, s, (const uint64_t *)b->d_out, b->wc,
+                       b->d_cnt + kOutLen);
+    G5_HIP(hipGetLastError());
+    int rc = sync_counters(b);
+    if (rc) return rc;
+    rc = account_bottom_up(b, b->h_cnt[kOutLen]);
+    if (rc) return rc;
+    *any_new = b->h_cnt[kOutLen] > 0;
     return G5_OK;
 }
 

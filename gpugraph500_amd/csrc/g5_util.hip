@@ -297,6 +297,37 @@ extern "C" int g5_bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint3
 }
 
 namespace {
+__global__ void k_popcount(const uint64_t *__restrict__ bits, int64_t n_words, unsigned long long *__restrict__ out)
+{
+    int64_t c = 0;
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n_words; w += (int64_t)gridDim.x * blockDim.x)
+        c += __popcll(bits[w]);
+    c = g5_block_sum256(c);
+    if (threadIdx.x == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+}  // namespace
+
+extern "C" int g5_bitmap_count(const uint64_t *d_bits, int64_t n_words, int64_t *count)
+{
+    G5_REQUIRE(n_words >= 0 && count && (n_words == 0 || d_bits), G5_E_ARG, "g5_bitmap_count: bad arguments");
+    *count = 0;
+    if (n_words == 0) return G5_OK;
+    unsigned long long *d_c = nullptr;
+    G5_HIP(hipMalloc(&d_c, 8));
+    hipError_t e = g5::fill_u64((uint64_t *)d_c, 1, 0, nullptr);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_popcount, g5::grid_for(n_words, 256, 2048), 256, 0, nullptr, d_bits, n_words, d_c);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpy(&h, d_c, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d_c);
+    G5_HIP(e);
+    *count = (int64_t)h;
+    return G5_OK;
+}
+
+namespace {
 // ids at or beyond n_words * 64 are counted in *bad and skipped
 __global__ void k_q2bm_checked(const uint32_t *__restrict__ q, int64_t n, uint64_t *__restrict__ bits,
                                int64_t n_words, unsigned long long *__restrict__ bad)

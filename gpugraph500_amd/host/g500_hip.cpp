@@ -5,7 +5,7 @@
 //
 //   g500_hip [-s scale] [-e edgefactor] [-R rows] [-C cols] [-n roots]
 //            [-x host|device] [-be codec] [-btr threshold] [-share r c]
-//            [-verify] [-validate] [-dump file] [-dumpk k]
+//            [-td] [-verify] [-validate] [-dump file] [-dumpk k]
 //
 //   -x host     the reference's hook protocol (host queues, every message
 //               through the Compression schema); device: bitmaps in HBM
@@ -13,6 +13,8 @@
 //               runs on the fold's queues
 //   -share r c  the rank whose share (expansion, its column's reduce, its
 //               row's fold) is timed separately (default 0 0)
+//   -td         top-down levels only (device exchange otherwise runs dense
+//               levels bottom-up over the transposed blocks)
 //   -verify     compare every search with the 1 x 1 search (g5_bfs_run on the
 //               undivided graph): predecessors must be identical
 //   -validate   check every search's predecessors on the device as the
@@ -49,7 +51,7 @@ void check(int rc, const char *what)
 struct Opts {
     int scale = 16, edgefactor = 16, R = 1, C = 1, roots = 64, r0 = 0, c0 = 0, dumpk = 4, threshold = 64;
     std::string exchange = "device", codec, dump;
-    bool verify = false, validate = false;
+    bool verify = false, validate = false, td = false;
 };
 
 Opts parse(int argc, char **argv)
@@ -74,6 +76,7 @@ Opts parse(int argc, char **argv)
             o.c0 = std::stoi(next());
         } else if (a == "-verify") o.verify = true;
         else if (a == "-validate") o.validate = true;
+        else if (a == "-td") o.td = true;
         else if (a == "-dump") o.dump = next();
         else if (a == "-dumpk") o.dumpk = std::stoi(next());
         else throw std::invalid_argument("unknown option " + a);
@@ -127,7 +130,7 @@ int run(const Opts &o)
     }
     std::unique_ptr<Grid2D> gridp(new Grid2D(o.R, o.C, blocks.data(),
                                              o.exchange == "host" ? Exchange::HostQueues : Exchange::Device, schema,
-                                             device_codec));
+                                             device_codec, !o.td));
     Grid2D &grid = *gridp;
 
     FILE *dump = nullptr;
@@ -140,7 +143,7 @@ int run(const Opts &o)
     std::vector<double> teps, share_expand, share_column, share_fold, totals;
     std::vector<int64_t> pred, ref, vpred;
     std::vector<double> valid_time;
-    int mismatches = 0, invalid = 0;
+    int mismatches = 0, invalid = 0, bu_levels = 0;
     double codec_s = 0, pred_s = 0;
     int64_t raw_words = 0, coded_words = 0;
     // one untimed search first (device buffers, code objects), then the
@@ -197,10 +200,11 @@ int run(const Opts &o)
             fwrite(&n, 8, 1, dump);
             fwrite(pred.data(), 8, (size_t)n, dump);
         }
-        printf("search %zu root %lld levels %d nedge %lld time %.6f s TEPS %.4e share expand %.6f column %.6f fold "
-               "%.6f s\n",
-               i, (long long)roots[i], st.levels, (long long)ne, st.total_s, ne / st.total_s, st.share_expand_s,
-               st.share_column_s, st.share_fold_s);
+        printf("search %zu root %lld levels %d (bottom-up %d) nedge %lld time %.6f s TEPS %.4e share expand %.6f "
+               "column %.6f fold %.6f s\n",
+               i, (long long)roots[i], st.levels, st.bottom_up_levels, (long long)ne, st.total_s, ne / st.total_s,
+               st.share_expand_s, st.share_column_s, st.share_fold_s);
+        bu_levels += st.bottom_up_levels;
     }
     if (dump) fclose(dump);
     const double hm = hmean(teps);
@@ -232,11 +236,11 @@ int run(const Opts &o)
            "\"share_column_s\": %.6f, \"share_fold_s\": %.6f, \"codec_s\": %.6f, \"pred_s\": %.6f, "
            "\"raw_words\": %lld, \"coded_words\": %lld, \"verify_mismatches\": %d, \"share_kexpand_ms\": %.6f, "
            "\"share_kexpand_launches\": %lld, \"share_kexpand_bytes\": %.6e, \"share_kexpand_edges\": %lld, "
-           "\"validate_failures\": %d}\n",
+           "\"validate_failures\": %d, \"bottom_up_levels\": %d}\n",
            o.scale, o.R, o.C, teps.size(), o.exchange.c_str(), o.codec.c_str(), hm, mean_t / k, o.r0, o.c0,
            mean_se / k, mean_sc / k, mean_sf / k, codec_s / k, pred_s / k, (long long)raw_words,
            (long long)coded_words, o.verify ? mismatches : -1, kx_ms, (long long)kx_launches, kx_bytes,
-           (long long)kx_edges, o.validate ? invalid : -1);
+           (long long)kx_edges, o.validate ? invalid : -1, bu_levels);
     gridp.reset();
     if (validator) g5_validator_free(validator);
     if (single) g5_bfs_free(single);

@@ -33,8 +33,9 @@ X *dalloc(size_t n)
 }  // namespace
 
 Grid2D::Grid2D(int R, int C, g5_graph **blocks, Exchange mode, Compression<int64_t, uint32_t> *schema,
-               bool device_codec)
-    : R_(R), C_(C), mode_(mode), schema_(schema), device_codec_(device_codec)
+               bool device_codec, bool bottom_up)
+    : R_(R), C_(C), mode_(mode), schema_(schema), device_codec_(device_codec),
+      bottom_up_(bottom_up && mode == Exchange::Device), blocks_(blocks, blocks + (size_t)R * C)
 {
     if (R < 1 || C < 1 || R != C) throw std::invalid_argument("Grid2D: needs R == C >= 1 (symmetric partitioning)");
     for (int i = 0; i < R * C; ++i) ranks_.emplace_back(new HipBFS(blocks[i]));
@@ -166,6 +167,11 @@ void Grid2D::exchange_device(GridStats &st, int r0, int c0)
         }
         uint64_t *red = d_reduced_ + (size_t)c * wmax;
         check(g5_vreduce_or(d_colmaps_, R_, w, red), "g5_vreduce_or");
+        if (bottom_up_) {
+            int64_t k = 0;
+            check(g5_bitmap_count(red, w, &k), "g5_bitmap_count");
+            last_new_ += k;
+        }
         for (int r = 0; r < R_; ++r) rank(r, c).setModOutgoingBitmap(red);
         st.raw_words += 2 * w * (R_ - 1);  // 32-bit words of the bitmaps gathered
         st.coded_words += 2 * w * (R_ - 1);
@@ -204,21 +210,31 @@ void Grid2D::runBFS(int64_t root, std::vector<int64_t> &pred, GridStats &st, int
     st = GridStats();
     const double t_begin = now();
     for (auto &h : ranks_) h->setStartVertex(root);
+    bool bu = false;  // the first level is top-down
     for (;;) {
         bool any = false;
         for (int r = 0; r < R_; ++r)
             for (int c = 0; c < C_; ++c) {
                 const double t0 = now();
-                rank(r, c).runLocalBFS();
+                if (bu) rank(r, c).runLocalBFSBottomUp(blocks_[(size_t)c * C_ + r]);
+                else rank(r, c).runLocalBFS();
                 const double dt = now() - t0;
                 st.expand_s += dt;
                 if (r == r0 && c == c0) st.share_expand_s += dt;
                 any = any || rank(r, c).istheresomethingnew();
             }
         if (!any) break;  // the allreduce of istheresomethingnew (globalbfs.hh:716-739)
+        st.bottom_up_levels += bu ? 1 : 0;
+        last_new_ = 0;
         if (mode_ == Exchange::HostQueues) exchange_host(st, r0, c0);
         else exchange_device(st, r0, c0);
         ++st.levels;
+        // next direction: bottom-up once a level reaches more than n / 64
+        // vertices, top-down again below n / 24 (the 1 x 1 search's return rule)
+        if (bottom_up_) {
+            if (!bu && last_new_ > n_total_ / 64) bu = true;
+            else if (bu && last_new_ < n_total_ / 24) bu = false;
+        }
     }
     // predecessors: per column, the minimum over the column's contributions
     const double t0 = now();

@@ -25,6 +25,10 @@
 //                         queue -> s4-bp128-d4 encode -> decode -> bitmap),
 //                         and g5_vreduce_min_pred for the predecessors
 // Both give the same predecessors as the 1 x 1 search (minimum-id parents).
+// With Exchange::Device the dense levels can run bottom-up (bottom_up): rank
+// (r, c) scans its unvisited columns' neighbours in block (c, r), the
+// transposed block, and stops at the first vertex of its row frontier; the
+// switch follows the global count of newly reached vertices.
 #pragma once
 
 #include <cstdint>
@@ -40,6 +44,7 @@ enum class Exchange { HostQueues, Device };
 
 struct GridStats {
     int levels = 0;
+    int bottom_up_levels = 0;
     double total_s = 0;      // whole emulated search (all ranks in sequence)
     double expand_s = 0;     // runLocalBFS over all ranks
     double column_s = 0;     // column step (reduce + adopt) over all columns
@@ -58,7 +63,8 @@ public:
     // blocks[r * C + c] as g5_graph_kronecker_2d returns them (owned by the
     // caller); R == C (the fold assumes symmetric partitioning, as the CUDA
     // path does).  schema may be null (raw transfer).
-    Grid2D(int R, int C, g5_graph **blocks, Exchange mode, Compression<int64_t, uint32_t> *schema, bool device_codec);
+    Grid2D(int R, int C, g5_graph **blocks, Exchange mode, Compression<int64_t, uint32_t> *schema, bool device_codec,
+           bool bottom_up = false);
     ~Grid2D();
 
     // One search; pred receives nglobal() global predecessors (-1 unreached,
@@ -76,6 +82,9 @@ private:
     Exchange mode_;
     Compression<int64_t, uint32_t> *schema_;
     bool device_codec_;
+    bool bottom_up_;
+    std::vector<g5_graph *> blocks_;  // [r * C + c], not owned
+    int64_t last_new_ = 0;            // vertices reached by the last level (bottom_up_)
     std::vector<std::unique_ptr<HipBFS>> ranks_;
     std::vector<int64_t> col_start_, col_len_;
     int64_t n_total_ = 0, max_cols_ = 0;

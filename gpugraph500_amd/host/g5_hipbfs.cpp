@@ -49,6 +49,13 @@ void HipBFS::runLocalBFS()
     something_new_ = any != 0;
 }
 
+void HipBFS::runLocalBFSBottomUp(const g5_graph *transposed)
+{
+    int any = 0;
+    check(g5_bfs_expand_bottom_up(b_, transposed, &any), "g5_bfs_expand_bottom_up");
+    something_new_ = any != 0;
+}
+
 bool HipBFS::istheresomethingnew() { return something_new_; }
 
 // cuda_bfs.cu:380-475: the column out-queue, sorted and unique, host-visible

@@ -50,6 +50,9 @@ public:
     const uint64_t *outBitmap(int64_t &n_words);
     void setModOutgoingBitmap(const uint64_t *d_bits);
     void setIncomingBitmap(const uint64_t *d_bits);
+    // runLocalBFS as a bottom-up level over the transposed block (row frontier
+    // from setIncomingBitmap)
+    void runLocalBFSBottomUp(const g5_graph *transposed);
     void predecessorDevice(int64_t *d_pred);
 
     g5_bfs *handle() { return b_; }

@@ -58,6 +58,9 @@ int g5_fill_u64(uint64_t *d, int64_t n, uint64_t value);
  * g5_queue_to_bitmap clears d_bits first and rejects ids >= n_words * 64. */
 int g5_bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint32_t *d_queue, int64_t *n);
 int g5_queue_to_bitmap(const uint32_t *d_queue, int64_t n, uint64_t *d_bits, int64_t n_words);
+/* *count = set bits of d_bits[n_words] (the frontier size a direction switch
+ * needs, without building the queue). */
+int g5_bitmap_count(const uint64_t *d_bits, int64_t n_words, int64_t *count);
 
 /* ---- the DistMatrix2d block (cpu_2d/distmatrix2d.hh:30-130) ----
  * Upload a host CSR block: rowptr int64[n_rows+1], cols uint32[nnz] local
@@ -141,6 +144,17 @@ int g5_bfs_out_bitmap(g5_bfs *b, const uint64_t **d_bits, int64_t *n_words);
 int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred);
+/* A bottom-up level of the hook path, the alternative to g5_bfs_expand for
+ * dense levels (no counterpart in the reference, whose CUDA path is
+ * top-down only): every unvisited column vertex of the block scans its
+ * neighbours in the block's row range in ascending order and stops at the
+ * first vertex of the row frontier, which must have been set with
+ * g5_bfs_set_in_bitmap.  gt is the transposed block (for a symmetric graph
+ * on an R x R grid, block (c, r) of rank (r, c)): its rows are this block's
+ * columns.  The out-bitmap, parents and *any_new are left as g5_bfs_expand
+ * leaves them, so the column and row steps are unchanged; each rank's
+ * parents are again the minimum-id frontier vertex of its row range. */
+int g5_bfs_expand_bottom_up(g5_bfs *b, const g5_graph *gt, int *any_new);
 /* The whole 1x1 runBFS (globalbfs.hh:548-1263) resident in HBM: no host
  * round trip per level.  Dense levels run bottom-up (every unvisited vertex
  * stops at its first -- minimum-id -- frontier neighbour), chosen by the

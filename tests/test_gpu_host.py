@@ -41,19 +41,25 @@ def _read_dump(path):
     (3, "host", "s4-bp128-d4"),
     (4, "device", "s4-bp128-d4"),
     (4, "host", "s4-bp128-d4"),
+    (2, "device-td", ""),
+    (4, "device-td", "s4-bp128-d4"),
 ])
 def test_grid_matches_oracle(tmp_path, oracle_csr, R, x, codec):
     """Every search of the emulated grid equals the 1 x 1 search (-verify) and
-    the dumped predecessors are the oracle's minimum-id parents."""
+    the dumped predecessors are the oracle's minimum-id parents; the device
+    exchange runs its dense levels bottom-up unless -td."""
     dump = str(tmp_path / "pred.bin")
-    cmd = [gbuild.DRIVER, "-s", str(SCALE), "-R", str(R), "-C", str(R), "-n", "6", "-x", x, "-btr", "16",
-           "-verify", "-dump", dump, "-dumpk", "3"]
+    cmd = [gbuild.DRIVER, "-s", str(SCALE), "-R", str(R), "-C", str(R), "-n", "6", "-x", x.split("-")[0], "-btr",
+           "16", "-verify", "-dump", dump, "-dumpk", "3"] + (["-td"] if x.endswith("-td") else [])
     if codec:
         cmd += ["-be", codec]
     print("case", cmd, flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "verify_mismatches: 0" in r.stdout
+    if x == "device":
+        import json
+        assert json.loads(r.stdout.strip().splitlines()[-1])["bottom_up_levels"] > 0
     for root, pred in _read_dump(dump):
         ref, _, _ = O.bfs_bin(oracle_csr, root)
         n = min(len(ref), len(pred))
