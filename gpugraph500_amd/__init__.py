@@ -65,6 +65,8 @@ def lib():
     _sig(L, "g5_bfs_set_out_bitmap", I, [P, P])
     _sig(L, "g5_bfs_set_in_bitmap", I, [P, P])
     _sig(L, "g5_bfs_pred_device", I, [P, P])
+    _sig(L, "g5_bfs_expand_bottom_up", I, [P, P, ctypes.POINTER(I)])
+    _sig(L, "g5_bitmap_count", I, [P, c_i64, ctypes.POINTER(c_i64)])
     _sig(L, "g5_graph_info", I, [P, P, P, P, P, P])
     _sig(L, "g5_graph_download", I, [P, P, P, P])
     _sig(L, "g5_graph_free", I, [P])
@@ -259,6 +261,13 @@ class BFS:
     def expand(self):
         a = ctypes.c_int()
         check(lib().g5_bfs_expand(self.h, ctypes.byref(a)))
+        return bool(a.value)
+
+    def expand_bottom_up(self, transposed):
+        """A bottom-up level over the transposed block (a Graph); the row
+        frontier must come from set_in_bitmap."""
+        a = ctypes.c_int()
+        check(lib().g5_bfs_expand_bottom_up(self.h, transposed.h, ctypes.byref(a)))
         return bool(a.value)
 
     def outqueue(self):
@@ -545,3 +554,10 @@ class Validator:
             self.free()
         except Exception:
             pass
+
+
+def bitmap_count(d_bits, n_words):
+    """Set bits of a device bitmap (DeviceArray of uint64)."""
+    c = c_i64()
+    check(lib().g5_bitmap_count(d_bits.ptr, n_words, ctypes.byref(c)))
+    return c.value
