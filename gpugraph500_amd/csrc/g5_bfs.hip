@@ -36,8 +36,11 @@ constexpr int kExpandTile = 2048;            // edges per block tile
 constexpr int kExpandGrid = 4096;            // persistent grid, grid-stride over tiles
 
 // kNewEdges: degree sum of the newly reached vertices (1x1 only); kBuEdges,
-// kBuVerts: neighbours checked and vertices scanned by a bottom-up level
-enum Cnt { kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNewEdges = 4, kBuEdges = 5, kBuVerts = 6, kNCnt = 8 };
+// kBuVerts: neighbours checked and vertices scanned by a bottom-up level;
+// kCodecErr: set when a level queue's codec round trip failed (g5_bfs_run)
+enum Cnt {
+    kQLen = 0, kEdges = 1, kNew = 2, kOutLen = 3, kNewEdges = 4, kBuEdges = 5, kBuVerts = 6, kCodecErr = 7, kNCnt = 8
+};
 // direction switch of the 1x1 search (Beamer, Asanovic, Patterson, SC'12):
 // top-down -> bottom-up when the frontier's edges exceed the unexplored
 // edges / kAlpha, back when the frontier holds fewer than n / kBeta vertices
@@ -740,6 +743,8 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
         if (bu) std::swap(b->d_out, b->d_next);
         rc = sync_counters(b);
         if (rc) return rc;
+        // the previous level's codec round trip has completed by now
+        G5_REQUIRE(b->h_cnt[kCodecErr] == 0, G5_E_STREAM, "g5_bfs_run: codec round trip lost values");
         const int64_t n_new = b->h_cnt[kNew];
         rc = bu ? account_bottom_up(b, n_new) : account_level(b, b->qlen, n_new);
         if (rc) return rc;
@@ -758,12 +763,10 @@ extern "C" int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, 
         if (!bu_next || codec)
             G5_HIP(g5::bitmap_to_queue(b->d_out, b->wc, b->d_queue, b->d_cnt + kQLen, b->d_scratch, s));
         if (codec) {
-            size_t nw = 0, nd = 0;
-            rc = b->codec.encode(b->d_queue, b->qlen, b->d_words, b->words_cap, &nw, s);
+            // encode -> decode in place with no host synchronisation; checked
+            // at the next level's counter read
+            rc = b->codec.roundtrip(b->d_queue, b->qlen, b->d_words, b->words_cap, b->d_queue, b->d_cnt + kCodecErr, s);
             if (rc) return rc;
-            rc = b->codec.decode(b->d_words, nw, b->d_queue, (size_t)g->n_rows, &nd, s);
-            if (rc) return rc;
-            G5_REQUIRE((int64_t)nd == b->qlen, G5_E_STREAM, "g5_bfs_run: codec round trip lost values");
         }
         bu = bu_next;
         b->depth++;

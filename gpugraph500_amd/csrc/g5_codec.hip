@@ -183,12 +183,13 @@ enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 
 constexpr int kWalkChunk = 8192;
 
 // One workgroup: stream header check and the group-header chain.
-__global__ __launch_bounds__(256) void k_dec_walk(const uint32_t *__restrict__ in, int64_t nwords,
+__global__ __launch_bounds__(256) void k_dec_walk(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp,
                                                   int64_t cap, int64_t *__restrict__ grp,
                                                   int64_t *__restrict__ meta)
 {
     __shared__ uint32_t lds[kWalkChunk];
     __shared__ int64_t s_pos, s_g, s_err;
+    const int64_t nwords = *nwp;
     const int64_t L = in[0];
     const int64_t nblk = L / 128, ngroups = (nblk + 15) / 16;
     if (threadIdx.x == 0) {
@@ -240,10 +241,11 @@ __global__ __launch_bounds__(256) void k_dec_walk(const uint32_t *__restrict__ i
 // One wave per miniblock: data offset and the (reset, lane sums) record.
 __global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ in, int64_t nblk,
                                                    const int64_t *__restrict__ grp,
-                                                   int64_t *__restrict__ off, uint32_t *__restrict__ carry)
+                                                   int64_t *__restrict__ off, uint32_t *__restrict__ carry,
+                                                   const int64_t *__restrict__ meta)
 {
     const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= nblk) return;
+    if (i >= nblk || meta[kMetaErr]) return;  // no group positions to read on an error
     const int l = threadIdx.x & 63;
     const int64_t g = i >> 4;
     const int64_t hp = grp[g];
@@ -340,8 +342,10 @@ constexpr int32_t kBadWidth = -1, kTruncated = -2;
 
 // nxt[i] = index of the group start after candidate start p = 4 + 4i, or
 // kBadWidth / kTruncated if a group header at p is impossible
-__global__ void k_dec_next(const uint32_t *__restrict__ in, int64_t nwords, int64_t ncand, int32_t *__restrict__ nxt)
+__global__ void k_dec_next(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp, int64_t ncand,
+                           int32_t *__restrict__ nxt)
 {
+    const int64_t nwords = *nwp;  // candidates past the stream get kTruncated
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncand; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = 4 + 4 * i;
         int32_t r = kTruncated;
@@ -378,10 +382,11 @@ __global__ void k_dec_jump(const int32_t *__restrict__ in, int64_t ncand, int32_
 // then anchor[k] = start index of group 32k, k = 0..(ngroups-1)/32; the
 // jumps between anchors never cross the last group, whose unused widths are
 // not read.  meta gets L, the group and anchor counts.
-__global__ void k_dec_anchors(const uint32_t *__restrict__ in, int64_t cap, int64_t ncand,
+__global__ void k_dec_anchors(const uint32_t *__restrict__ in, int64_t cap, const int64_t *__restrict__ nwp,
                               const int32_t *__restrict__ jmp, int32_t *__restrict__ anchor,
                               int64_t *__restrict__ meta)
 {
+    const int64_t ncand = (*nwp - 4) / 4 + 1;
     const int64_t L = in[0], ngroups = (L / 128 + 15) / 16;
     meta[kMetaL] = L;
     meta[kMetaGroups] = ngroups;
@@ -414,12 +419,13 @@ __global__ void k_dec_anchors(const uint32_t *__restrict__ in, int64_t cap, int6
 
 // thread k: group starts 32k .. 32k+31; the last group's own widths (only
 // the miniblocks it holds) give the tail position
-__global__ void k_dec_fill(const uint32_t *__restrict__ in, int64_t nwords, const int32_t *__restrict__ nxt,
-                           const int32_t *__restrict__ anchor, int64_t ncand, int64_t *__restrict__ grp,
-                           int64_t *__restrict__ meta)
+__global__ void k_dec_fill(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp,
+                           const int32_t *__restrict__ nxt, const int32_t *__restrict__ anchor, int64_t ncand,
+                           int64_t *__restrict__ grp, int64_t *__restrict__ meta)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= meta[kMetaAnchors] || meta[kMetaErr]) return;
+    const int64_t nwords = *nwp;
     const int64_t ngroups = meta[kMetaGroups], nblk = meta[kMetaL] / 128;
     int32_t a = anchor[k];
     for (int j = 0; j < (1 << kJumpLog); ++j) {
@@ -446,6 +452,18 @@ __global__ void k_dec_fill(const uint32_t *__restrict__ in, int64_t nwords, cons
             return;
         }
     }
+}
+
+// round trip: the decoded stream must hold the L values the encoder wrote
+__global__ void k_dec_expect(int64_t *meta, int64_t L)
+{
+    if (!meta[kMetaErr] && meta[kMetaL] != L) meta[kMetaErr] = 6;
+}
+
+// round trip: *flag := 1 unless exactly n values came back without error
+__global__ void k_dec_verify(const int64_t *meta, int64_t n, int64_t *flag)
+{
+    if (meta[kMetaErr] || meta[kMetaL] + meta[kMetaTail] != n) *flag = 1;
 }
 
 int64_t carry_scratch_records(int64_t n)
@@ -478,8 +496,9 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ 
                                                    const int64_t *__restrict__ grp,
                                                    const int64_t *__restrict__ off,
                                                    const uint32_t *__restrict__ carry,
-                                                   uint32_t *__restrict__ out)
+                                                   uint32_t *__restrict__ out, const int64_t *__restrict__ meta)
 {
+    if (meta[kMetaErr]) return;
     const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= nblk) return;
     const int l = threadIdx.x & 63;
@@ -519,9 +538,11 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ 
 }
 
 // varint tail, one thread; meta[kMetaTail] = values, or kMetaErr on overflow
-__global__ void k_dec_tail(const uint32_t *__restrict__ in, int64_t nwords, int64_t L, int64_t cap,
+__global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp, int64_t L, int64_t cap,
                            uint32_t *__restrict__ out, int64_t *__restrict__ meta)
 {
+    if (meta[kMetaErr]) return;  // the group positions are not valid
+    const int64_t nwords = *nwp;
     const int64_t tp = meta[kMetaTailPos];
     const uint8_t *bi = (const uint8_t *)(in + tp);
     const int64_t nb = (nwords - tp) * 4;
@@ -568,6 +589,7 @@ int S4Codec::reserve(int64_t n)
     if (e == hipSuccess) e = hipMalloc(&d_jtmp, (size_t)ncand_cap * 4);
     if (e == hipSuccess) e = hipMalloc(&d_anchor, (size_t)(ngrp / 32 + 2) * 4);
     if (e == hipSuccess) e = hipMalloc(&d_meta, 16 * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_nw, 2 * 8);
     if (e == hipSuccess) e = hipHostMalloc(&h_meta, 16 * 8, hipHostMallocDefault);
     if (e != hipSuccess) {
         release();
@@ -579,25 +601,22 @@ int S4Codec::reserve(int64_t n)
 
 void S4Codec::release()
 {
-    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta, d_nxt, d_jmp, d_jtmp, d_anchor};
+    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta, d_nxt, d_jmp, d_jtmp, d_anchor, d_nw};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (h_meta) (void)hipHostFree(h_meta);
     d_words = d_scan = d_off = d_grp = d_meta = nullptr;
     d_carry = d_carry_scan = nullptr;
     d_nxt = d_jmp = d_jtmp = d_anchor = nullptr;
+    d_nw = nullptr;
     ncand_cap = 0;
     h_meta = nullptr;
     cap = 0;
 }
 
-int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap_words, size_t *nwords,
-                    hipStream_t s)
+// the encode kernels; the stream's word count goes to d_nw[0]
+int S4Codec::encode_launch(const uint32_t *d_in, int64_t n, uint32_t *d_out, hipStream_t s)
 {
-    G5_REQUIRE(n >= 0 && cap_words >= g5_s4_max_words((size_t)n), G5_E_OVERFLOW,
-               "s4 encode: output capacity %zu below the bound for %lld values", cap_words, (long long)n);
-    int rc = reserve(n);
-    if (rc) return rc;
     const int64_t L = n / 128 * 128, nblk = L / 128;
     if (nblk > 0) {
         hipLaunchKernelGGL(k_enc_widths, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk, d_words);
@@ -609,11 +628,73 @@ int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap
                            (const int64_t *)d_words, d_out);
         G5_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_enc_tail, 1, 64, 0, s, d_in, n, L, nblk, (const int64_t *)d_words, d_out, d_meta);
+    hipLaunchKernelGGL(k_enc_tail, 1, 64, 0, s, d_in, n, L, nblk, (const int64_t *)d_words, d_out, d_nw);
     G5_HIP(hipGetLastError());
-    G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8, hipMemcpyDeviceToHost, s));
+    return G5_OK;
+}
+
+int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap_words, size_t *nwords,
+                    hipStream_t s)
+{
+    G5_REQUIRE(n >= 0 && cap_words >= g5_s4_max_words((size_t)n), G5_E_OVERFLOW,
+               "s4 encode: output capacity %zu below the bound for %lld values", cap_words, (long long)n);
+    int rc = reserve(n);
+    if (rc) return rc;
+    rc = encode_launch(d_in, n, d_out, s);
+    if (rc) return rc;
+    G5_HIP(hipMemcpyAsync(h_meta, d_nw, 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     *nwords = (size_t)h_meta[0];
+    return G5_OK;
+}
+
+// the group-header search of a stream whose word count is *nwp on the device
+// and at most words_bound: meta gets L, the groups, their positions in grp
+// and the tail position, or an error
+int S4Codec::search_launch(const uint32_t *d_in, const int64_t *nwp, int64_t words_bound, int64_t cap_values,
+                           hipStream_t s)
+{
+    G5_HIP(fill_i64(d_meta, 16, 0, s));
+    const int64_t ncand = (words_bound - 4) / 4 + 1;  // bound on the 4-word group starts
+    if (words_bound <= kSerialWords || ncand > ncand_cap) {
+        hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, nwp, cap_values, d_grp, d_meta);
+        G5_HIP(hipGetLastError());
+        return G5_OK;
+    }
+    // a stream holds at most ncand groups (k_dec_anchors rejects more)
+    const int64_t nanch_max = ((ncand - 1) >> kJumpLog) + 1;
+    const unsigned grid = (unsigned)grid_for(ncand, 256, 16384);
+    hipLaunchKernelGGL(k_dec_next, grid, 256, 0, s, d_in, nwp, ncand, d_nxt);
+    const int32_t *src = d_nxt;
+    int32_t *bufs[2] = {d_jmp, d_jtmp};
+    for (int r = 0; r < kJumpLog; ++r) {
+        hipLaunchKernelGGL(k_dec_jump, grid, 256, 0, s, src, ncand, bufs[r & 1]);
+        src = bufs[r & 1];
+    }
+    hipLaunchKernelGGL(k_dec_anchors, 1, 1, 0, s, d_in, cap_values, nwp, src, d_anchor, d_meta);
+    hipLaunchKernelGGL(k_dec_fill, (unsigned)((nanch_max + 255) / 256), 256, 0, s, d_in, nwp,
+                       (const int32_t *)d_nxt, (const int32_t *)d_anchor, ncand, d_grp, d_meta);
+    G5_HIP(hipGetLastError());
+    return G5_OK;
+}
+
+// miniblocks and the varint tail of a stream of L values whose header search
+// has run (every kernel does nothing when meta holds an error)
+int S4Codec::unpack_launch(const uint32_t *d_in, const int64_t *nwp, int64_t L, int64_t cap_values, uint32_t *d_out,
+                           hipStream_t s)
+{
+    const int64_t nblk = L / 128;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_dec_local, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk, (const int64_t *)d_grp,
+                           d_off, d_carry, (const int64_t *)d_meta);
+        G5_HIP(hipGetLastError());
+        G5_HIP(carry_scan(d_carry, nblk, d_carry_scan, s));
+        hipLaunchKernelGGL(k_dec_write, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk, (const int64_t *)d_grp,
+                           (const int64_t *)d_off, (const uint32_t *)d_carry, d_out, (const int64_t *)d_meta);
+        G5_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_dec_tail, 1, 1, 0, s, d_in, nwp, L, cap_values, d_out, d_meta);
+    G5_HIP(hipGetLastError());
     return G5_OK;
 }
 
@@ -623,47 +704,41 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
     G5_REQUIRE(nwords >= 4, G5_E_STREAM, "s4 decode: stream of %zu words has no header", nwords);
     int rc = reserve((int64_t)cap_values);
     if (rc) return rc;
-    G5_HIP(fill_i64(d_meta, 16, 0, s));
-    const int64_t ncand = ((int64_t)nwords - 4) / 4 + 1;
-    if ((int64_t)nwords <= kSerialWords || ncand > ncand_cap) {
-        hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, (int64_t)nwords, (int64_t)cap_values, d_grp, d_meta);
-        G5_HIP(hipGetLastError());
-    } else {
-        // a stream holds at most ncand groups (k_dec_anchors rejects more)
-        const int64_t nanch_max = ((ncand - 1) >> kJumpLog) + 1;
-        const unsigned grid = (unsigned)grid_for(ncand, 256, 16384);
-        hipLaunchKernelGGL(k_dec_next, grid, 256, 0, s, d_in, (int64_t)nwords, ncand, d_nxt);
-        const int32_t *src = d_nxt;
-        int32_t *bufs[2] = {d_jmp, d_jtmp};
-        for (int r = 0; r < kJumpLog; ++r) {
-            hipLaunchKernelGGL(k_dec_jump, grid, 256, 0, s, src, ncand, bufs[r & 1]);
-            src = bufs[r & 1];
-        }
-        hipLaunchKernelGGL(k_dec_anchors, 1, 1, 0, s, d_in, (int64_t)cap_values, ncand, src, d_anchor, d_meta);
-        hipLaunchKernelGGL(k_dec_fill, (unsigned)((nanch_max + 255) / 256), 256, 0, s, d_in, (int64_t)nwords,
-                           (const int32_t *)d_nxt, (const int32_t *)d_anchor, ncand, d_grp, d_meta);
-        G5_HIP(hipGetLastError());
-    }
+    G5_HIP(fill_i64(d_nw + 1, 1, (int64_t)nwords, s));
+    rc = search_launch(d_in, d_nw + 1, (int64_t)nwords, (int64_t)cap_values, s);
+    if (rc) return rc;
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_STREAM, "s4 decode: malformed stream (code %lld)",
                (long long)h_meta[kMetaErr]);
-    const int64_t L = h_meta[kMetaL], nblk = L / 128;
-    if (nblk > 0) {
-        hipLaunchKernelGGL(k_dec_local, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk,
-                           (const int64_t *)d_grp, d_off, d_carry);
-        G5_HIP(hipGetLastError());
-        G5_HIP(carry_scan(d_carry, nblk, d_carry_scan, s));
-        hipLaunchKernelGGL(k_dec_write, (unsigned)((nblk + 3) / 4), 256, 0, s, d_in, nblk,
-                           (const int64_t *)d_grp, (const int64_t *)d_off, (const uint32_t *)d_carry, d_out);
-        G5_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_dec_tail, 1, 1, 0, s, d_in, (int64_t)nwords, L, (int64_t)cap_values, d_out, d_meta);
-    G5_HIP(hipGetLastError());
+    const int64_t L = h_meta[kMetaL];
+    rc = unpack_launch(d_in, d_nw + 1, L, (int64_t)cap_values, d_out, s);
+    if (rc) return rc;
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_OVERFLOW, "s4 decode: output capacity %zu exceeded", cap_values);
     *n = (size_t)(L + h_meta[kMetaTail]);
+    return G5_OK;
+}
+
+int S4Codec::roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, size_t cap_words, uint32_t *d_out,
+                       int64_t *d_flag, hipStream_t s)
+{
+    G5_REQUIRE(n >= 0 && cap_words >= g5_s4_max_words((size_t)n), G5_E_OVERFLOW,
+               "s4 round trip: word capacity %zu below the bound for %lld values", cap_words, (long long)n);
+    int rc = reserve(n);
+    if (rc) return rc;
+    rc = encode_launch(d_in, n, d_words_out, s);
+    if (rc) return rc;
+    // the stream's length stays on the device; sizes the host needs follow from n
+    const int64_t L = n / 128 * 128;
+    rc = search_launch(d_words_out, d_nw, (int64_t)g5_s4_max_words((size_t)n), n, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dec_expect, 1, 1, 0, s, d_meta, L);
+    rc = unpack_launch(d_words_out, d_nw, L, n, d_out, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dec_verify, 1, 1, 0, s, (const int64_t *)d_meta, n, d_flag);
+    G5_HIP(hipGetLastError());
     return G5_OK;
 }
 
