@@ -112,6 +112,7 @@ def lib():
     _sig(L, "g5_s4_max_words", c_sz, [c_sz])
     _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
+    _sig(L, "g5_s4_roundtrip", I, [P, c_sz, P, c_sz, P, ctypes.POINTER(c_sz)])
     _sig(L, "g5_vreduce_or", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_min_pred", I, [P, I, c_i64, P])
     _lib = L
@@ -561,3 +562,16 @@ def bitmap_count(d_bits, n_words):
     c = c_i64()
     check(lib().g5_bitmap_count(d_bits.ptr, n_words, ctypes.byref(c)))
     return c.value
+
+
+def s4_roundtrip(x):
+    """Host uint32 array -> encoded and decoded on the GPU in one call;
+    returns (decoded values, stream length in words)."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    din = DeviceArray.from_host(x)
+    cap = s4_max_words(x.size)
+    dw = DeviceArray(cap, np.uint32)
+    dout = DeviceArray(max(x.size, 1), np.uint32)
+    nw = c_sz()
+    check(lib().g5_s4_roundtrip(din.ptr, x.size, dw.ptr, cap, dout.ptr, ctypes.byref(nw)))
+    return dout.to_host(x.size), nw.value

@@ -178,7 +178,8 @@ __global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in
 
 // ------------------------------------------------------------------ decode
 
-enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 4, kMetaAnchors = 5 };
+enum { kMetaL = 0, kMetaGroups = 1, kMetaTailPos = 2, kMetaErr = 3, kMetaTail = 4, kMetaAnchors = 5, kMetaFlag = 6,
+       kMetaNw = 7 };
 
 constexpr int kWalkChunk = 8192;
 
@@ -244,8 +245,10 @@ __global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ 
                                                    int64_t *__restrict__ off, uint32_t *__restrict__ carry,
                                                    const int64_t *__restrict__ meta)
 {
+    // on an error, or past the blocks the stream's header names, there are no
+    // group positions to read
     const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= nblk || meta[kMetaErr]) return;  // no group positions to read on an error
+    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
     const int l = threadIdx.x & 63;
     const int64_t g = i >> 4;
     const int64_t hp = grp[g];
@@ -333,10 +336,14 @@ __global__ __launch_bounds__(256) void k_carry_apply(uint32_t *__restrict__ c, i
     if (i < n) store_carry(c, i, ex);
 }
 
-// ---- parallel group-header search (streams of more than kSerialWords words) ----
-// The choice needs only the word count, so the host reads nothing of the
-// stream before launching; the header's checks run on the device.
+// ---- parallel group-header search ----
+// g5_s4_decode takes the serial walk for streams of at most kSerialWords
+// words (the word count is all the host knows before launching; the
+// header's checks run on the device); an encode -> decode round trip knows
+// the group count from the value count and walks at most kSerialGroups
+// groups serially.
 constexpr int64_t kSerialWords = 4096;
+constexpr int64_t kSerialGroups = 64;
 constexpr int kJumpLog = 5;  // anchors every 2^5 groups
 constexpr int32_t kBadWidth = -1, kTruncated = -2;
 
@@ -454,18 +461,6 @@ __global__ void k_dec_fill(const uint32_t *__restrict__ in, const int64_t *__res
     }
 }
 
-// round trip: the decoded stream must hold the L values the encoder wrote
-__global__ void k_dec_expect(int64_t *meta, int64_t L)
-{
-    if (!meta[kMetaErr] && meta[kMetaL] != L) meta[kMetaErr] = 6;
-}
-
-// round trip: *flag := 1 unless exactly n values came back without error
-__global__ void k_dec_verify(const int64_t *meta, int64_t n, int64_t *flag)
-{
-    if (meta[kMetaErr] || meta[kMetaL] + meta[kMetaTail] != n) *flag = 1;
-}
-
 int64_t carry_scratch_records(int64_t n)
 {
     int64_t w = 0;
@@ -498,9 +493,8 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ 
                                                    const uint32_t *__restrict__ carry,
                                                    uint32_t *__restrict__ out, const int64_t *__restrict__ meta)
 {
-    if (meta[kMetaErr]) return;
     const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= nblk) return;
+    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
     const int l = threadIdx.x & 63;
     const int64_t g = i >> 4, ii = i - 16 * g;
     const uint32_t b = (in[grp[g] + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
@@ -538,11 +532,18 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ 
 }
 
 // varint tail, one thread; meta[kMetaTail] = values, or kMetaErr on overflow
+// flag (round trip; may be null): set to 1 unless the stream held exactly
+// n_expect values, L of them in miniblocks
 __global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp, int64_t L, int64_t cap,
-                           uint32_t *__restrict__ out, int64_t *__restrict__ meta)
+                           uint32_t *__restrict__ out, int64_t *__restrict__ meta, int64_t n_expect,
+                           int64_t *__restrict__ flag)
 {
-    if (meta[kMetaErr]) return;  // the group positions are not valid
     const int64_t nwords = *nwp;
+    meta[kMetaNw] = nwords;
+    if (meta[kMetaErr]) {  // the group positions are not valid
+        if (flag) *flag = 1;
+        return;
+    }
     const int64_t tp = meta[kMetaTailPos];
     const uint8_t *bi = (const uint8_t *)(in + tp);
     const int64_t nb = (nwords - tp) * 4;
@@ -557,6 +558,7 @@ __global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__res
             prev += v;
             if (o >= cap) {
                 meta[kMetaErr] = 5;
+                if (flag) *flag = 1;
                 return;
             }
             out[o++] = prev;
@@ -565,6 +567,7 @@ __global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__res
         }
     }
     meta[kMetaTail] = o - L;
+    if (flag && (meta[kMetaL] != L || o != n_expect)) *flag = 1;
 }
 
 }  // namespace
@@ -652,11 +655,11 @@ int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap
 // and at most words_bound: meta gets L, the groups, their positions in grp
 // and the tail position, or an error
 int S4Codec::search_launch(const uint32_t *d_in, const int64_t *nwp, int64_t words_bound, int64_t cap_values,
-                           hipStream_t s)
+                           bool serial, hipStream_t s)
 {
     G5_HIP(fill_i64(d_meta, 16, 0, s));
     const int64_t ncand = (words_bound - 4) / 4 + 1;  // bound on the 4-word group starts
-    if (words_bound <= kSerialWords || ncand > ncand_cap) {
+    if (serial || ncand > ncand_cap) {
         hipLaunchKernelGGL(k_dec_walk, 1, 256, 0, s, d_in, nwp, cap_values, d_grp, d_meta);
         G5_HIP(hipGetLastError());
         return G5_OK;
@@ -681,7 +684,7 @@ int S4Codec::search_launch(const uint32_t *d_in, const int64_t *nwp, int64_t wor
 // miniblocks and the varint tail of a stream of L values whose header search
 // has run (every kernel does nothing when meta holds an error)
 int S4Codec::unpack_launch(const uint32_t *d_in, const int64_t *nwp, int64_t L, int64_t cap_values, uint32_t *d_out,
-                           hipStream_t s)
+                           int64_t *d_flag, hipStream_t s)
 {
     const int64_t nblk = L / 128;
     if (nblk > 0) {
@@ -693,7 +696,7 @@ int S4Codec::unpack_launch(const uint32_t *d_in, const int64_t *nwp, int64_t L, 
                            (const int64_t *)d_off, (const uint32_t *)d_carry, d_out, (const int64_t *)d_meta);
         G5_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_dec_tail, 1, 1, 0, s, d_in, nwp, L, cap_values, d_out, d_meta);
+    hipLaunchKernelGGL(k_dec_tail, 1, 1, 0, s, d_in, nwp, L, cap_values, d_out, d_meta, cap_values, d_flag);
     G5_HIP(hipGetLastError());
     return G5_OK;
 }
@@ -705,14 +708,14 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
     int rc = reserve((int64_t)cap_values);
     if (rc) return rc;
     G5_HIP(fill_i64(d_nw + 1, 1, (int64_t)nwords, s));
-    rc = search_launch(d_in, d_nw + 1, (int64_t)nwords, (int64_t)cap_values, s);
+    rc = search_launch(d_in, d_nw + 1, (int64_t)nwords, (int64_t)cap_values, (int64_t)nwords <= kSerialWords, s);
     if (rc) return rc;
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_STREAM, "s4 decode: malformed stream (code %lld)",
                (long long)h_meta[kMetaErr]);
     const int64_t L = h_meta[kMetaL];
-    rc = unpack_launch(d_in, d_nw + 1, L, (int64_t)cap_values, d_out, s);
+    rc = unpack_launch(d_in, d_nw + 1, L, (int64_t)cap_values, d_out, nullptr, s);
     if (rc) return rc;
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
@@ -732,14 +735,11 @@ int S4Codec::roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, s
     if (rc) return rc;
     // the stream's length stays on the device; sizes the host needs follow from n
     const int64_t L = n / 128 * 128;
-    rc = search_launch(d_words_out, d_nw, (int64_t)g5_s4_max_words((size_t)n), n, s);
+    const int64_t ngroups = (L / 128 + 15) / 16;
+    rc = search_launch(d_words_out, d_nw, (int64_t)g5_s4_max_words((size_t)n), n, ngroups <= kSerialGroups, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dec_expect, 1, 1, 0, s, d_meta, L);
-    rc = unpack_launch(d_words_out, d_nw, L, n, d_out, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_dec_verify, 1, 1, 0, s, (const int64_t *)d_meta, n, d_flag);
-    G5_HIP(hipGetLastError());
-    return G5_OK;
+    // cap n: the tail kernel's flag check is against exactly n values
+    return unpack_launch(d_words_out, d_nw, L, n, d_out, d_flag ? d_flag : d_meta + kMetaFlag, s);
 }
 
 }  // namespace g5
@@ -769,4 +769,20 @@ extern "C" int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out
 {
     G5_REQUIRE(d_in && (d_out || out_cap == 0) && n, G5_E_ARG, "g5_s4_decode: null argument");
     return abi_codec().decode(d_in, nwords, d_out, out_cap, n, nullptr);
+}
+
+extern "C" int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t cap_words, uint32_t *d_out,
+                               size_t *nwords)
+{
+    G5_REQUIRE(d_in && d_words && d_out && nwords, G5_E_ARG, "g5_s4_roundtrip: null argument");
+    g5::S4Codec &c = abi_codec();
+    int rc = c.reserve((int64_t)n);
+    if (rc) return rc;
+    // the flag lives in the decoder's record, which the header search clears
+    rc = c.roundtrip(d_in, (int64_t)n, d_words, cap_words, d_out, nullptr, nullptr);
+    if (rc) return rc;
+    G5_HIP(hipMemcpy(c.h_meta, c.d_meta, 8 * 8, hipMemcpyDeviceToHost));
+    G5_REQUIRE(c.h_meta[kMetaFlag] == 0, G5_E_STREAM, "g5_s4_roundtrip: the decoded stream lost values");
+    *nwords = (size_t)c.h_meta[kMetaNw];
+    return G5_OK;
 }

@@ -19,7 +19,7 @@ struct S4Codec {
     int32_t *d_anchor = nullptr;   // decode: every 32nd group start
     int64_t ncand_cap = 0;
     int64_t *d_meta = nullptr;     // small device record
-    int64_t *d_nw = nullptr;       // stream word counts: [0] the last encode's, [1] decode's input
+    int64_t *d_nw = nullptr;       // [0] the last encode's stream length, [1] decode's input length
     int64_t *h_meta = nullptr;     // pinned mirror
 
     int reserve(int64_t n_values);
@@ -33,16 +33,17 @@ struct S4Codec {
     // encode d_in[n] into d_words_out, then decode it into d_out (may alias
     // d_in), with no host synchronisation: *d_flag (device) is set to 1 if
     // the round trip did not give back exactly n values, and left alone
-    // otherwise.
+    // otherwise (d_flag null: the flag slot of the decoder's record, which
+    // the round trip clears first).
     int roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, size_t cap_words, uint32_t *d_out,
                   int64_t *d_flag, hipStream_t s);
 
 private:
     int encode_launch(const uint32_t *d_in, int64_t n, uint32_t *d_out, hipStream_t s);
     int search_launch(const uint32_t *d_in, const int64_t *nwp, int64_t words_bound, int64_t cap_values,
-                      hipStream_t s);
+                      bool serial, hipStream_t s);
     int unpack_launch(const uint32_t *d_in, const int64_t *nwp, int64_t L, int64_t cap_values, uint32_t *d_out,
-                      hipStream_t s);
+                      int64_t *d_flag, hipStream_t s);
 };
 
 }  // namespace g5

@@ -187,10 +187,9 @@ void Grid2D::exchange_device(GridStats &st, int r0, int c0)
             const double tc = now();
             int64_t n = 0;
             check(g5_bitmap_to_queue(src, w, d_q_, &n), "g5_bitmap_to_queue");
-            size_t words = 0, back = 0;
-            check(g5_s4_encode(d_q_, (size_t)n, d_words_, g5_s4_max_words((size_t)n), &words), "g5_s4_encode");
-            check(g5_s4_decode(d_words_, words, d_q2_, (size_t)n, &back), "g5_s4_decode");
-            if ((int64_t)back != n) throw std::runtime_error("Grid2D: codec round trip lost values");
+            size_t words = 0;
+            check(g5_s4_roundtrip(d_q_, (size_t)n, d_words_, g5_s4_max_words((size_t)n), d_q2_, &words),
+                  "g5_s4_roundtrip");
             check(g5_queue_to_bitmap(d_q2_, n, d_fold_, w), "g5_queue_to_bitmap");
             src = d_fold_;
             st.raw_words += n;

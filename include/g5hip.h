@@ -259,6 +259,12 @@ int g5_s4_encode(const uint32_t *d_in, size_t n, uint32_t *d_out, size_t out_cap
                  size_t *nwords);
 /* d_in words[nwords] -> d_out uint32 (capacity out_cap); *n = values decoded. */
 int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t out_cap, size_t *n);
+/* The exchange's codec hop in one call (compress at the sender, decompress at
+ * the receiver): d_in uint32[n] -> d_words (stream, *nwords long) -> d_out
+ * uint32[n] (may alias d_in), with one host synchronisation at the end;
+ * G5_E_STREAM unless exactly the n values came back. */
+int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t cap_words, uint32_t *d_out,
+                    size_t *nwords);
 
 /* ---- vreduce combines (vreduce.hpp:41-544, globalbfs.hh:235-524) ----
  * d_maps: k contiguous bitmaps of n_words uint64 each -> d_out = OR. */

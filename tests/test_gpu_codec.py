@@ -134,3 +134,13 @@ def test_serial_parallel_switch():
         w = g5.s4_encode(x)
         assert np.array_equal(w, O.s4_encode(x))
         assert np.array_equal(g5.s4_decode(w, n), x)
+
+
+@pytest.mark.parametrize("n", [0, 3, 128, 5000, 300001])
+def test_roundtrip_in_one_call(n):
+    """g5_s4_roundtrip gives back the values with the reference stream's length."""
+    rng = np.random.default_rng(n + 1)
+    for x in (np.sort(rng.choice(1 << 26, size=n, replace=False)).astype(np.uint32),
+              rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)):
+        back, nw = g5.s4_roundtrip(x)
+        assert np.array_equal(back, x) and nw == len(O.s4_encode(x))
