@@ -24,9 +24,10 @@
 // n * 8 bytes.  Every row keeps its neighbours in ascending ORIGINAL id
 // order and parents are recorded as original ids, so the first hit of a
 // scan is still the minimum-id parent.  The degree bins are contiguous id
-// ranges: one thread per vertex up to 32 neighbours, one wavefront (64
-// neighbours per step, first hits from an exclusive prefix-OR across lanes)
-// up to 2048, one 256-thread workgroup beyond.  Each pull level's launch
+// ranges: one thread per vertex up to thread_max neighbours, one wavefront
+// (64 neighbours per step, first hits from an exclusive prefix-OR across
+// lanes) up to wave_max, one 256-thread workgroup beyond (Tune; 32 and 2048
+// by default).  Each pull level's launch
 // sequence is captured once into a hipGraph per frontier parity and
 // replayed; the level number lives in device memory.
 //
@@ -193,7 +194,7 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
     return __lane_id() == 0 ? 0ull : prev;
 }
 
-// One thread per vertex (degree <= 32): a sequential ascending scan.  Each
+// One thread per vertex (degree <= thread_max): a sequential ascending scan.  Each
 // thread keeps its first kHits hits (parent id, new lanes) in registers; once
 // the wave's 64 vertices are scanned, the wave writes them out vertex by
 // vertex, lane r storing root r's parent, so every store instruction covers
@@ -303,8 +304,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
 }
 
-// One wavefront per vertex (degrees 33..2048): a step loads 64 neighbours at
-// once; the wave's exclusive prefix-OR gives each root's first hit, the
+// One wavefront per vertex (thread_max < degree <= wave_max): a step loads
+// 64 neighbours at once; the wave's exclusive prefix-OR gives each root's first hit, the
 // inclusive OR the new bits.  First-hit lanes drop the parent id into the
 // wave's 64-slot LDS row (slot r = root r); when the vertex is done lane r
 // stores slot r, so the vertex's parents leave in one contiguous store.
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
 }
 
-// One workgroup per vertex (degree > kMediumMax): kBlockUnroll sub-rounds of
+// One workgroup per vertex (degree > wave_max): kBlockUnroll sub-rounds of
 // 256 neighbours per pair of barriers.
 constexpr int kBlockUnroll = 1;
 
