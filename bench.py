@@ -333,7 +333,8 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     kst = m.stats()
     if args.levels:
         for i, rec in enumerate(m.levels()):
-            log(f"level {i + 1} {'push' if rec[9] else 'pull'}: ms " + "/".join(f"{x:.2f}" for x in rec[:4]) +
+            kind = {0: "pull", 1: "push", 2: "split"}[int(rec[9])]
+            log(f"level {i + 1} {kind}: ms " + "/".join(f"{x:.2f}" for x in rec[:4]) +
                 " edges " + "/".join(f"{x:.2e}" for x in rec[4:8]) +
                 f" new {rec[8]:.3e} frontier {rec[10]:.3e} v {rec[11]:.3e} e")
     m.profile(False)

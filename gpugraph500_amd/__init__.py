@@ -379,7 +379,7 @@ class MSBFS:
         check(lib().g5_msbfs_pred_device(self.h, lane, out.ptr))
 
     # ---- shape parameters ----
-    TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3}
+    TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3, "lane_ratio": 4}
 
     def tune(self, **params):
         """Set shape parameters; the bin bounds are applied in the order that

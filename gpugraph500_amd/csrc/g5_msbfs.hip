@@ -17,6 +17,18 @@
 // parent.  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
 // an empty frontier word.
 //
+// Split levels: a pull scan stops only when all of the vertex's unvisited
+// lanes are found, so one lane whose frontier is still small (a root of low
+// degree, a root that joined late) makes most vertices scan their whole
+// neighbour list.  Pull levels therefore work only on the lanes whose
+// frontier is non-empty, and when the previous level counted each lane's
+// frontier edges, the lanes with fewer than nnz / lane_ratio frontier edges
+// are pushed instead in the same level: the pull kernels run on the other
+// lanes, then the push kernels, restricted to the pushed lanes, OR their
+// bits into next and set their parents with atomicMin.  Lanes are disjoint,
+// so each (vertex, root) pair gets the same level and minimum-id parent as
+// before.
+//
 // Relabelling: the search runs on a copy of the block whose vertices are
 // renumbered by descending degree (ties by id).  Frontier words of the hubs,
 // which most neighbour lists point at, then sit in a small id range that the
@@ -63,6 +75,8 @@ struct Tune {
     int64_t push_ratio = 16;   // push when the frontier's edges < nnz / push_ratio
     int64_t active_ratio = 2;  // pull levels consult the active bitmap when the
                                // frontier's edges < nnz / active_ratio
+    int64_t lane_ratio = 64;   // a pull level pushes the lanes whose frontier
+                               // edges < nnz / lane_ratio (0: never)
 };
 
 // Level counters.  Per accounting unit u (the kBins pull bins, then
@@ -71,13 +85,14 @@ struct Tune {
 // the next frontier's vertices and edges; then the OR of the next frontier's
 // words (the lanes still searching).  Then set-once slots: the level number,
 // the newly reached queue length (push), the frontier queue length and its
-// edge total (push).
+// edge total (push), and the lanes the pull kernels work on (written by the
+// host before each pull level).
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
-    kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3,
+    kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
     kCntStride = 32
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -85,7 +100,11 @@ enum MsCnt {
 // kStripes (stripes 256 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kAccum + 4 <= kCntStride, "counter stripe too small");
+static_assert(kPullLanes < kCntStride, "counter stripe too small");
+// Per-lane frontier edges (counted where a push or split level builds the
+// next frontier's active bitmap): kLaneStripes stripes of 64 slots, summed
+// on the host.
+constexpr int kLaneStripes = 8;
 
 // The degree-ordered copy of the block.
 struct Relabeled {
@@ -108,6 +127,7 @@ struct g5_msbfs {
     int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
     int32_t *d_parent = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
+    int64_t *d_lane = nullptr, *h_lane = nullptr;  // [kLaneStripes][64] per-lane frontier edges
     int64_t *d_scratch = nullptr;
     int64_t *d_roots = nullptr;
     Relabeled rl;
@@ -128,6 +148,13 @@ struct g5_msbfs {
     // frontier's active vertices / edges (edges < 0: unknown, take push)
     int p = 0;
     int64_t act_verts = 0, act_edges = -1;
+    // the lanes with a non-empty frontier, and whether h_lane holds each
+    // lane's frontier edges for the frontier the next level reads
+    uint64_t front_lanes = 0;
+    bool lane_edges_valid = false;
+    int64_t lane_edges[64] = {};
+    hipEvent_t ek_push[2] = {};  // split levels (profiled): around the pushed lanes' kernels
+    int64_t split_levels = 0;
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
     uint64_t busy = 0;
@@ -205,7 +232,8 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // gathers are in flight per thread.
 // kActive (all pull kernels): skip neighbours whose bit in the active bitmap
 // is clear before gathering their frontier word (pays only when few sources
-// are active).
+// are active).  All pull kernels work on the lanes in cnt[kPullLanes] only:
+// a vertex is done when every unvisited lane among them is found.
 constexpr int kHits = 4;
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4 (scale 26)
 
@@ -217,6 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t *__restrict__ cnt_all, int bin)
 {
     const int lane = __lane_id();
+    const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     // wave-uniform loop: the wave takes 64 consecutive vertices per step
     for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < nlist;
@@ -233,8 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
         }
         if (i < nlist) {
             const uint64_t vis = visited[v];
-            if (vis != ~0ull) {
-                const uint64_t unv = ~vis;
+            const uint64_t unv = ~vis & lanes;
+            if (unv) {
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
                 int nh = 0;
@@ -323,16 +352,17 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     __shared__ int32_t s_par[kBlock / 64][64];
     int32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
     const int lane = __lane_id();
+    const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += stride) {
         const uint32_t v = (uint32_t)(first_v + i);
         const uint64_t vis = visited[v];
-        if (vis == ~0ull) {
+        const uint64_t unv = ~vis & lanes;
+        if (!unv) {
             if (lane == 0) next[v] = 0;
             continue;
         }
-        const uint64_t unv = ~vis;
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
@@ -408,18 +438,19 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     __shared__ int32_t s_par[64];
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     int64_t nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         const uint32_t v = (uint32_t)(first_v + i);
         if (threadIdx.x == 0) s_vis = visited[v];
         __syncthreads();
         const uint64_t vis = s_vis;
-        if (vis == ~0ull) {
+        const uint64_t unv = ~vis & lanes;
+        if (!unv) {
             if (threadIdx.x == 0) next[v] = 0;
             __syncthreads();
             continue;
         }
-        const uint64_t unv = ~vis;
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
@@ -499,7 +530,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 // newly reached pair is set to INT32_MAX, and a second pass over the same
 // edges lowers it with atomicMin.  Work is balanced by edges: each block tile takes
 // 2048 consecutive edges of the frontier and finds each edge's source by
-// binary search in the frontier's degree prefix.
+// binary search in the frontier's degree prefix.  The push kernels work on
+// the lanes in lm (all lanes on push levels, the pushed lanes of a split
+// level).
 constexpr int kPushTile = 2048;
 constexpr int kPushGrid = 4096;
 constexpr int kPushVertGrid = 2048;
@@ -514,14 +547,21 @@ __device__ __forceinline__ int64_t fq_src(const int64_t *__restrict__ qoff, int6
     return lo;
 }
 
+// qoff[i] = degree of queue entry i; entries [queue length, bound) := 0, so
+// that a scan over `bound` entries (a host-side upper bound of the queue's
+// length) ends with the queue's edge total
 __global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__restrict__ cnt,
-                             const int64_t *__restrict__ rowptr, int64_t *__restrict__ qoff)
+                             const int64_t *__restrict__ rowptr, int64_t *__restrict__ qoff, int64_t bound)
 {
     const int64_t n = cnt[kFqLen];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bound;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = fq[i];
-        qoff[i] = rowptr[u + 1] - rowptr[u];
+        if (i < n) {
+            const uint32_t u = fq[i];
+            qoff[i] = rowptr[u + 1] - rowptr[u];
+        } else {
+            qoff[i] = 0;
+        }
     }
 }
 
@@ -530,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt)
+    int64_t *__restrict__ cnt, uint64_t lm)
 {
     __shared__ int64_t s_lo, s_hi;
     const int64_t qn = cnt[kFqLen], E = cnt[kFqEdges];
@@ -546,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
             const int64_t si = fq_src(qoff, lo, hi, e);
             const uint32_t u = fq[si];
             const uint32_t v = cols[rowptr[u] + (e - qoff[si])];
-            const uint64_t f = frontier[u] & ~visited[v];
+            const uint64_t f = frontier[u] & lm & ~visited[v];
             if (!kMin) {
                 // no return value needed: the newly reached queue is built
                 // from next's active bitmap afterwards
@@ -568,13 +608,14 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
         atomicAdd((unsigned long long *)&cnt[kUnitSlots * kPushUnit + kEdgesRead], (unsigned long long)E);
 }
 
-// Newly reached pairs' parent slots := INT32_MAX before the atomicMin pass.
+// Newly reached pairs' parent slots (lanes lm) := INT32_MAX before the
+// atomicMin pass.
 // A wave loads 64 queue entries at once, then stores each vertex's 256 B
 // parent record with one masked store (lane = root).
 __global__ __launch_bounds__(kBlock) void k_push_init_parents(const uint32_t *__restrict__ nq,
                                                               const uint64_t *__restrict__ next,
                                                               int32_t *__restrict__ parent,
-                                                              const int64_t *__restrict__ cnt)
+                                                              const int64_t *__restrict__ cnt, uint64_t lm)
 {
     const int64_t n = cnt[kNq];
     const int lane = __lane_id();
@@ -582,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_push_init_parents(const uint32_t *__
          base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + lane;
         const uint32_t v = i < n ? nq[i] : 0u;
-        const uint64_t w = i < n ? next[v] : 0ull;
+        const uint64_t w = i < n ? next[v] & lm : 0ull;
         uint64_t todo = __ballot(w != 0);
         while (todo) {
             const int src = __ffsll((long long)todo) - 1;
@@ -593,15 +634,17 @@ __global__ __launch_bounds__(kBlock) void k_push_init_parents(const uint32_t *__
 }
 
 // visited |= next over the newly reached queue, and the push unit's counts
+// (lanes lm only)
 __global__ __launch_bounds__(kBlock) void k_push_commit(const uint32_t *__restrict__ nq,
                                                         const uint64_t *__restrict__ next,
-                                                        uint64_t *__restrict__ visited, int64_t *__restrict__ cnt)
+                                                        uint64_t *__restrict__ visited, int64_t *__restrict__ cnt,
+                                                        uint64_t lm)
 {
     const int64_t n = cnt[kNq];
     int64_t nbits = 0, nv = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const uint32_t v = nq[i];
-        const uint64_t w = next[v];
+        const uint64_t w = next[v] & lm;
         visited[v] |= w;
         nbits += __popcll(w);
         ++nv;
@@ -610,55 +653,101 @@ __global__ __launch_bounds__(kBlock) void k_push_commit(const uint32_t *__restri
 }
 
 // one workgroup of kBlock threads: clear every stripe's accumulated slots
-__global__ void k_level_begin(int64_t *cnt)
+__global__ void k_level_begin(int64_t *cnt, int64_t *lane_edges)
 {
     for (int i = threadIdx.x; i < kStripes * kCntStride; i += blockDim.x)
         if (i % kCntStride < kAccum) cnt[i] = 0;
+    for (int i = threadIdx.x; i < kLaneStripes * 64; i += blockDim.x) lane_edges[i] = 0;
     if (threadIdx.x == 0) cnt[kNq] = 0;
 }
 
-// active[w] bit i = (next[64w + i] != 0).  A wave takes four 64-vertex words
-// per step (four independent loads in flight per lane).  kCount: also add
-// the active vertices and their edges to the push unit's next-frontier
-// slots (pull levels count them in the pull kernels).
-template <bool kCount>
+// active[w] bit i = ((next[64w + i] & lm) != 0).  A wave takes four
+// 64-vertex words per step (four independent loads in flight per lane).
+// kCount: also add the active vertices and their edges to the push unit's
+// next-frontier slots (pull levels count them in the pull kernels) and, when
+// lane_edges is given, each lane's edges (the degrees of the vertices whose
+// word has that lane's bit) to lane_edges[stripe][lane].  kOrLive: OR the
+// words into the level's live-lanes slot (kLive).
+// The per-lane edges only steer the split of the next level, so beyond the
+// first kLaneExactWords words (the highest-degree vertices) they are
+// estimated from every kLaneSample-th step of four words, weighted by
+// kLaneSample; the exact count costs a few instructions per frontier vertex.
+constexpr int64_t kLaneExactWords = 1024;
+constexpr int kLaneSample = 16;
+template <bool kCount, bool kOrLive>
 __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restrict__ next, int64_t nwords,
                                                          const int64_t *__restrict__ rowptr,
-                                                         uint64_t *__restrict__ active, int64_t *__restrict__ cnt)
+                                                         uint64_t *__restrict__ active, int64_t *__restrict__ cnt,
+                                                         uint64_t lm, int64_t *__restrict__ lane_edges)
 {
     __shared__ uint64_t s_live[kBlock / 64];
+    __shared__ int64_t s_lane[kBlock / 64][64];
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-    int64_t nv = 0, ne = 0;
+    int64_t nv = 0, ne = 0, le = 0;
     uint64_t live = 0;
     for (int64_t w0 = (blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6)) * 4; w0 < nwords; w0 += waves * 4) {
         uint64_t x[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = w0 + k < nwords ? next[(w0 + k) * 64 + lane] : 0ull;
+        for (int k = 0; k < 4; ++k) x[k] = w0 + k < nwords ? next[(w0 + k) * 64 + lane] & lm : 0ull;
         live |= x[0] | x[1] | x[2] | x[3];
         uint64_t mine = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint64_t bits = __ballot(x[k] != 0);
             if (lane == k) mine = bits;
-            if (kCount && x[k]) {
-                const int64_t v = (w0 + k) * 64 + lane;
-                ++nv;
-                ne += rowptr[v + 1] - rowptr[v];
+            if (kCount) {
+                uint32_t d = 0;
+                if (x[k]) {
+                    const int64_t v = (w0 + k) * 64 + lane;
+                    d = (uint32_t)(rowptr[v + 1] - rowptr[v]);
+                    ++nv;
+                    ne += d;
+                }
+                const bool exact = w0 < kLaneExactWords;
+                if (lane_edges && (exact || ((w0 >> 2) & (kLaneSample - 1)) == 0)) {
+                    // lane r adds the degrees of this word's vertices whose
+                    // word holds bit r (wave-uniform loop over the set words)
+                    const int64_t wt = exact ? 1 : kLaneSample;
+                    uint64_t todo = bits;
+                    while (todo) {
+                        const int src = __ffsll((long long)todo) - 1;
+                        todo &= todo - 1;
+                        const uint64_t xs = lane_read64(x[k], src);
+                        if (xs >> lane & 1) le += wt * lane_read(d, src);
+                    }
+                }
             }
         }
         if (lane < 4 && w0 + lane < nwords) active[w0 + lane] = mine;
     }
-    live = g5_wave_or(live);
-    if (lane == 0) s_live[threadIdx.x >> 6] = live;
+    if (kOrLive) {
+        live = g5_wave_or(live);
+        if (lane == 0) s_live[threadIdx.x >> 6] = live;
+    }
+    if (kCount && lane_edges) s_lane[threadIdx.x >> 6][lane] = le;
     if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});  // its barrier orders s_live
     else __syncthreads();
-    if (threadIdx.x == 0) {
+    if (kOrLive && threadIdx.x == 0) {
         uint64_t t = 0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
         if (t) atomicOr((unsigned long long *)&cnt[(blockIdx.x % kStripes) * kCntStride + kLive], (unsigned long long)t);
     }
+    if (kCount && lane_edges && threadIdx.x < 64) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += s_lane[w][threadIdx.x];
+        if (t)
+            atomicAdd((unsigned long long *)&lane_edges[(blockIdx.x % kLaneStripes) * 64 + threadIdx.x],
+                      (unsigned long long)t);
+    }
+}
+
+// One thread: the lanes the next pull kernels work on.
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes)
+{
+    cnt[kPullLanes] = (int64_t)lanes;
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -688,17 +777,21 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
 }
 
 // Roots: one thread; lanes of duplicate roots share the vertex.
+// A root without neighbours is reached and done: it never enters the
+// frontier, so no frontier word of a vertex outside the degree bins (which
+// the pull kernels never rewrite) is ever non-zero.
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
-                           uint64_t *__restrict__ visited, uint64_t *__restrict__ frontier,
-                           uint64_t *__restrict__ active, int32_t *__restrict__ parent,
-                           int64_t *__restrict__ cnt)
+                           const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
+                           uint64_t *__restrict__ frontier, uint64_t *__restrict__ active,
+                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt)
 {
     for (int r = 0; r < nroots; ++r) {
         const int64_t v = old2new[roots[r]];
         visited[v] |= 1ull << r;
+        parent[v * 64 + r] = (int32_t)roots[r];
+        if (rowptr[v + 1] == rowptr[v]) continue;
         frontier[v] |= 1ull << r;
         active[v >> 6] |= 1ull << (v & 63);
-        parent[v * 64 + r] = (int32_t)roots[r];
     }
     cnt[kDepth] = 1;
 }
@@ -957,19 +1050,17 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
     }
 }
 
-// act: pull kernels consult the active bitmap; prof: events between the
-// three pull kernels (the uncaptured path)
-int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof = false)
+// The three pull kernels of parity p (act: they consult the active bitmap;
+// prof: events between them).  They work on the lanes in cnt[kPullLanes].
+void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
 {
     const Relabeled &R = m->rl;
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
-    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
-    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt);
     for (int k = 0; k < kBins; ++k) {
-        if (prof) G5_HIP(hipEventRecord(m->ek[k], s));
+        if (prof) (void)hipEventRecord(m->ek[k], s);
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.new2old, \
@@ -983,10 +1074,66 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
         hipLaunchKernelGGL(kern, grid, kBlock, 0, s, G5_PULL_ARGS);
 #undef G5_PULL_ARGS
     }
-    if (prof) G5_HIP(hipEventRecord(m->ek[kBins], s));
-    hipLaunchKernelGGL(k_build_active<false>, g5::grid_for(nw, 16, 4096), kBlock, 0, s, (const uint64_t *)next, nw,
-                       (const int64_t *)R.rowptr, act_next, m->d_cnt);
+    if (prof) (void)hipEventRecord(m->ek[kBins], s);
+}
+
+// One pull level of parity p (the launches a hipGraph captures).
+int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof = false)
+{
+    uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
+    const int64_t nw = m->n / 64;
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
+    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
+    pull_bins(m, s, p, act_check, prof);
+    hipLaunchKernelGGL((k_build_active<false, true>), g5::grid_for(nw, 16, 4096), kBlock, 0, s,
+                       (const uint64_t *)next, nw, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull,
+                       (int64_t *)nullptr);
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
+    return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
+}
+
+// The push kernels for the frontier lanes lm of parity p, after next has been
+// cleared (push level) or written by the pull kernels for the other lanes
+// (split level).  `act_front` holds the frontier's active bitmap restricted
+// to lm and fq_bound bounds its vertex count.  Ends with next's active
+// bitmap, the next frontier's counts and each lane's frontier edges.
+int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int64_t fq_bound, uint64_t lm)
+{
+    const Relabeled &R = m->rl;
+    const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
+    uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
+    const int64_t nw = m->n / 64;
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
+    const unsigned agrid = g5::grid_for(nw, 16, 4096);
+    G5_HIP(g5::bitmap_to_queue(act_front, nw, m->d_fq, m->d_cnt + kFqLen, m->d_scratch, s));
+    hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fq_bound, 256, 8192), 256, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff, fq_bound);
+    G5_HIP(g5::exclusive_scan_i64(m->d_qoff, fq_bound, m->d_cnt + kFqEdges, m->d_scratch, s));
+    hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       m->d_cnt, lm);
+    // the newly reached vertices of the lanes lm; with every lane pushed that
+    // is the whole next frontier, whose counts are taken on the way
+    if (lm == ~0ull)
+        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull, m->d_lane);
+    else
+        hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, lm, (int64_t *)nullptr);
+    G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
+    hipLaunchKernelGGL(k_push_init_parents, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
+                       (const uint64_t *)next, m->d_parent, (const int64_t *)m->d_cnt, lm);
+    hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       m->d_cnt, lm);
+    hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
+                       (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
+    // the whole next frontier: active bitmap, counts, live lanes, lane edges
+    if (lm != ~0ull)
+        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull, m->d_lane);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
 
@@ -994,35 +1141,38 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
 // sizes the degree scan on the host).  fqn = active vertices of the frontier.
 int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
 {
-    const Relabeled &R = m->rl;
-    const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
-    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
-    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
     G5_HIP(g5::fill_u64(next, m->n, 0, s));
     if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
-    G5_HIP(g5::bitmap_to_queue(act, nw, m->d_fq, m->d_cnt + kFqLen, m->d_scratch, s));
-    hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fqn, 256, 8192), 256, 0, s, (const uint32_t *)m->d_fq,
-                       (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff);
-    G5_HIP(g5::exclusive_scan_i64(m->d_qoff, fqn, m->d_cnt + kFqEdges, m->d_scratch, s));
-    hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
-                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt);
-    hipLaunchKernelGGL(k_build_active<true>, g5::grid_for(nw, 16, 4096), kBlock, 0, s, (const uint64_t *)next,
-                       nw, (const int64_t *)R.rowptr, act_next, m->d_cnt);
-    G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
-    hipLaunchKernelGGL(k_push_init_parents, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
-                       (const uint64_t *)next, m->d_parent, (const int64_t *)m->d_cnt);
-    hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
-                       (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt);
-    hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
-                       (const uint64_t *)next, m->d_visited, m->d_cnt);
+    int rc = push_lanes(m, s, p, act, fqn, ~0ull);
+    if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
+    G5_HIP(hipGetLastError());
+    return G5_OK;
+}
+
+// One split level on parity p: the pull kernels on the lanes in
+// cnt[kPullLanes], then the push kernels on the lanes pm (disjoint from
+// them).  fqn = active vertices of the whole frontier.
+int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, int64_t fqn, bool prof)
+{
+    const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
+    const int64_t nw = m->n / 64;
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
+    hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
+    pull_bins(m, s, p, act_check, prof);
+    if (prof) G5_HIP(hipEventRecord(m->ek_push[0], s));
+    // the frontier vertices with a pushed lane, in act_next until push_lanes
+    // rebuilds it
+    hipLaunchKernelGGL((k_build_active<false, false>), g5::grid_for(nw, 16, 4096), kBlock, 0, s, front, nw,
+                       (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, pm, (int64_t *)nullptr);
+    int rc = push_lanes(m, s, p, act_next, fqn, pm);
+    if (rc) return rc;
+    if (prof) G5_HIP(hipEventRecord(m->ek_push[1], s));
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
     G5_HIP(hipGetLastError());
     return G5_OK;
@@ -1121,9 +1271,29 @@ void destroy_graphs(g5_msbfs *m)
             }
 }
 
-// One level at parity m->p: push or pull by the frontier's edges, then the
-// level's counters on the host; sets *new_bits (new (vertex, root) pairs)
-// and *live (OR of the next frontier), and moves the frontier state on.
+// The lanes a pull level pushes instead: those of the frontier with fewer
+// than nnz / lane_ratio frontier edges, provided they are not all of the
+// frontier's lanes and their edges together are few enough for a push level.
+// 0: no split (no per-lane counts for this frontier, or nothing to gain).
+uint64_t split_lanes(const g5_msbfs *m)
+{
+    const int64_t nnz = m->g->nnz;
+    if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->lane_edges_valid) return 0;
+    uint64_t pm = 0;
+    int64_t pe = 0;
+    for (int r = 0; r < 64; ++r)
+        if ((m->front_lanes >> r & 1) && m->lane_edges[r] * m->tune.lane_ratio < nnz) {
+            pm |= 1ull << r;
+            pe += m->lane_edges[r];
+        }
+    if (pm == m->front_lanes || pe * m->tune.push_ratio >= nnz) return 0;
+    return pm;
+}
+
+// One level at parity m->p: push or pull by the frontier's edges (a pull
+// level may push some lanes, split_lanes), then the level's counters on the
+// host; sets *new_bits (new (vertex, root) pairs) and *live (OR of the next
+// frontier), and moves the frontier state on.
 int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
 {
     hipStream_t s = m->s;
@@ -1132,39 +1302,59 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     const int64_t nnz = m->g->nnz;
     const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * m->tune.push_ratio < nnz));
     const bool act_check = act_edges < 0 || act_edges * m->tune.active_ratio < nnz;
+    const uint64_t pm = push ? 0 : split_lanes(m);
     if (push) {
         int rc = push_level(m, s, p, act_verts, m->prof);
         if (rc) return rc;
-    } else if (m->prof) {
-        int rc = level_launches(m, s, p, act_check, true);
-        if (rc) return rc;
     } else {
-        G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
+        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm);
+        if (pm) {
+            int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
+            if (rc) return rc;
+        } else if (m->prof) {
+            int rc = level_launches(m, s, p, act_check, true);
+            if (rc) return rc;
+        } else {
+            G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
+        }
     }
+    const bool lane_counts = push || pm;
     G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kCntStride * 8, hipMemcpyDeviceToHost, s));
+    if (lane_counts) G5_HIP(hipMemcpyAsync(m->h_lane, m->d_lane, kLaneStripes * 64 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     int64_t new_bits = 0;
     const int64_t *h = m->h_cnt;
     int64_t next_verts = 0, next_edges = 0;
     for (int c = 0; c < kUnits; ++c) {
         new_bits += h[kUnitSlots * c + kNewBits];
+        // a split level counts the whole next frontier in the push unit
+        if (pm && c != kPushUnit) continue;
         next_verts += h[kUnitSlots * c + kNextVerts];
         next_edges += h[kUnitSlots * c + kNextEdges];
     }
+    m->lane_edges_valid = lane_counts;
+    if (lane_counts)
+        for (int r = 0; r < 64; ++r) {
+            int64_t t = 0;
+            for (int st = 0; st < kLaneStripes; ++st) t += m->h_lane[st * 64 + r];
+            m->lane_edges[r] = t;
+        }
+    if (pm) ++m->split_levels;
     if (m->prof) {
         double rec[kLevelRec] = {0};
         rec[8] = (double)new_bits;
-        rec[9] = push ? 1.0 : 0.0;
+        rec[9] = push ? 1.0 : pm ? 2.0 : 0.0;
         rec[10] = (double)act_verts;
         rec[11] = (double)act_edges;
-        if (push) {
+        if (push || pm) {
             // 16 B row offsets per frontier vertex; per frontier edge and
             // pass (two passes) 4 B column + 8 B frontier word + 8 B visited
             // word + 8 B next word; 16 B visited read-modify-write per newly
             // reached vertex; 4 B parent init + 4 B parent per new (vertex,
             // root) pair
             float kms = 0;
-            G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
+            if (pm) G5_HIP(hipEventElapsedTime(&kms, m->ek_push[0], m->ek_push[1]));
+            else G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
             const double E = (double)h[kFqEdges];
             m->k_ms[kPushUnit] += kms;
             m->k_launches[kPushUnit] += 1;
@@ -1172,7 +1362,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
                                      8.0 * h[kUnitSlots * kPushUnit + kNewBits];
             rec[kPushUnit] = kms;
             rec[4 + kPushUnit] = E;
-        } else {
+        }
+        if (!push) {
             // per bin: 16 B row offsets + 8 B visited read + 8 B next write
             // per vertex with an unvisited lane; 4 B column (+ 1 bit of the
             // active bitmap when checked) per edge read; 8 B per frontier
@@ -1194,6 +1385,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     }
     *new_bits_out = new_bits;
     *live_out = (uint64_t)h[kLive];
+    m->front_lanes = (uint64_t)h[kLive];
     m->act_verts = next_verts;
     m->act_edges = next_edges;
     m->p ^= 1;
@@ -1228,6 +1420,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kStripes * kCntStride * 8);
     if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kCntStride * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m->d_lane, kLaneStripes * 64 * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&m->h_lane, kLaneStripes * 64 * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_join, 132 * 8);
@@ -1235,6 +1429,7 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     for (int k = 0; k <= kBins && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&m->ek_push[k]);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_msbfs_create allocation", __FILE__, __LINE__));
     int rc = build_relabel(m);
     if (rc) return fail(rc);
@@ -1255,14 +1450,18 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
             if (ge) (void)hipGraphExecDestroy(ge);
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
-                   m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg};
+                   m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
+                   m->d_lane};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
+    if (m->h_lane) (void)hipHostFree(m->h_lane);
     hipEvent_t evs[] = {m->ev0, m->ev1};
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : m->ek)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : m->ek_push)
         if (ev) (void)hipEventDestroy(ev);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
@@ -1288,7 +1487,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
                        m->d_act, m->n, unused);
     hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
-                       m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
+                       (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
     G5_HIP(hipGetLastError());
     int levels = 1;
     m->lvl.clear();
@@ -1296,6 +1495,9 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     // known on the host, and a handful of roots always takes the push path
     m->p = 0;
     m->act_edges = -1;
+    m->front_lanes = nroots == 64 ? ~0ull : (1ull << nroots) - 1;
+    m->lane_edges_valid = false;
+    m->split_levels = 0;
     {
         int64_t tmp[64];
         std::copy(roots, roots + nroots, tmp);
@@ -1333,6 +1535,9 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->p = 0;
     m->act_verts = 0;
     m->act_edges = 0;
+    m->front_lanes = 0;
+    m->lane_edges_valid = false;
+    m->split_levels = 0;
     m->lvl.clear();
     return G5_OK;
 }
@@ -1367,6 +1572,9 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
     m->busy |= mask;
     m->act_verts += added[0];
     m->act_edges += added[1];
+    // the joined lanes' frontier is their root: not in the per-lane counts
+    m->front_lanes |= mask;
+    m->lane_edges_valid = false;
     return G5_OK;
 }
 
@@ -1410,6 +1618,8 @@ extern "C" int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint6
             if (nedge) nedge[r] = (int64_t)(h[64 + r] / 2);
         }
     m->busy &= ~lanes;
+    // a freed lane is visited everywhere: no pull or push finds anything on it
+    m->front_lanes &= ~lanes;
     return G5_OK;
 }
 
@@ -1554,6 +1764,7 @@ int64_t *tune_slot(g5_msbfs *m, int key)
     case G5_TUNE_WAVE_MAX: return &m->tune.wave_max;
     case G5_TUNE_PUSH_RATIO: return &m->tune.push_ratio;
     case G5_TUNE_ACTIVE_RATIO: return &m->tune.active_ratio;
+    case G5_TUNE_LANE_RATIO: return &m->tune.lane_ratio;
     default: return nullptr;
     }
 }
@@ -1583,11 +1794,14 @@ extern "C" int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value)
     case G5_TUNE_THREAD_MAX: t.thread_max = value; break;
     case G5_TUNE_WAVE_MAX: t.wave_max = value; break;
     case G5_TUNE_PUSH_RATIO: t.push_ratio = value; break;
-    default: t.active_ratio = value; break;
+    case G5_TUNE_ACTIVE_RATIO: t.active_ratio = value; break;
+    default: t.lane_ratio = value; break;
     }
     G5_REQUIRE(t.thread_max >= 0 && t.wave_max >= t.thread_max && t.wave_max <= (int64_t)1 << 40, G5_E_ARG,
                "g5_msbfs_tune: need 0 <= thread_max <= wave_max");
     G5_REQUIRE(t.push_ratio >= 1 && t.active_ratio >= 1, G5_E_ARG, "g5_msbfs_tune: ratios must be >= 1");
+    G5_REQUIRE(t.lane_ratio >= 0 && t.lane_ratio <= (int64_t)1 << 40, G5_E_ARG,
+               "g5_msbfs_tune: lane_ratio must be >= 0");
     return apply_tune(m, t);
 }
 
@@ -1606,10 +1820,12 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     G5_REQUIRE(m && roots && nroots >= 1 && nroots <= 64 && passes >= 1 && passes <= 4, G5_E_ARG,
                "g5_msbfs_autotune: bad arguments");
     G5_REQUIRE(!(m->cont && m->busy), G5_E_STATE, "g5_msbfs_autotune: lanes in use");
-    static const int64_t cand[4][6] = {{8, 16, 32, 64, 128, 0},
+    // candidates per key; lane_ratio -1 stands for 0 (never split)
+    static const int64_t cand[5][6] = {{8, 16, 32, 64, 128, 0},
                                        {512, 1024, 2048, 4096, 8192, 16384},
                                        {4, 8, 16, 32, 64, 0},
-                                       {1, 2, 4, 8, 0, 0}};
+                                       {1, 2, 4, 8, 0, 0},
+                                       {-1, 16, 32, 64, 128, 256}};
     const bool prof = m->prof;
     m->prof = false;
     int n_trials = 0;
@@ -1633,13 +1849,16 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     double best_t = 0;
     int rc = measure(best, &best_t);
     for (int pass = 0; pass < passes && !rc; ++pass)
-        for (int key = 0; key < 4 && !rc; ++key)
+        for (int key = 0; key < 5 && !rc; ++key)
             for (int i = 0; i < 6 && !rc; ++i) {
-                const int64_t v = cand[key][i];
-                if (v == 0) continue;
+                if (cand[key][i] == 0) continue;
+                const int64_t v = cand[key][i] < 0 ? 0 : cand[key][i];
                 Tune t = best;
-                int64_t *f = key == 0 ? &t.thread_max : key == 1 ? &t.wave_max : key == 2 ? &t.push_ratio
-                                                                                          : &t.active_ratio;
+                int64_t *f = key == 0   ? &t.thread_max
+                             : key == 1 ? &t.wave_max
+                             : key == 2 ? &t.push_ratio
+                             : key == 3 ? &t.active_ratio
+                                        : &t.lane_ratio;
                 if (*f == v) continue;
                 *f = v;
                 if (t.wave_max < t.thread_max) continue;

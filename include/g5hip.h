@@ -207,14 +207,16 @@ int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
 int g5_msbfs_set_mode(g5_msbfs *m, int mode);
 /* Per-kernel accounting.  Units 0..2 are the pull kernels of the degree bins
  * (0: workgroup per vertex, degree > 2048; 1: wavefront per vertex,
- * 33..2048; 2: thread per vertex, 1..32), unit 3 the push levels.  Enabling runs pull levels as direct launches timed with
+ * 33..2048; 2: thread per vertex, 1..32), unit 3 the push levels and the
+ * pushed lanes of split levels.  Enabling runs pull levels as direct launches timed with
  * events (otherwise each pull level replays a captured hipGraph); bytes are
  * the algorithmic bytes of the unit's launches. */
 int g5_msbfs_profile(g5_msbfs *m, int enable);
 int g5_msbfs_stats(g5_msbfs *m, int unit, double *ms, int64_t *launches, double *bytes);
 /* Per level of the last profiled run, 12 doubles each: ms of units 0..3 (as
  * in g5_msbfs_stats), edges read by units 0..3, new (vertex, root) pairs, 1
- * if the level pushed, the frontier's vertices and edges.  *nlev = levels recorded; at most cap are copied. */
+ * if the level pushed (2 if it pulled some lanes and pushed the others), the
+ * frontier's vertices and edges.  *nlev = levels recorded; at most cap are copied. */
 int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev);
 /* Vertex counts of the three degree bins (units 0..2 above), up to cap. */
 int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
@@ -237,11 +239,20 @@ int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
  * thread-per-vertex and wavefront-per-vertex pull kernels (default 32, 2048;
  * higher degrees take a workgroup per vertex); G5_TUNE_PUSH_RATIO pushes a
  * level whose frontier edges are below nnz / ratio (16); G5_TUNE_ACTIVE_RATIO
- * makes pull levels consult the active bitmap below nnz / ratio edges (2).
+ * makes pull levels consult the active bitmap below nnz / ratio edges (2);
+ * G5_TUNE_LANE_RATIO: a pull level that follows a push level pushes the
+ * lanes whose frontier edges are below nnz / ratio (64; 0 = never), which
+ * stops one lagging root from making every vertex scan its whole list.
  * g5_msbfs_autotune searches these by coordinate descent over candidate
  * values (passes rounds, two timed batch searches of roots per candidate),
  * keeps the fastest and reports its time and the number of candidates run. */
-enum { G5_TUNE_THREAD_MAX = 0, G5_TUNE_WAVE_MAX = 1, G5_TUNE_PUSH_RATIO = 2, G5_TUNE_ACTIVE_RATIO = 3 };
+enum {
+    G5_TUNE_THREAD_MAX = 0,
+    G5_TUNE_WAVE_MAX = 1,
+    G5_TUNE_PUSH_RATIO = 2,
+    G5_TUNE_ACTIVE_RATIO = 3,
+    G5_TUNE_LANE_RATIO = 4
+};
 int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value);
 int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value);
 int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, int passes, double *best_ms, int *trials);

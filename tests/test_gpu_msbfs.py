@@ -131,6 +131,8 @@ def _check_all_lanes(m, ref, roots):
     dict(thread_max=0, wave_max=1 << 20),        # every vertex in the wavefront kernel
     dict(thread_max=4, wave_max=64, push_ratio=1, active_ratio=1),
     dict(push_ratio=1000, active_ratio=1000),
+    dict(lane_ratio=0),                          # no split levels
+    dict(lane_ratio=4, push_ratio=2),            # split levels push most lanes
 ])
 def test_shape_parameters_do_not_change_results(shape):
     e, ref, g = kron(14)
@@ -141,6 +143,28 @@ def test_shape_parameters_do_not_change_results(shape):
     for k, v in shape.items():
         assert got[k] == v
     m.run(roots)
+    _check_all_lanes(m, ref, roots)
+
+
+@pytest.mark.parametrize("scale", [12, 14])
+def test_split_levels_match_oracle(scale):
+    # a pull level after a push level pushes the lanes with few frontier
+    # edges; every lane ratio gives the oracle's levels and parents, and some
+    # ratio splits at least one level
+    e, ref, g = kron(scale)
+    roots = O.select_roots(ref, 1 << scale)
+    m = g5.MSBFS(g)
+    m.profile(True)
+    kinds = set()
+    for ratio in (16, 64, 256, 1024):
+        m.tune(lane_ratio=ratio)
+        m.run(roots)
+        kinds |= {int(r[9]) for r in m.levels()}
+        _check_all_lanes(m, ref, roots)
+    assert 2 in kinds, kinds
+    m.profile(False)
+    m.tune(lane_ratio=64)
+    m.run(roots)           # the graph-replayed pull levels with a lane mask
     _check_all_lanes(m, ref, roots)
 
 
@@ -158,7 +182,8 @@ def test_autotune_keeps_results():
 def test_tune_rejects_bad_values():
     e, ref, g = kron(10)
     m = g5.MSBFS(g)
-    for bad in (dict(thread_max=-1), dict(thread_max=100, wave_max=50), dict(push_ratio=0), dict(active_ratio=0)):
+    for bad in (dict(thread_max=-1), dict(thread_max=100, wave_max=50), dict(push_ratio=0), dict(active_ratio=0),
+                dict(lane_ratio=-1)):
         m2 = g5.MSBFS(g)
         with pytest.raises(g5.G5Error):
             m2.tune(**bad)
