@@ -462,6 +462,12 @@ class MSBFS:
         check(lib().g5_msbfs_info(self.h, _p(out), 3))
         return out.tolist()
 
+    def split_levels(self):
+        """Levels since the last run/open that pulled some lanes and pushed the others."""
+        out = np.zeros(4, np.int64)
+        check(lib().g5_msbfs_info(self.h, _p(out), 4))
+        return int(out[3])
+
     def free(self):
         if self.h:
             lib().g5_msbfs_free(self.h)

@@ -12,9 +12,10 @@
 //
 // Sparse frontiers (fewer than nnz/16 frontier edges) are pushed instead:
 // next[v] |= frontier[u] & ~visited[v] along the frontier's edges
-// (edge-balanced tiles, atomicOr), then the new (vertex, root) parent slots
-// are lowered with atomicMin over the same edges -- the same minimum-id
-// parent.  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
+// (edge-balanced tiles, atomicOr; the edge that sets a pair's bit first
+// leaves its source in the pair's parent slot), then the new (vertex, root)
+// parent slots are lowered with atomicMin over the same edges -- the same
+// minimum-id parent.  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
 // an empty frontier word.
 //
 // Split levels: a pull scan stops only when all of the vertex's unvisited
@@ -105,6 +106,8 @@ static_assert(kPullLanes < kCntStride, "counter stripe too small");
 // next frontier's active bitmap): kLaneStripes stripes of 64 slots, summed
 // on the host.
 constexpr int kLaneStripes = 8;
+constexpr int kHostSlots = kCntStride + 64;  // stripe 0, then the folded per-lane edges
+static_assert(kStripes * kCntStride >= kHostSlots, "per-lane edges do not fit the folded stripes");
 
 // The degree-ordered copy of the block.
 struct Relabeled {
@@ -127,7 +130,7 @@ struct g5_msbfs {
     int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
     int32_t *d_parent = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
-    int64_t *d_lane = nullptr, *h_lane = nullptr;  // [kLaneStripes][64] per-lane frontier edges
+    int64_t *d_lane = nullptr;  // [kLaneStripes][64] per-lane frontier edges
     int64_t *d_scratch = nullptr;
     int64_t *d_roots = nullptr;
     Relabeled rl;
@@ -148,17 +151,20 @@ struct g5_msbfs {
     // frontier's active vertices / edges (edges < 0: unknown, take push)
     int p = 0;
     int64_t act_verts = 0, act_edges = -1;
-    // the lanes with a non-empty frontier, and whether h_lane holds each
+    // the lanes with a non-empty frontier, and whether lane_edges holds each
     // lane's frontier edges for the frontier the next level reads
     uint64_t front_lanes = 0;
     bool lane_edges_valid = false;
+    // a split may pay: the previous level pushed (a lane may still lag
+    // behind the others) or lanes joined since
+    bool split_ok = false;
     int64_t lane_edges[64] = {};
     hipEvent_t ek_push[2] = {};  // split levels (profiled): around the pushed lanes' kernels
     int64_t split_levels = 0;
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
     uint64_t busy = 0;
-    int64_t *d_join = nullptr;             // roots [0, 64), lanes [64, 128), added frontier [128, 130)
+    int64_t *d_join = nullptr;  // roots [0, 64), lanes [64, 128), added frontier [128, 130), root degrees [130, 194)
     unsigned long long *d_sums = nullptr;  // leave: checksums [0, 64), generated-degree sums [64, 128)
 };
 
@@ -252,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
          base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + lane;
         const uint32_t v = (uint32_t)(first_v + i);
-        uint64_t found = 0, spilled = 0;
+        uint64_t found = 0;
+        int nh = 0;  // hits kept in hu / hf
         uint32_t hu[kHits];
         uint64_t hf[kHits];
 #pragma unroll
@@ -266,7 +273,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
             if (unv) {
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
-                int nh = 0;
                 while (j < e && found != unv) {
                     uint32_t u[kThreadUnroll];
                     uint64_t fr[kThreadUnroll];
@@ -296,7 +302,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                             ++nh;
                         } else {
                             write_parents(parent, v, f, uo);
-                            spilled |= f;
                         }
                     }
                 }
@@ -312,22 +317,26 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
             }
             next[v] = found;
         }
-        // write-out: one vertex per step, lane = root
-        uint64_t todo = __ballot(found != 0);
+        // write-out: one vertex per step, lane = root; the vertex's kept
+        // hits are disjoint and cover its found lanes but the spilled ones
+        uint64_t todo = __ballot(nh != 0);
         while (todo) {
             const int src = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
-            const uint64_t fj = lane_read64(found, src);
-            const uint64_t sj = lane_read64(spilled, src);
-            const int64_t rec = (int64_t)lane_read(v, src) * 64 + lane;
+            const int nhj = (int)lane_read((uint32_t)nh, src);
             uint32_t p = 0;
+            bool mine = false;
 #pragma unroll
             for (int k = 0; k < kHits; ++k) {
+                if (k >= nhj) break;
                 const uint64_t fk = lane_read64(hf[k], src);
                 const uint32_t uk = lane_read(hu[k], src);
-                if (fk >> lane & 1) p = uk;
+                if (fk >> lane & 1) {
+                    p = uk;
+                    mine = true;
+                }
             }
-            if ((fj & ~sj) >> lane & 1) __builtin_nontemporal_store((int32_t)p, &parent[rec]);
+            if (mine) __builtin_nontemporal_store((int32_t)p, &parent[(int64_t)lane_read(v, src) * 64 + lane]);
         }
     }
     block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
@@ -524,11 +533,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 
 // ---- push (top-down) levels for sparse frontiers ----
 // The frontier's vertices u (active bitmap -> sorted queue) push their lane
-// bits along their edges: next[v] |= frontier[u] & ~visited[v].  The newly
-// reached vertices (next's active bitmap -> sorted queue) then get their
-// parents, the minimum frontier neighbour per (v, root): every slot of a
-// newly reached pair is set to INT32_MAX, and a second pass over the same
-// edges lowers it with atomicMin.  Work is balanced by edges: each block tile takes
+// bits along their edges: next[v] |= frontier[u] & ~visited[v].  The edge
+// whose atomicOr sets a (v, root) bit first stores its source in the pair's
+// parent slot; a second pass over the same edges lowers every newly reached
+// pair's slot to the minimum frontier neighbour with atomicMin.  Work is balanced by edges: each block tile takes
 // 2048 consecutive edges of the frontier and finds each edge's source by
 // binary search in the frontier's degree prefix.  The push kernels work on
 // the lanes in lm (all lanes on push levels, the pushed lanes of a split
@@ -565,6 +573,38 @@ __global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__r
     }
 }
 
+// one pushed edge: u (relabelled id) -> v
+template <bool kMin>
+__device__ __forceinline__ void push_edge(uint32_t u, uint32_t v, const uint32_t *__restrict__ new2old,
+                                          const uint64_t *__restrict__ visited,
+                                          const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
+                                          int32_t *__restrict__ parent, uint64_t lm)
+{
+    const uint64_t f = frontier[u] & lm & ~visited[v];
+    if (!kMin) {
+        // bits of next only appear during the level, so a plain load that
+        // already shows all of f (even a stale one) makes the atomic moot
+        if (!f || (next[v] & f) == f) return;
+        const uint64_t fresh = f & ~(uint64_t)atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
+        // the edge that sets a (v, root) bit first stores its source as the
+        // pair's first parent candidate; the kMin pass lowers it
+        if (fresh) write_parents(parent, v, fresh, new2old[u]);
+    } else if (f) {
+        uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
+        const int32_t uo = (int32_t)new2old[u];
+        while (bits) {
+            const int r = __ffsll((long long)bits) - 1;
+            int32_t *slot = &parent[(int64_t)v * 64 + r];
+            if (*slot > uo) atomicMin(slot, uo);
+            bits &= bits - 1;
+        }
+    }
+}
+
+// A tile's sources (at most kPushTile: every queue vertex has an edge) are
+// staged in LDS first -- their first edge's offset within the tile, their id
+// and their row start less their queue offset -- so that each edge finds its
+// source by a binary search in LDS rather than in the global prefix.
 template <bool kMin>
 __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
@@ -573,6 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     int64_t *__restrict__ cnt, uint64_t lm)
 {
     __shared__ int64_t s_lo, s_hi;
+    __shared__ int32_t s_off[kPushTile];
+    __shared__ uint32_t s_u[kPushTile];
+    __shared__ int64_t s_rb[kPushTile];
     const int64_t qn = cnt[kFqLen], E = cnt[kFqEdges];
     for (int64_t t0 = blockIdx.x * (int64_t)kPushTile; t0 < E; t0 += (int64_t)gridDim.x * kPushTile) {
         const int64_t t1 = min(E, t0 + (int64_t)kPushTile);
@@ -582,55 +625,39 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
         }
         __syncthreads();
         const int64_t lo = s_lo, hi = s_hi;
-        for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
-            const int64_t si = fq_src(qoff, lo, hi, e);
-            const uint32_t u = fq[si];
-            const uint32_t v = cols[rowptr[u] + (e - qoff[si])];
-            const uint64_t f = frontier[u] & lm & ~visited[v];
-            if (!kMin) {
-                // no return value needed: the newly reached queue is built
-                // from next's active bitmap afterwards
-                if (f) atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
-            } else if (f) {
-                uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
-                const int32_t uo = (int32_t)new2old[u];
-                while (bits) {
-                    const int r = __ffsll((long long)bits) - 1;
-                    int32_t *slot = &parent[(int64_t)v * 64 + r];
-                    if (*slot > uo) atomicMin(slot, uo);
-                    bits &= bits - 1;
+        const int ns = (int)min(hi - lo + 1, (int64_t)kPushTile + 1);
+        if (ns <= kPushTile) {
+            for (int j = threadIdx.x; j < ns; j += kBlock) {
+                const uint32_t u = fq[lo + j];
+                const int64_t q = qoff[lo + j];
+                s_u[j] = u;
+                s_rb[j] = rowptr[u] - q;
+                s_off[j] = (int32_t)max(q - t0, (int64_t)0);
+            }
+            __syncthreads();
+            int j = 0;  // the sources of a thread's edges ascend
+            for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
+                const int32_t rel = (int32_t)(e - t0);
+                int a = j, b = ns - 1;  // last j with s_off[j] <= rel
+                while (a < b) {
+                    const int mid = (a + b + 1) >> 1;
+                    if (s_off[mid] <= rel) a = mid;
+                    else b = mid - 1;
                 }
+                j = a;
+                push_edge<kMin>(s_u[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, lm);
+            }
+        } else {
+            for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
+                const int64_t si = fq_src(qoff, lo, hi, e);
+                const uint32_t u = fq[si];
+                push_edge<kMin>(u, cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next, parent, lm);
             }
         }
         __syncthreads();
     }
     if (!kMin && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd((unsigned long long *)&cnt[kUnitSlots * kPushUnit + kEdgesRead], (unsigned long long)E);
-}
-
-// Newly reached pairs' parent slots (lanes lm) := INT32_MAX before the
-// atomicMin pass.
-// A wave loads 64 queue entries at once, then stores each vertex's 256 B
-// parent record with one masked store (lane = root).
-__global__ __launch_bounds__(kBlock) void k_push_init_parents(const uint32_t *__restrict__ nq,
-                                                              const uint64_t *__restrict__ next,
-                                                              int32_t *__restrict__ parent,
-                                                              const int64_t *__restrict__ cnt, uint64_t lm)
-{
-    const int64_t n = cnt[kNq];
-    const int lane = __lane_id();
-    for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = base + lane;
-        const uint32_t v = i < n ? nq[i] : 0u;
-        const uint64_t w = i < n ? next[v] & lm : 0ull;
-        uint64_t todo = __ballot(w != 0);
-        while (todo) {
-            const int src = __ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            if (lane_read64(w, src) >> lane & 1) parent[(int64_t)lane_read(v, src) * 64 + lane] = 0x7FFFFFFF;
-        }
-    }
 }
 
 // visited |= next over the newly reached queue, and the push unit's counts
@@ -664,9 +691,9 @@ __global__ void k_level_begin(int64_t *cnt, int64_t *lane_edges)
 // active[w] bit i = ((next[64w + i] & lm) != 0).  A wave takes four
 // 64-vertex words per step (four independent loads in flight per lane).
 // kCount: also add the active vertices and their edges to the push unit's
-// next-frontier slots (pull levels count them in the pull kernels) and, when
+// next-frontier slots (pull levels count them in the pull kernels).  When
 // lane_edges is given, each lane's edges (the degrees of the vertices whose
-// word has that lane's bit) to lane_edges[stripe][lane].  kOrLive: OR the
+// word has that lane's bit) go to lane_edges[stripe][lane].  kOrLive: OR the
 // words into the level's live-lanes slot (kLive).
 // The per-lane edges only steer the split of the next level, so beyond the
 // first kLaneExactWords words (the highest-degree vertices) they are
@@ -696,26 +723,27 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         for (int k = 0; k < 4; ++k) {
             const uint64_t bits = __ballot(x[k] != 0);
             if (lane == k) mine = bits;
-            if (kCount) {
-                uint32_t d = 0;
-                if (x[k]) {
-                    const int64_t v = (w0 + k) * 64 + lane;
-                    d = (uint32_t)(rowptr[v + 1] - rowptr[v]);
+            const bool exact = w0 < kLaneExactWords;
+            const bool sampled = lane_edges && (exact || ((w0 >> 2) & (kLaneSample - 1)) == 0);
+            uint32_t d = 0;
+            if ((kCount || sampled) && x[k]) {
+                const int64_t v = (w0 + k) * 64 + lane;
+                d = (uint32_t)(rowptr[v + 1] - rowptr[v]);
+                if (kCount) {
                     ++nv;
                     ne += d;
                 }
-                const bool exact = w0 < kLaneExactWords;
-                if (lane_edges && (exact || ((w0 >> 2) & (kLaneSample - 1)) == 0)) {
-                    // lane r adds the degrees of this word's vertices whose
-                    // word holds bit r (wave-uniform loop over the set words)
-                    const int64_t wt = exact ? 1 : kLaneSample;
-                    uint64_t todo = bits;
-                    while (todo) {
-                        const int src = __ffsll((long long)todo) - 1;
-                        todo &= todo - 1;
-                        const uint64_t xs = lane_read64(x[k], src);
-                        if (xs >> lane & 1) le += wt * lane_read(d, src);
-                    }
+            }
+            if (sampled) {
+                // lane r adds the degrees of this word's vertices whose word
+                // holds bit r (wave-uniform loop over the set words)
+                const int64_t wt = exact ? 1 : kLaneSample;
+                uint64_t todo = bits;
+                while (todo) {
+                    const int src = __ffsll((long long)todo) - 1;
+                    todo &= todo - 1;
+                    const uint64_t xs = lane_read64(x[k], src);
+                    if (xs >> lane & 1) le += wt * lane_read(d, src);
                 }
             }
         }
@@ -725,7 +753,7 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         live = g5_wave_or(live);
         if (lane == 0) s_live[threadIdx.x >> 6] = live;
     }
-    if (kCount && lane_edges) s_lane[threadIdx.x >> 6][lane] = le;
+    if (lane_edges) s_lane[threadIdx.x >> 6][lane] = le;
     if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});  // its barrier orders s_live
     else __syncthreads();
     if (kOrLive && threadIdx.x == 0) {
@@ -734,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
         if (t) atomicOr((unsigned long long *)&cnt[(blockIdx.x % kStripes) * kCntStride + kLive], (unsigned long long)t);
     }
-    if (kCount && lane_edges && threadIdx.x < 64) {
+    if (lane_edges && threadIdx.x < 64) {
         int64_t t = 0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) t += s_lane[w][threadIdx.x];
@@ -751,7 +779,10 @@ __global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes)
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
-__global__ void k_level_end(int64_t *cnt)
+// ... and the per-lane frontier edges, summed over their stripes, into
+// slots [kCntStride, kCntStride + 64) (stripes 1 and 2, free once folded),
+// so that one copy of kHostSlots slots brings the level's counters home
+__global__ void k_level_end(int64_t *cnt, const int64_t *lane_edges)
 {
     for (int i = threadIdx.x; i < kAccum; i += blockDim.x) {
         int64_t t = 0;
@@ -762,6 +793,12 @@ __global__ void k_level_end(int64_t *cnt)
         cnt[i] = t;
     }
     if (threadIdx.x == 0) cnt[kDepth] += 1;
+    __syncthreads();
+    for (int r = threadIdx.x; r < 64; r += blockDim.x) {
+        int64_t t = 0;
+        for (int st = 0; st < kLaneStripes; ++st) t += lane_edges[st * 64 + r];
+        cnt[kCntStride + r] = t;
+    }
 }
 
 __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__ fa, uint64_t *__restrict__ fb,
@@ -955,6 +992,7 @@ __global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__
         const int64_t deg = rowptr[v + 1] - rowptr[v];
         visited[v] |= bit;
         parent[v * 64 + lane] = (int32_t)root;
+        added[2 + i] = deg;
         if (deg == 0) continue;
         if (front[v] == 0) {
             ++nv;
@@ -1087,8 +1125,8 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
     pull_bins(m, s, p, act_check, prof);
     hipLaunchKernelGGL((k_build_active<false, true>), g5::grid_for(nw, 16, 4096), kBlock, 0, s,
                        (const uint64_t *)next, nw, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull,
-                       (int64_t *)nullptr);
-    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
+                       m->d_lane);
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
 
@@ -1122,8 +1160,6 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
         hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
                            (const int64_t *)R.rowptr, act_next, m->d_cnt, lm, (int64_t *)nullptr);
     G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
-    hipLaunchKernelGGL(k_push_init_parents, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
-                       (const uint64_t *)next, m->d_parent, (const int64_t *)m->d_cnt, lm);
     hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
@@ -1150,7 +1186,7 @@ int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
     int rc = push_lanes(m, s, p, act, fqn, ~0ull);
     if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
-    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     G5_HIP(hipGetLastError());
     return G5_OK;
 }
@@ -1173,7 +1209,7 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
     int rc = push_lanes(m, s, p, act_next, fqn, pm);
     if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek_push[1], s));
-    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt);
+    hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     G5_HIP(hipGetLastError());
     return G5_OK;
 }
@@ -1274,11 +1310,13 @@ void destroy_graphs(g5_msbfs *m)
 // The lanes a pull level pushes instead: those of the frontier with fewer
 // than nnz / lane_ratio frontier edges, provided they are not all of the
 // frontier's lanes and their edges together are few enough for a push level.
-// 0: no split (no per-lane counts for this frontier, or nothing to gain).
+// Only right after a push level or a join: after a pull level every lane has
+// had a dense level, and the unvisited vertices left are few (a split there
+// has measured slower).  0: no split.
 uint64_t split_lanes(const g5_msbfs *m)
 {
     const int64_t nnz = m->g->nnz;
-    if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->lane_edges_valid) return 0;
+    if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->lane_edges_valid || !m->split_ok) return 0;
     uint64_t pm = 0;
     int64_t pe = 0;
     for (int r = 0; r < 64; ++r)
@@ -1318,9 +1356,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             G5_HIP(hipGraphLaunch(m->graph[p][act_check ? 1 : 0], s));
         }
     }
-    const bool lane_counts = push || pm;
-    G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kCntStride * 8, hipMemcpyDeviceToHost, s));
-    if (lane_counts) G5_HIP(hipMemcpyAsync(m->h_lane, m->d_lane, kLaneStripes * 64 * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipMemcpyAsync(m->h_cnt, m->d_cnt, kHostSlots * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     int64_t new_bits = 0;
     const int64_t *h = m->h_cnt;
@@ -1332,13 +1368,9 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         next_verts += h[kUnitSlots * c + kNextVerts];
         next_edges += h[kUnitSlots * c + kNextEdges];
     }
-    m->lane_edges_valid = lane_counts;
-    if (lane_counts)
-        for (int r = 0; r < 64; ++r) {
-            int64_t t = 0;
-            for (int st = 0; st < kLaneStripes; ++st) t += m->h_lane[st * 64 + r];
-            m->lane_edges[r] = t;
-        }
+    m->lane_edges_valid = true;
+    for (int r = 0; r < 64; ++r) m->lane_edges[r] = h[kCntStride + r];
+    m->split_ok = push || pm;
     if (pm) ++m->split_levels;
     if (m->prof) {
         double rec[kLevelRec] = {0};
@@ -1350,8 +1382,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             // 16 B row offsets per frontier vertex; per frontier edge and
             // pass (two passes) 4 B column + 8 B frontier word + 8 B visited
             // word + 8 B next word; 16 B visited read-modify-write per newly
-            // reached vertex; 4 B parent init + 4 B parent per new (vertex,
-            // root) pair
+            // reached vertex; 4 B first parent candidate + 4 B parent per new
+            // (vertex, root) pair
             float kms = 0;
             if (pm) G5_HIP(hipEventElapsedTime(&kms, m->ek_push[0], m->ek_push[1]));
             else G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
@@ -1419,12 +1451,11 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_qoff, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kStripes * kCntStride * 8);
-    if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kCntStride * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kHostSlots * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_lane, kLaneStripes * 64 * 8);
-    if (e == hipSuccess) e = hipHostMalloc(&m->h_lane, kLaneStripes * 64 * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
-    if (e == hipSuccess) e = hipMalloc(&m->d_join, 132 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_join, 194 * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_sums, 128 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
@@ -1455,7 +1486,6 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
-    if (m->h_lane) (void)hipHostFree(m->h_lane);
     hipEvent_t evs[] = {m->ev0, m->ev1};
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
@@ -1536,7 +1566,8 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->act_verts = 0;
     m->act_edges = 0;
     m->front_lanes = 0;
-    m->lane_edges_valid = false;
+    m->lane_edges_valid = true;  // no lane has a frontier; joins add theirs
+    std::fill(m->lane_edges, m->lane_edges + 64, 0);
     m->split_levels = 0;
     m->lvl.clear();
     return G5_OK;
@@ -1566,15 +1597,19 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
     hipLaunchKernelGGL(k_join, 1, 1, 0, s, (const int64_t *)m->d_join, n, (const uint32_t *)m->rl.old2new,
                        (const int64_t *)m->rl.rowptr, m->d_visited, front, act, m->d_parent, m->d_join + 128);
     G5_HIP(hipGetLastError());
-    int64_t added[2] = {0, 0};
-    G5_HIP(hipMemcpyAsync(added, m->d_join + 128, 16, hipMemcpyDeviceToHost, s));
+    int64_t added[66] = {};
+    G5_HIP(hipMemcpyAsync(added, m->d_join + 128, (size_t)(2 + n) * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     m->busy |= mask;
     m->act_verts += added[0];
     m->act_edges += added[1];
-    // the joined lanes' frontier is their root: not in the per-lane counts
-    m->front_lanes |= mask;
-    m->lane_edges_valid = false;
+    // a joined lane's frontier is its root alone (none if the root has no
+    // neighbours): its frontier edges are the root's degree
+    for (int i = 0; i < n; ++i) {
+        m->lane_edges[lanes[i]] = added[2 + i];
+        if (added[2 + i]) m->front_lanes |= 1ull << lanes[i];
+    }
+    m->split_ok = true;
     return G5_OK;
 }
 
@@ -1742,6 +1777,7 @@ extern "C" int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap)
 {
     G5_REQUIRE(m && bins, G5_E_ARG, "g5_msbfs_info: null argument");
     for (int k = 0; k < kBins && k < cap; ++k) bins[k] = m->rl.bin_first[k + 1] - m->rl.bin_first[k];
+    if (cap > kBins) bins[kBins] = m->split_levels;
     return G5_OK;
 }
 

@@ -166,25 +166,29 @@ int run(const Opts &o)
         }
         fclose(f);
     }
-    int64_t levels = 0;
+    int64_t levels = 0, split = 0;
     double dev_ms = 0;
     std::string per;
     for (auto &e : eng) {
         const EngineStats st = e->stats();
         levels += st.levels;
+        split += st.split_levels;
         dev_ms += st.device_ms;
         per += (per.empty() ? "" : ", ") + std::string("\"") + std::to_string(e->scale()) + "\": {\"levels\": " +
-               std::to_string(st.levels) + ", \"roots\": " + std::to_string(st.joins) + ", \"device_ms\": " +
+               std::to_string(st.levels) + ", \"split_levels\": " + std::to_string(st.split_levels) +
+               ", \"roots\": " + std::to_string(st.joins) + ", \"device_ms\": " +
                std::to_string(st.device_ms) + "}";
     }
     std::string scales;
     for (size_t i = 0; i < o.scales.size(); ++i) scales += (i ? "," : "") + std::to_string(o.scales[i]);
     printf("{\"mode\": \"%s\", \"graphs\": [%s], \"queries\": %d, \"rate_qps\": %.1f, \"lanes\": %d, \"window\": %d, "
            "\"wall_s\": %.4f, \"achieved_qps\": %.1f, \"gteps\": %.4f, \"latency_ms\": {\"p50\": %.3f, "
-           "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"device_ms\": %.3f, "
+           "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"split_levels\": %lld, "
+           "\"device_ms\": %.3f, "
            "\"mismatches\": %d, \"engines\": {%s}}\n",
            o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, wall, o.nq / wall,
-           nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100), (long long)levels, dev_ms,
+           nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100), (long long)levels,
+           (long long)split, dev_ms,
            mismatches, per.c_str());
     return mismatches > 0 ? 3 : 0;
 }

@@ -122,6 +122,9 @@ bool Engine::advance()
     uint64_t live = 0;
     float ms = 0;
     check(g5_msbfs_step(m_, &live, &ms), "g5_msbfs_step");
+    int64_t info[4] = {0, 0, 0, 0};
+    check(g5_msbfs_info(m_, info, 4), "g5_msbfs_info");
+    stats_.split_levels = info[3];
     stats_.levels += 1;
     level_ += 1;
     stats_.device_ms += ms;

@@ -45,6 +45,7 @@ struct Query {
 
 struct EngineStats {
     int64_t levels = 0, joins = 0, leaves = 0;
+    int64_t split_levels = 0;  // levels that pulled some lanes and pushed the others
     double device_ms = 0;  // sum of the levels' device time
 };
 

@@ -96,6 +96,46 @@ def test_staggered_joins_and_lane_reuse(setup):
     assert done == len(roots)
 
 
+@pytest.mark.parametrize("lane_ratio", [0, 64, 4096])
+def test_late_joiners_pushed_while_cohort_pulls(setup, lane_ratio):
+    # 48 roots start together; once they run pull levels, 8 more join: a
+    # split level pushes the late lanes while the cohort keeps pulling
+    scale, o, g, m, roots = setup
+    m.tune(lane_ratio=lane_ratio)
+    m.open()
+    first = [int(r) for r in roots[:48]]
+    late = [int(r) for r in roots[48:56]]
+    m.join(first, list(range(48)))
+    lane_root = dict(zip(range(48), first))
+    results = {}
+    steps = 0
+    while lane_root.keys() - results.keys():
+        live, _ = m.step()
+        steps += 1
+        if steps == 2:
+            m.join(late, list(range(48, 56)))
+            lane_root.update(zip(range(48, 56), late))
+            live |= sum(1 << ln for ln in range(48, 56))
+        for ln in lane_root:
+            if ln not in results and not (live >> ln) & 1:
+                results[ln] = m.result(ln)
+        assert steps < 1000
+    split = m.split_levels()
+    mask = sum(1 << ln for ln in lane_root)
+    ne, ck = m.leave(mask)
+    for ln, root in lane_root.items():
+        rp, rl, _ = O.bfs_bin(o, root)
+        pred, lev = results[ln]
+        n = min(len(rp), len(pred))
+        assert np.array_equal(pred[:n], rp[:n]) and np.array_equal(lev[:n], rl[:n]), f"lane {ln} root {root}"
+        assert int(ck[ln]) == checksum(rp) and int(ne[ln]) == O.nedge(o, rl)
+    if lane_ratio == 0:
+        assert split == 0
+    if lane_ratio == 64:
+        assert split > 0
+    m.tune(lane_ratio=64)
+
+
 def test_isolated_and_duplicate_roots(setup):
     scale, o, g, m, roots = setup
     deg = np.diff(o.rowptr)
