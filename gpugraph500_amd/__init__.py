@@ -84,6 +84,7 @@ def lib():
     _sig(L, "g5_bfs_set_inqueue", I, [P, P, c_i64])
     _sig(L, "g5_bfs_result", I, [P, P, P])
     _sig(L, "g5_bfs_run", I, [P, c_i64, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_bfs_tune", I, [P, c_i64, c_i64])
     _sig(L, "g5_bfs_nedge", I, [P, ctypes.POINTER(c_i64)])
     _sig(L, "g5_msbfs_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_msbfs_free", I, [P])
@@ -310,6 +311,12 @@ class BFS:
         flags = (self.RUN_CODEC if codec else 0) | (self.RUN_TOPDOWN if topdown else 0)
         check(lib().g5_bfs_run(self.h, root, flags, ctypes.byref(nl), ctypes.byref(ms)))
         return nl.value, ms.value
+
+    def tune(self, alpha=20, beta=24):
+        """Direction switch of run(): bottom-up once the frontier's edges exceed
+        the edges not yet checked top-down / alpha, top-down again below
+        n / beta frontier vertices."""
+        check(lib().g5_bfs_tune(self.h, alpha, beta))
 
     def profile(self, enable=True):
         check(lib().g5_bfs_profile(self.h, 1 if enable else 0))

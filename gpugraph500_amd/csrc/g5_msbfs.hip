@@ -537,11 +537,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 // whose atomicOr sets a (v, root) bit first stores its source in the pair's
 // parent slot; a second pass over the same edges lowers every newly reached
 // pair's slot to the minimum frontier neighbour with atomicMin.  Work is balanced by edges: each block tile takes
-// 2048 consecutive edges of the frontier and finds each edge's source by
+// kPushTile consecutive edges of the frontier and finds each edge's source by
 // binary search in the frontier's degree prefix.  The push kernels work on
 // the lanes in lm (all lanes on push levels, the pushed lanes of a split
 // level).
-constexpr int kPushTile = 2048;
+constexpr int kPushTile = 1024;  // 16 KB of LDS per block (2048: 32 KB, 4 waves per SIMD)
 constexpr int kPushGrid = 4096;
 constexpr int kPushVertGrid = 2048;
 
@@ -814,23 +814,26 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
 }
 
 // Roots: one thread; lanes of duplicate roots share the vertex.
-// A root without neighbours is reached and done: it never enters the
-// frontier, so no frontier word of a vertex outside the degree bins (which
-// the pull kernels never rewrite) is ever non-zero.
+// One thread per root (roots may share a vertex, hence the atomics).  A root
+// without neighbours is reached and done: it never enters the frontier, so
+// no frontier word of a vertex outside the degree bins (which the pull
+// kernels never rewrite) is ever non-zero.
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
                            const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
                            uint64_t *__restrict__ frontier, uint64_t *__restrict__ active,
                            int32_t *__restrict__ parent, int64_t *__restrict__ cnt)
 {
-    for (int r = 0; r < nroots; ++r) {
+    const int r = threadIdx.x;
+    if (r < nroots) {
         const int64_t v = old2new[roots[r]];
-        visited[v] |= 1ull << r;
+        atomicOr((unsigned long long *)&visited[v], 1ull << r);
         parent[v * 64 + r] = (int32_t)roots[r];
-        if (rowptr[v + 1] == rowptr[v]) continue;
-        frontier[v] |= 1ull << r;
-        active[v >> 6] |= 1ull << (v & 63);
+        if (rowptr[v + 1] != rowptr[v]) {
+            atomicOr((unsigned long long *)&frontier[v], 1ull << r);
+            atomicOr((unsigned long long *)&active[v >> 6], 1ull << (v & 63));
+        }
     }
-    cnt[kDepth] = 1;
+    if (r == 0) cnt[kDepth] = 1;
 }
 
 // Degree-order keys: (2^32-1 - degree) << 32 | id, so an ascending sort
@@ -1516,7 +1519,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     const uint64_t unused = nroots == 64 ? 0ull : ~((1ull << nroots) - 1);
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
                        m->d_act, m->n, unused);
-    hipLaunchKernelGGL(k_ms_roots, 1, 1, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
+    hipLaunchKernelGGL(k_ms_roots, 1, 64, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
                        (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
     G5_HIP(hipGetLastError());
     int levels = 1;

@@ -159,7 +159,7 @@ int g5_bfs_expand_bottom_up(g5_bfs *b, const g5_graph *gt, int *any_new);
  * round trip per level.  Dense levels run bottom-up (every unvisited vertex
  * stops at its first -- minimum-id -- frontier neighbour), chosen by the
  * direction switch of Beamer et al. (SC'12: bottom-up once the frontier's
- * edges exceed the unexplored edges / 14, top-down again below n / 24
+ * edges exceed the unexplored edges / 20, top-down again below n / 24
  * frontier vertices); results are identical in both directions.  flags:
  * G5_RUN_CODEC routes every level's queue through s4-bp128-d4 encode ->
  * decode on the device, as the row broadcast delivers it to non-root ranks
@@ -167,6 +167,10 @@ int g5_bfs_expand_bottom_up(g5_bfs *b, const g5_graph *gt, int *any_new);
  * *ms = device time. */
 enum { G5_RUN_CODEC = 1, G5_RUN_TOPDOWN = 2 };
 int g5_bfs_run(g5_bfs *b, int64_t root, int flags, int32_t *nlevels, float *ms);
+/* The direction switch of g5_bfs_run: bottom-up once the frontier's edges
+ * exceed the edges not yet checked top-down / alpha (20), top-down again
+ * below n / beta frontier vertices (24).  Results never depend on them. */
+int g5_bfs_tune(g5_bfs *b, int64_t alpha, int64_t beta);
 /* Per-kernel accounting of the expansion kernel across g5_bfs_run and
  * g5_bfs_expand calls:
  * enable (resets the totals), then read total device ms, launches,

@@ -62,6 +62,23 @@ def test_directions_identical(scale):
     assert ks["k_bottom_up"]["launches"] > 0 and ks["k_expand"]["launches"] > 0
 
 
+@pytest.mark.parametrize("alpha,beta", [(1, 1), (1, 1 << 30), (1 << 30, 1), (2, 4), (64, 64)])
+def test_direction_switch_parameters_do_not_change_results(alpha, beta):
+    # alpha 1 / beta 2^30: bottom-up from the second level on, never back;
+    # alpha 2^30: top-down throughout; the others switch at other levels
+    e, ref, g = kron(13)
+    roots = O.select_roots(ref, 1 << 13)[:8]
+    b = g5.BFS(g)
+    b.tune(alpha, beta)
+    for r in roots.tolist():
+        nl, _ = b.run(r)
+        pred, lev = b.result()
+        opred, olev, onl = O.bfs_bin(ref, r)
+        assert nl == onl and np.array_equal(lev, olev) and np.array_equal(pred, opred), (alpha, beta, r)
+    with pytest.raises(g5.G5Error):
+        b.tune(0, 24)
+
+
 def test_hook_path_matches_oracle_levels_and_queues():
     e, ref, g = kron(12)
     roots = O.select_roots(ref, 1 << 12)[:6]
