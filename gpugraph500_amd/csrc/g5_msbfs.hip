@@ -235,13 +235,57 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // Hits beyond kHits (rare: they need more than kHits distinct first-hit
 // neighbours) are stored directly.  The scan loads kThreadUnroll neighbours
 // and their frontier words before using any of them, so that several
-// gathers are in flight per thread.
+// gathers are in flight per thread; its first step loads only kThreadFirst,
+// because on the dense levels most vertices are done after one or two
+// neighbours and every gather past the early exit is a wasted line fill.
 // kActive (all pull kernels): skip neighbours whose bit in the active bitmap
 // is clear before gathering their frontier word (pays only when few sources
 // are active).  All pull kernels work on the lanes in cnt[kPullLanes] only:
 // a vertex is done when every unvisited lane among them is found.
 constexpr int kHits = 4;
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4 (scale 26)
+constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
+
+// One step of a thread's scan: neighbours [j, j + K) of the row ending at e.
+template <int K, bool kActive>
+__device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32_t v, uint64_t unv, uint64_t &found,
+                                                    int &nh, uint32_t (&hu)[kHits], uint64_t (&hf)[kHits],
+                                                    int64_t &ngath, const uint32_t *__restrict__ cols,
+                                                    const uint32_t *__restrict__ new2old,
+                                                    const uint64_t *__restrict__ frontier,
+                                                    const uint64_t *__restrict__ active, int32_t *__restrict__ parent)
+{
+    uint32_t u[K];
+    uint64_t fr[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool take = j + k < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
+        fr[k] = take ? frontier[u[k]] : 0ull;
+        ngath += take;
+    }
+    // in neighbour order: the first hit of a lane is its parent
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t f = fr[k] & unv & ~found;
+        if (!f) continue;
+        found |= f;
+        const uint32_t uo = new2old[u[k]];
+        if (nh < kHits) {
+#pragma unroll
+            for (int h = 0; h < kHits; ++h)
+                if (h == nh) {
+                    hu[h] = uo;
+                    hf[h] = f;
+                }
+            ++nh;
+        } else {
+            write_parents(parent, v, f, uo);
+        }
+    }
+    return j + K;
+}
 
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
@@ -273,38 +317,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
             if (unv) {
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
-                while (j < e && found != unv) {
-                    uint32_t u[kThreadUnroll];
-                    uint64_t fr[kThreadUnroll];
-#pragma unroll
-                    for (int k = 0; k < kThreadUnroll; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
-#pragma unroll
-                    for (int k = 0; k < kThreadUnroll; ++k) {
-                        const bool take = j + k < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
-                        fr[k] = take ? frontier[u[k]] : 0ull;
-                        ngath += take;
-                    }
-                    j += kThreadUnroll;
-                    // in neighbour order: the first hit of a lane is its parent
-#pragma unroll
-                    for (int k = 0; k < kThreadUnroll; ++k) {
-                        const uint64_t f = fr[k] & unv & ~found;
-                        if (!f) continue;
-                        found |= f;
-                        const uint32_t uo = new2old[u[k]];
-                        if (nh < kHits) {
-#pragma unroll
-                            for (int h = 0; h < kHits; ++h)
-                                if (h == nh) {
-                                    hu[h] = uo;
-                                    hf[h] = f;
-                                }
-                            ++nh;
-                        } else {
-                            write_parents(parent, v, f, uo);
-                        }
-                    }
-                }
+                if (j < e)
+                    j = thread_scan_step<kThreadFirst, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols, new2old,
+                                                                frontier, active, parent);
+                while (j < e && found != unv)
+                    j = thread_scan_step<kThreadUnroll, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
+                                                                 new2old, frontier, active, parent);
                 j = j < e ? j : e;
                 if (found) {
                     visited[v] = vis | found;
@@ -348,8 +366,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
 // wave's 64-slot LDS row (slot r = root r); when the vertex is done lane r
 // stores slot r, so the vertex's parents leave in one contiguous store.
 // Each iteration loads kWaveUnroll steps (64 neighbours each) and their
-// frontier words before processing the steps in order.
+// frontier words before processing the steps in order.  The first step takes
+// only kWaveFirst neighbours (the other lanes idle): on dense levels most
+// vertices are done after a few, and every gather past the early exit is a
+// wasted line fill.
 constexpr int kWaveUnroll = 1;
+constexpr int kWaveFirst = 16;  // 8: same time
+static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
 
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
@@ -375,22 +398,25 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
+        int64_t step = kWaveFirst;
         while (base < e && found != unv) {
+            const int64_t lim = min(e, base + step * kWaveUnroll);
             uint32_t u[kWaveUnroll];
             uint64_t fr[kWaveUnroll];
 #pragma unroll
             for (int k = 0; k < kWaveUnroll; ++k) {
-                const int64_t j = base + 64 * k + lane;
-                u[k] = j < e ? cols[j] : 0u;
+                const int64_t j = base + step * k + lane;
+                u[k] = j < lim ? cols[j] : 0u;
             }
 #pragma unroll
             for (int k = 0; k < kWaveUnroll; ++k) {
-                const int64_t j = base + 64 * k + lane;
-                const bool take = j < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
+                const int64_t j = base + step * k + lane;
+                const bool take = j < lim && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
                 fr[k] = take ? frontier[u[k]] : 0ull;
                 ngath += take;
             }
-            base += 64 * kWaveUnroll;
+            base += step * kWaveUnroll;
+            step = 64;
 #pragma unroll
             for (int k = 0; k < kWaveUnroll; ++k) {
                 const uint64_t fresh = fr[k] & unv & ~found;
