@@ -86,6 +86,7 @@ def lib():
     _sig(L, "g5_bfs_run", I, [P, c_i64, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_bfs_tune", I, [P, c_i64, c_i64])
     _sig(L, "g5_bfs_nedge", I, [P, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_bfs_summary", I, [P, ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_uint64)])
     _sig(L, "g5_msbfs_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_msbfs_free", I, [P])
     _sig(L, "g5_msbfs_run", I, [P, P, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
@@ -343,6 +344,12 @@ class BFS:
         n = c_i64()
         check(lib().g5_bfs_nedge(self.h, ctypes.byref(n)))
         return n.value
+
+    def summary(self):
+        """(nedge, checksum of the predecessor map) of the last run, on the device."""
+        n, ck = c_i64(), ctypes.c_uint64()
+        check(lib().g5_bfs_summary(self.h, ctypes.byref(n), ctypes.byref(ck)))
+        return n.value, ck.value
 
     def free(self):
         if self.h:

@@ -318,6 +318,29 @@ __global__ void k_nedge(const uint32_t *__restrict__ gdeg, const int32_t *__rest
     if (threadIdx.x == 0 && c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
 }
 
+// nedge's generated-degree sum and the predecessor-map checksum of a 1x1
+// search in one pass: sum over reached v of mix64(v << 32 | (pred(v) + 1))
+__global__ void k_summary(const uint32_t *__restrict__ gdeg, const int32_t *__restrict__ level,
+                          const uint32_t *__restrict__ parent, int64_t n, int64_t *__restrict__ dst)
+{
+    int64_t c = 0;
+    uint64_t h = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (level[i] >= 0) {
+            c += gdeg[i];
+            const uint32_t p = parent[i];
+            const uint64_t pred = p == kRootMark ? (uint64_t)i : (uint64_t)p;
+            h += g5_mix64(((uint64_t)i << 32) | (uint32_t)(pred + 1));
+        }
+    c = g5_block_sum256(c);
+    h = g5_block_sum256_u64(h);
+    if (threadIdx.x == 0) {
+        if (c) atomicAdd((unsigned long long *)dst, (unsigned long long)c);
+        if (h) atomicAdd((unsigned long long *)(dst + 1), (unsigned long long)h);
+    }
+}
+
 __global__ void k_set_cnt(int64_t *cnt, int idx, int64_t v) { cnt[idx] = v; }
 
 // One level's expansion from the current in-queue (d_queue, cnt[kQLen]).
@@ -700,6 +723,26 @@ extern "C" int g5_bfs_nedge(g5_bfs *b, int64_t *nedge)
     int rc = sync_counters(b);
     if (rc) return rc;
     *nedge = b->h_cnt[kOutLen] / 2;
+    return G5_OK;
+}
+
+extern "C" int g5_bfs_summary(g5_bfs *b, int64_t *nedge, uint64_t *checksum)
+{
+    G5_REQUIRE(b && nedge && checksum, G5_E_ARG, "g5_bfs_summary: null argument");
+    G5_REQUIRE(b->g->n_rows == b->g->n_cols && b->g->row_start == b->g->col_start, G5_E_STATE,
+               "g5_bfs_summary: needs a diagonal (1x1) block");
+    // kOutLen and kNewEdges as the two sums (free between levels)
+    static_assert(kNewEdges == kOutLen + 1, "the summary's two slots must be adjacent");
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kOutLen, (int64_t)0);
+    hipLaunchKernelGGL(k_set_cnt, 1, 1, 0, b->s, b->d_cnt, (int)kNewEdges, (int64_t)0);
+    hipLaunchKernelGGL(k_summary, g5::grid_for(b->g->n_rows, 256, 4096), 256, 0, b->s,
+                       (const uint32_t *)b->g->d_gdeg, (const int32_t *)b->d_level, (const uint32_t *)b->d_parent,
+                       b->g->n_rows, b->d_cnt + kOutLen);
+    G5_HIP(hipGetLastError());
+    int rc = sync_counters(b);
+    if (rc) return rc;
+    *nedge = b->h_cnt[kOutLen] / 2;
+    *checksum = (uint64_t)b->h_cnt[kNewEdges];
     return G5_OK;
 }
 

@@ -114,6 +114,32 @@ __device__ __forceinline__ int64_t g5_wave_sum(int64_t v)
     return v;
 }
 
+// The same modulo 2^64.
+__device__ __forceinline__ uint64_t g5_block_sum256_u64(uint64_t v)
+{
+    __shared__ uint64_t s_part_u[4];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (__lane_id() == 0) s_part_u[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint64_t t = s_part_u[0] + s_part_u[1] + s_part_u[2] + s_part_u[3];
+    __syncthreads();
+    return t;
+}
+
+// 64-bit finaliser of SplitMix64 (Steele, Lea, Flood 2014; constants as
+// published by Vigna): a bijective mix, used for predecessor-map checksums
+// sum over reached v of mix64(v << 32 | (pred(v) + 1)) mod 2^64.
+__device__ __forceinline__ uint64_t g5_mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
 // Sum of v over a 256-thread workgroup, returned to every thread (every
 // thread must call it).  Kernels add it to a global counter from one thread:
 // same-address atomics serialise in one L2 channel, so one per workgroup

@@ -1034,18 +1034,6 @@ __global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__
     added[1] = ne;
 }
 
-// 64-bit finaliser of SplitMix64 (Steele, Lea, Flood 2014; constants as
-// published by Vigna): a bijective mix for the predecessor-map checksum
-__device__ __forceinline__ uint64_t mix64(uint64_t x)
-{
-    x ^= x >> 30;
-    x *= 0xbf58476d1ce4e5b9ull;
-    x ^= x >> 27;
-    x *= 0x94d049bb133111ebull;
-    x ^= x >> 31;
-    return x;
-}
-
 // Lanes in `mask` leave: per lane the sum of generated degrees over its
 // reached vertices (2 x nedge) and the checksum sum over reached v of
 // mix64(orig(v) << 32 | (pred(v) + 1)) mod 2^64, in original ids; then the
@@ -1100,7 +1088,7 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
             for (int k = 0; k < kLeaveGroup; ++k) {
                 const uint32_t oj = lane_read(o, j0 + k), dj = lane_read(d, j0 + k);
                 if (rj[k] >> lane & 1) {
-                    ck += mix64(((uint64_t)oj << 32) | (uint32_t)(pj[k] + 1));
+                    ck += g5_mix64(((uint64_t)oj << 32) | (uint32_t)(pj[k] + 1));
                     dg += dj;
                 }
             }

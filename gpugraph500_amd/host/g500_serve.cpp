@@ -3,7 +3,8 @@
 // queries in real time.
 //
 //   g500_serve [-g 18,20,22,24,25,26] [-e 16] [-rate q/s] [-nq queries] [-seed s]
-//              [-maxroots k] [-lanes l] [-window w] [-alone] [-compare] [-dump file]
+//              [-maxroots k] [-lanes l] [-window w] [-small k] [-alone] [-compare]
+//              [-dump file]
 //
 //   -rate      mean arrival rate; inter-arrival times are exponential
 //   -maxroots  roots per query drawn uniformly from 1..k (default 8)
@@ -11,6 +12,9 @@
 //   -window    new roots join a running cohort only while all its lanes are
 //              within their first w levels (default 1; -1: whenever lanes
 //              are free)
+//   -small     a query of at most k roots that finds its engine idle and
+//              nothing queued behind it runs as single-root searches
+//              (default 4; 0: always the multi-lane search)
 //   -alone     every query by itself on an idle engine, one after another
 //   -compare   after the served run, run every query alone and compare each
 //              root's (nedge, checksum): results must be identical
@@ -37,7 +41,7 @@ namespace {
 
 struct Opts {
     std::vector<int> scales{18, 20, 22, 24, 25, 26};
-    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64, window = 1;
+    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64, window = 1, small = 4;
     double rate = 1000;
     uint64_t seed = 1;
     bool alone = false, compare = false;
@@ -65,6 +69,7 @@ Opts parse(int argc, char **argv)
         else if (a == "-maxroots") o.maxroots = std::stoi(next());
         else if (a == "-lanes") o.lanes = std::stoi(next());
         else if (a == "-window") o.window = std::stoi(next());
+        else if (a == "-small") o.small = std::stoi(next());
         else if (a == "-alone") o.alone = true;
         else if (a == "-compare") o.compare = true;
         else if (a == "-dump") o.dump = next();
@@ -94,7 +99,7 @@ int run(const Opts &o)
 {
     check(g5_init(0), "g5_init");
     std::vector<std::unique_ptr<Engine>> eng;
-    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window));
+    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window, o.small));
 
     // the query stream: seeded, independent of the serving run
     std::mt19937_64 rng(o.seed);
@@ -166,29 +171,32 @@ int run(const Opts &o)
         }
         fclose(f);
     }
-    int64_t levels = 0, split = 0;
+    int64_t levels = 0, split = 0, single = 0;
     double dev_ms = 0;
     std::string per;
     for (auto &e : eng) {
         const EngineStats st = e->stats();
         levels += st.levels;
         split += st.split_levels;
+        single += st.single_roots;
         dev_ms += st.device_ms;
         per += (per.empty() ? "" : ", ") + std::string("\"") + std::to_string(e->scale()) + "\": {\"levels\": " +
                std::to_string(st.levels) + ", \"split_levels\": " + std::to_string(st.split_levels) +
-               ", \"roots\": " + std::to_string(st.joins) + ", \"device_ms\": " +
+               ", \"roots\": " + std::to_string(st.joins) + ", \"single_roots\": " +
+               std::to_string(st.single_roots) + ", \"device_ms\": " +
                std::to_string(st.device_ms) + "}";
     }
     std::string scales;
     for (size_t i = 0; i < o.scales.size(); ++i) scales += (i ? "," : "") + std::to_string(o.scales[i]);
     printf("{\"mode\": \"%s\", \"graphs\": [%s], \"queries\": %d, \"rate_qps\": %.1f, \"lanes\": %d, \"window\": %d, "
+           "\"small\": %d, "
            "\"wall_s\": %.4f, \"achieved_qps\": %.1f, \"gteps\": %.4f, \"latency_ms\": {\"p50\": %.3f, "
            "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"split_levels\": %lld, "
-           "\"device_ms\": %.3f, "
+           "\"single_roots\": %lld, \"device_ms\": %.3f, "
            "\"mismatches\": %d, \"engines\": {%s}}\n",
-           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, wall, o.nq / wall,
-           nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100), (long long)levels,
-           (long long)split, dev_ms,
+           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, o.small, wall,
+           o.nq / wall, nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100),
+           (long long)levels, (long long)split, (long long)single, dev_ms,
            mismatches, per.c_str());
     return mismatches > 0 ? 3 : 0;
 }

@@ -25,7 +25,8 @@ double now_seconds()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-Engine::Engine(int scale, int edgefactor, int lanes, int window) : scale_(scale), max_lanes_(lanes), window_(window)
+Engine::Engine(int scale, int edgefactor, int lanes, int window, int small)
+    : scale_(scale), max_lanes_(lanes), window_(window), small_(small)
 {
     if (lanes < 1 || lanes > 64) throw std::invalid_argument("Engine: lanes must be in [1, 64]");
     check(g5_graph_kronecker(scale, edgefactor, 1, 2, &g_), "g5_graph_kronecker");
@@ -48,6 +49,7 @@ Engine::~Engine()
         }
     }
     g5_msbfs_free(m_);
+    g5_bfs_free(b_);
     g5_graph_free(g_);
 }
 
@@ -116,6 +118,29 @@ int Engine::admit(std::deque<Query *> &from)
     return (int)roots.size();
 }
 
+bool Engine::run_small(std::deque<Query *> &from)
+{
+    if (busy_ || small_ <= 0 || from.size() != 1 || (int)from.front()->roots.size() > small_) return false;
+    Query *q = from.front();
+    from.pop_front();
+    q->admitted = clock();
+    const size_t k = q->roots.size();
+    q->nedge.assign(k, 0);
+    q->checksum.assign(k, 0);
+    if (!b_) check(g5_bfs_create(g_, &b_), "g5_bfs_create");
+    for (size_t i = 0; i < k; ++i) {
+        int32_t nl = 0;
+        float ms = 0;
+        check(g5_bfs_run(b_, q->roots[i], 0, &nl, &ms), "g5_bfs_run");
+        check(g5_bfs_summary(b_, &q->nedge[i], &q->checksum[i]), "g5_bfs_summary");
+        stats_.device_ms += ms;
+    }
+    stats_.single_roots += (int64_t)k;
+    q->remaining = 0;
+    q->done = clock();
+    return true;
+}
+
 bool Engine::advance()
 {
     if (!busy_) return false;
@@ -172,6 +197,7 @@ void Engine::loop_body()
             }
             if (stop_ && busy_ == 0 && local.empty()) return;
         }
+        if (run_small(local)) continue;
         admit(local);  // between two levels
         advance();
     }

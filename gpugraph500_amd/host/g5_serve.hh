@@ -16,6 +16,11 @@
 // drain (window < 0: join whenever lanes are free).  A root's lane leaves as
 // soon as its own search is complete; its result is (nedge, checksum of the
 // predecessor map), and is identical to the root searched alone.
+// Small queries on an idle engine: a query of at most `small` roots that
+// finds the engine idle with nothing queued behind it is answered by the
+// direction-optimised single-root search, root after root (a few ms at
+// scale 26), instead of a multi-lane search whose pull levels sweep every
+// vertex for a handful of lanes; the results are the same.
 #pragma once
 
 #include <condition_variable>
@@ -46,6 +51,7 @@ struct Query {
 struct EngineStats {
     int64_t levels = 0, joins = 0, leaves = 0;
     int64_t split_levels = 0;  // levels that pulled some lanes and pushed the others
+    int64_t single_roots = 0;  // roots answered by the single-root search
     double device_ms = 0;  // sum of the levels' device time
 };
 
@@ -53,7 +59,7 @@ class Engine
 {
 public:
     // owns the graph (built on the device from the Kronecker generator)
-    Engine(int scale, int edgefactor, int lanes, int window = 1);
+    Engine(int scale, int edgefactor, int lanes, int window = 1, int small = 0);
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -75,10 +81,12 @@ private:
     int scale_;
     int max_lanes_;
     int window_;
+    int small_;
     int64_t level_ = 0;  // levels run by this engine
     int64_t n_ = 0;
     g5_graph *g_ = nullptr;
     g5_msbfs *m_ = nullptr;
+    g5_bfs *b_ = nullptr;  // single-root search for small queries (created on first use)
     std::vector<int64_t> keys_;
     std::thread th_;
     std::mutex mu_;
@@ -102,6 +110,9 @@ private:
     // the join window; returns the number of roots joined
     int admit(std::deque<Query *> &from);
     bool in_window() const;
+    // answers the only queued query by single-root searches if it is small
+    // and the engine idle; returns true if it did
+    bool run_small(std::deque<Query *> &from);
     // one level plus the departures it causes; returns true while lanes are in use
     bool advance();
 };

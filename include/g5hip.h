@@ -185,6 +185,12 @@ int g5_bfs_kernel_stats(g5_bfs *b, int kernel, double *ms, int64_t *launches, do
 /* TEPS numerator of the last search: generated undirected edges with a
  * reached endpoint (validate/validate.h:439-485). */
 int g5_bfs_nedge(g5_bfs *b, int64_t *nedge);
+/* After g5_bfs_run: nedge as above and the checksum of the predecessor map,
+ * sum over reached v of mix64(v << 32 | (pred(v) + 1)) mod 2^64 (mix64 =
+ * SplitMix64's finaliser), in one pass on the device -- what
+ * g5_msbfs_leave reports per lane, so a server can answer a query from
+ * either search.  Needs a diagonal (1x1) block. */
+int g5_bfs_summary(g5_bfs *b, int64_t *nedge, uint64_t *checksum);
 
 /* ---- batched multi-source BFS (no counterpart in the reference: it stands
  * beside the single-root plug-in; results equal nroots separate searches

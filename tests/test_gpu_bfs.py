@@ -7,6 +7,7 @@ import pytest
 
 import gpugraph500_amd as g5
 import oracle_lib as O
+from test_gpu_continuous import checksum
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +37,7 @@ def test_resident_all_roots_bit_exact(scale):
         assert np.array_equal(pred, opred), f"parents differ, root {r}"
         assert nl == onl
         assert b.nedge() == O.nedge(ref, olev)
+        assert b.summary() == (O.nedge(ref, olev), checksum(opred))
     bad, depth, ne = O.validate(e, ref, 1 << scale, int(roots[-1]), pred)
     assert bad == 0 and np.array_equal(depth, lev)
 

@@ -68,6 +68,22 @@ def test_alone_mode(tmp_path):
     assert rec["mode"] == "alone" and len(rows) == 20
 
 
+@pytest.mark.parametrize("small", ["2", "8"])
+def test_small_queries_on_idle_engines(tmp_path, small):
+    """A slow stream: most queries find their engine idle, and those of at
+    most `small` roots are answered by single-root searches -- with the same
+    (nedge, checksum) as the multi-lane search run alone, and the oracle's."""
+    rec, rows = _serve(tmp_path, "-g", "12", "-rate", "200", "-nq", "30", "-seed", "4", "-small", small,
+                       "-compare")
+    assert rec["mismatches"] == 0 and rec["small"] == int(small)
+    assert rec["single_roots"] > 0
+    o = O.build_csr(O.kronecker_edges(12, 16))
+    for _, _, _, _, roots in rows:
+        for root, ne, ck in roots:
+            rp, rl, _ = O.bfs_bin(o, root)
+            assert ne == O.nedge(o, rl) and ck == checksum(rp), root
+
+
 @pytest.mark.parametrize("window", ["-1", "0", "3"])
 def test_join_window_does_not_change_results(tmp_path, window):
     """Joining whenever lanes are free (-1), only into an idle engine (0),
