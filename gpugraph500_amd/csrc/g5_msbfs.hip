@@ -243,7 +243,7 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // are active).  All pull kernels work on the lanes in cnt[kPullLanes] only:
 // a vertex is done when every unvisited lane among them is found.
 constexpr int kHits = 4;
-constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4 (scale 26)
+constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4; 12 after the short first step: the same
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
 
 // One step of a thread's scan: neighbours [j, j + K) of the row ending at e.
