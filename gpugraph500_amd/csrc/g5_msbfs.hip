@@ -161,6 +161,13 @@ struct g5_msbfs {
     int64_t lane_edges[64] = {};
     hipEvent_t ek_push[2] = {};  // split levels (profiled): around the pushed lanes' kernels
     int64_t split_levels = 0;
+    // the last push level's frontier queue, which lists the only non-zero
+    // words of the buffer the next push level writes (listed_ok: the level
+    // just run was that push level, or both buffers are clear)
+    uint32_t *d_fq2 = nullptr;
+    uint32_t *listed = nullptr;
+    int64_t listed_len = 0;
+    bool listed_ok = false;
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
     uint64_t busy = 0;
@@ -567,6 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 // binary search in the frontier's degree prefix.  The push kernels work on
 // the lanes in lm (all lanes on push levels, the pushed lanes of a split
 // level).
+// a push level clears next entry by entry when the list is below n / kListedFrac
+constexpr int64_t kListedFrac = 32;
 constexpr int kPushTile = 1024;  // 16 KB of LDS per block (2048: 32 KB, 4 waves per SIMD)
 constexpr int kPushGrid = 4096;
 constexpr int kPushVertGrid = 2048;
@@ -599,22 +608,26 @@ __global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__r
     }
 }
 
-// one pushed edge: u (relabelled id) -> v
+// one pushed edge: u (relabelled id) -> v.  mark (first pass only, when
+// given): the edge that makes next[v] non-zero sets v's bit in mark, the
+// next frontier's active bitmap.
 template <bool kMin>
 __device__ __forceinline__ void push_edge(uint32_t u, uint32_t v, const uint32_t *__restrict__ new2old,
                                           const uint64_t *__restrict__ visited,
                                           const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
-                                          int32_t *__restrict__ parent, uint64_t lm)
+                                          int32_t *__restrict__ parent, uint64_t lm, uint64_t *__restrict__ mark)
 {
     const uint64_t f = frontier[u] & lm & ~visited[v];
     if (!kMin) {
         // bits of next only appear during the level, so a plain load that
         // already shows all of f (even a stale one) makes the atomic moot
         if (!f || (next[v] & f) == f) return;
-        const uint64_t fresh = f & ~(uint64_t)atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
+        const uint64_t old = (uint64_t)atomicOr((unsigned long long *)&next[v], (unsigned long long)f);
+        const uint64_t fresh = f & ~old;
         // the edge that sets a (v, root) bit first stores its source as the
         // pair's first parent candidate; the kMin pass lowers it
         if (fresh) write_parents(parent, v, fresh, new2old[u]);
+        if (mark && old == 0) atomicOr((unsigned long long *)&mark[v >> 6], 1ull << (v & 63));
     } else if (f) {
         uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
         const int32_t uo = (int32_t)new2old[u];
@@ -636,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt, uint64_t lm)
+    int64_t *__restrict__ cnt, uint64_t lm, uint64_t *__restrict__ mark)
 {
     __shared__ int64_t s_lo, s_hi;
     __shared__ int32_t s_off[kPushTile];
@@ -671,19 +684,66 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                     else b = mid - 1;
                 }
                 j = a;
-                push_edge<kMin>(s_u[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, lm);
+                push_edge<kMin>(s_u[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, lm, mark);
             }
         } else {
             for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
                 const int64_t si = fq_src(qoff, lo, hi, e);
                 const uint32_t u = fq[si];
-                push_edge<kMin>(u, cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next, parent, lm);
+                push_edge<kMin>(u, cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next, parent, lm,
+                                mark);
             }
         }
         __syncthreads();
     }
     if (!kMin && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd((unsigned long long *)&cnt[kUnitSlots * kPushUnit + kEdgesRead], (unsigned long long)E);
+}
+
+// next[list[i]] := 0 for i < len (the entries a pure push level's frontier
+// queue listed: the only non-zero words of the buffer two levels on)
+__global__ void k_zero_listed(uint64_t *__restrict__ next, const uint32_t *__restrict__ list, int64_t len)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+        next[list[i]] = 0;
+}
+
+// A pure push level's next frontier from its queue nq (cnt[kNq] entries)
+// instead of a pass over all n words: the push unit's next-frontier vertices
+// and edges, the live lanes and each lane's frontier edges (exact).
+__global__ __launch_bounds__(kBlock) void k_nq_counts(const uint32_t *__restrict__ nq,
+                                                      const uint64_t *__restrict__ next,
+                                                      const int64_t *__restrict__ rowptr, int64_t *__restrict__ cnt,
+                                                      int64_t *__restrict__ lane_edges)
+{
+    __shared__ unsigned long long s_lane[64];
+    __shared__ uint64_t s_live[kBlock / 64];
+    const int64_t n = cnt[kNq];
+    if (threadIdx.x < 64) s_lane[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t nv = 0, ne = 0;
+    uint64_t live = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const uint32_t v = nq[i];
+        const uint64_t w = next[v];
+        const int64_t d = rowptr[v + 1] - rowptr[v];
+        ++nv;
+        ne += d;
+        live |= w;
+        for (uint64_t b = w; b; b &= b - 1) atomicAdd(&s_lane[__ffsll((long long)b) - 1], (unsigned long long)d);
+    }
+    live = g5_wave_or(live);
+    if (__lane_id() == 0) s_live[threadIdx.x >> 6] = live;
+    block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});  // its barriers order s_lane, s_live
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
+        if (t) atomicOr((unsigned long long *)&cnt[(blockIdx.x % kStripes) * kCntStride + kLive], (unsigned long long)t);
+    }
+    if (threadIdx.x < 64 && s_lane[threadIdx.x])
+        atomicAdd((unsigned long long *)&lane_edges[(blockIdx.x % kLaneStripes) * 64 + threadIdx.x],
+                  s_lane[threadIdx.x]);
 }
 
 // visited |= next over the newly reached queue, and the push unit's counts
@@ -1150,9 +1210,15 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
 // The push kernels for the frontier lanes lm of parity p, after next has been
 // cleared (push level) or written by the pull kernels for the other lanes
 // (split level).  `act_front` holds the frontier's active bitmap restricted
-// to lm and fq_bound bounds its vertex count.  Ends with next's active
-// bitmap, the next frontier's counts and each lane's frontier edges.
-int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int64_t fq_bound, uint64_t lm)
+// to lm, fq_bound bounds its vertex count and fq receives its queue.  Ends
+// with next's active bitmap, the next frontier's counts and each lane's
+// frontier edges.  A push level (lm = every lane) finds its newly reached
+// vertices as it goes -- the edge that makes next[v] non-zero sets v's
+// active bit, in act_next cleared beforehand -- and counts them from their
+// queue, so it makes no pass over all n words; a split level builds the
+// bitmaps from next.
+int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int64_t fq_bound, uint64_t lm,
+               uint32_t *fq)
 {
     const Relabeled &R = m->rl;
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
@@ -1160,31 +1226,32 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     const int64_t nw = m->n / 64;
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     const unsigned agrid = g5::grid_for(nw, 16, 4096);
-    G5_HIP(g5::bitmap_to_queue(act_front, nw, m->d_fq, m->d_cnt + kFqLen, m->d_scratch, s));
-    hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fq_bound, 256, 8192), 256, 0, s, (const uint32_t *)m->d_fq,
+    const bool whole = lm == ~0ull;
+    G5_HIP(g5::bitmap_to_queue(act_front, nw, fq, m->d_cnt + kFqLen, m->d_scratch, s));
+    hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fq_bound, 256, 8192), 256, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff, fq_bound);
     G5_HIP(g5::exclusive_scan_i64(m->d_qoff, fq_bound, m->d_cnt + kFqEdges, m->d_scratch, s));
-    hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+    if (whole) G5_HIP(g5::fill_u64(act_next, nw, 0, s));
+    hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt, lm);
-    // the newly reached vertices of the lanes lm; with every lane pushed that
-    // is the whole next frontier, whose counts are taken on the way
-    if (lm == ~0ull)
-        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull, m->d_lane);
-    else
+                       m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr);
+    // the newly reached vertices of the lanes lm
+    if (!whole)
         hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
                            (const int64_t *)R.rowptr, act_next, m->d_cnt, lm, (int64_t *)nullptr);
     G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
-    hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)m->d_fq,
+    if (whole)
+        hipLaunchKernelGGL(k_nq_counts, g5::grid_for(m->n, kBlock, 2048), kBlock, 0, s, (const uint32_t *)m->d_nq,
+                           (const uint64_t *)next, (const int64_t *)R.rowptr, m->d_cnt, m->d_lane);
+    hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt, lm);
+                       m->d_cnt, lm, (uint64_t *)nullptr);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
     // the whole next frontier: active bitmap, counts, live lanes, lane edges
-    if (lm != ~0ull)
+    if (!whole)
         hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
                            (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull, m->d_lane);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
@@ -1192,16 +1259,28 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
 
 // One push level on parity p: direct launches (the frontier queue's length
 // sizes the degree scan on the host).  fqn = active vertices of the frontier.
+// next is cleared entry by entry when the level before was a push level too
+// (its frontier queue lists the only non-zero words of this level's next)
+// and the list is short; otherwise by a fill of all n words.  The two push
+// queue buffers alternate, so that list survives into this level.
 int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
 {
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
-    G5_HIP(g5::fill_u64(next, m->n, 0, s));
+    if (m->listed_ok && m->listed_len * kListedFrac < m->n) {
+        if (m->listed_len)
+            hipLaunchKernelGGL(k_zero_listed, g5::grid_for(m->listed_len, 256, 4096), 256, 0, s, next,
+                               (const uint32_t *)m->listed, m->listed_len);
+    } else {
+        G5_HIP(g5::fill_u64(next, m->n, 0, s));
+    }
     if (prof) G5_HIP(hipEventRecord(m->ek[0], s));
-    int rc = push_lanes(m, s, p, act, fqn, ~0ull);
+    uint32_t *fq = m->listed == m->d_fq ? m->d_fq2 : m->d_fq;
+    int rc = push_lanes(m, s, p, act, fqn, ~0ull, fq);
     if (rc) return rc;
+    m->listed = fq;
     if (prof) G5_HIP(hipEventRecord(m->ek[1], s));
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     G5_HIP(hipGetLastError());
@@ -1223,7 +1302,7 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
     // rebuilds it
     hipLaunchKernelGGL((k_build_active<false, false>), g5::grid_for(nw, 16, 4096), kBlock, 0, s, front, nw,
                        (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, pm, (int64_t *)nullptr);
-    int rc = push_lanes(m, s, p, act_next, fqn, pm);
+    int rc = push_lanes(m, s, p, act_next, fqn, pm, m->listed == m->d_fq ? m->d_fq2 : m->d_fq);
     if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek_push[1], s));
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
@@ -1387,6 +1466,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     }
     m->lane_edges_valid = true;
     for (int r = 0; r < 64; ++r) m->lane_edges[r] = h[kCntStride + r];
+    m->listed_ok = push;
+    m->listed_len = push ? h[kFqLen] : 0;
     m->split_ok = push || pm;
     if (pm) ++m->split_levels;
     if (m->prof) {
@@ -1464,6 +1545,7 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_fb, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_act, (size_t)(n / 64) * 2 * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_fq, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&m->d_fq2, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_nq, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_qoff, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
@@ -1499,7 +1581,7 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
-                   m->d_lane};
+                   m->d_lane,    m->d_fq2};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
@@ -1545,6 +1627,8 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     m->front_lanes = nroots == 64 ? ~0ull : (1ull << nroots) - 1;
     m->lane_edges_valid = false;
     m->split_levels = 0;
+    m->listed_ok = true;  // k_ms_init cleared both frontier buffers
+    m->listed_len = 0;
     {
         int64_t tmp[64];
         std::copy(roots, roots + nroots, tmp);
@@ -1583,6 +1667,8 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->act_verts = 0;
     m->act_edges = 0;
     m->front_lanes = 0;
+    m->listed_ok = true;  // k_ms_init cleared both frontier buffers
+    m->listed_len = 0;
     m->lane_edges_valid = true;  // no lane has a frontier; joins add theirs
     std::fill(m->lane_edges, m->lane_edges + 64, 0);
     m->split_levels = 0;
