@@ -584,9 +584,10 @@ def bitmap_count(d_bits, n_words):
     return c.value
 
 
-def s4_roundtrip(x):
+def s4_roundtrip(x, words=False):
     """Host uint32 array -> encoded and decoded on the GPU in one call;
-    returns (decoded values, stream length in words)."""
+    returns (decoded values, stream length in words), and the stream itself
+    as a third item when words is true."""
     x = np.ascontiguousarray(x, dtype=np.uint32)
     din = DeviceArray.from_host(x)
     cap = s4_max_words(x.size)
@@ -594,4 +595,6 @@ def s4_roundtrip(x):
     dout = DeviceArray(max(x.size, 1), np.uint32)
     nw = c_sz()
     check(lib().g5_s4_roundtrip(din.ptr, x.size, dw.ptr, cap, dout.ptr, ctypes.byref(nw)))
+    if words:
+        return dout.to_host(x.size), nw.value, dw.to_host(nw.value)
     return dout.to_host(x.size), nw.value

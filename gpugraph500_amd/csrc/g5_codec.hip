@@ -28,6 +28,8 @@
 // unpacks; the 4-lane running sums across miniblocks come from an
 // exclusive scan of per-miniblock (reset, sum) records, where b = 32
 // miniblocks reset the running value to their raw last four values.
+// A round trip of at most kSmallRoundTrip values runs all of these stages in
+// one workgroup (k_rt_small).
 #include "g5_codec.hpp"
 
 namespace {
@@ -67,28 +69,41 @@ __device__ __forceinline__ uint32_t width_of(uint32_t acc) { return acc ? 32u - 
 
 // ------------------------------------------------------------------ encode
 
+// LDS written by some lanes of a wave, then read by others
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 4 * b of miniblock i (one wave, every lane calls it)
+__device__ __forceinline__ int64_t enc_block_words(const uint32_t *__restrict__ in, int64_t i, int l)
+{
+    const int64_t base = i * 128;
+    const uint32_t x0 = in[base + l], x1 = in[base + l + 64];
+    const uint32_t r0 = (base + l >= 4) ? in[base + l - 4] : 0u;
+    const uint32_t r1 = in[base + l + 60];
+    const uint32_t acc = g5_wave_or32((x0 - r0) | (x1 - r1));
+    return 4 * (int64_t)width_of(acc);
+}
+
 __global__ __launch_bounds__(256) void k_enc_widths(const uint32_t *__restrict__ in, int64_t nblk,
                                                     int64_t *__restrict__ words)
 {
     const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= nblk) return;
     const int l = threadIdx.x & 63;
-    const int64_t base = i * 128;
-    const uint32_t x0 = in[base + l], x1 = in[base + l + 64];
-    const uint32_t r0 = (base + l >= 4) ? in[base + l - 4] : 0u;
-    const uint32_t r1 = in[base + l + 60];
-    const uint32_t acc = g5_wave_or32((x0 - r0) | (x1 - r1));
-    if (l == 0) words[i] = 4 * (int64_t)width_of(acc);
+    const int64_t w = enc_block_words(in, i, l);
+    if (l == 0) words[i] = w;
 }
 
-__global__ __launch_bounds__(256) void k_enc_pack(const uint32_t *__restrict__ in, int64_t nblk,
-                                                  const int64_t *__restrict__ P, uint32_t *__restrict__ out)
+// miniblock i's packed words (and its group's header if it is a group's
+// first): one wave, P = word-count prefix (global or LDS), lds = 128 words
+// of the wave's own LDS
+__device__ __forceinline__ void enc_pack_block(const uint32_t *__restrict__ in, int64_t nblk, int64_t i,
+                                               const int64_t *P, uint32_t *__restrict__ out, uint32_t *lds, int l)
 {
-    __shared__ uint32_t lds[4][128];
-    const int wv = threadIdx.x >> 6;
-    const int64_t i = blockIdx.x * 4 + wv;
-    if (i >= nblk) return;
-    const int l = threadIdx.x & 63;
     const int64_t base = i * 128;
     const int64_t g = i >> 4;
     const uint32_t b = (uint32_t)((P[i + 1] - P[i]) >> 2);
@@ -112,11 +127,9 @@ __global__ __launch_bounds__(256) void k_enc_pack(const uint32_t *__restrict__ i
     if (b == 0) return;
     const uint32_t r0 = (base + l >= 4) ? in[base + l - 4] : 0u;
     const uint32_t r1 = in[base + l + 60];
-    lds[wv][l] = x0 - r0;
-    lds[wv][l + 64] = x1 - r1;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    lds[l] = x0 - r0;
+    lds[l + 64] = x1 - r1;
+    wave_sync();
     for (uint32_t t = l; t < 4 * b; t += 64) {
         const uint32_t j = t & 3, w = t >> 2, lo = 32 * w;
         uint32_t word = 0;
@@ -125,11 +138,22 @@ __global__ __launch_bounds__(256) void k_enc_pack(const uint32_t *__restrict__ i
         if (i1 > 31) i1 = 31;
         for (uint32_t idx = i0; idx <= i1; ++idx) {
             const int sh = (int)(idx * b) - (int)lo;
-            const uint32_t v = lds[wv][4 * idx + j];
+            const uint32_t v = lds[4 * idx + j];
             word |= sh >= 0 ? (v << sh) : (v >> (-sh));
         }
         dst[t] = word;
     }
+    wave_sync();  // lds is reused by the wave's next miniblock
+}
+
+__global__ __launch_bounds__(256) void k_enc_pack(const uint32_t *__restrict__ in, int64_t nblk,
+                                                  const int64_t *__restrict__ P, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t lds[4][128];
+    const int wv = threadIdx.x >> 6;
+    const int64_t i = blockIdx.x * 4 + wv;
+    if (i >= nblk) return;
+    enc_pack_block(in, nblk, i, P, out, lds[wv], threadIdx.x & 63);
 }
 
 __device__ __forceinline__ uint32_t varint_len(uint32_t v)
@@ -137,21 +161,22 @@ __device__ __forceinline__ uint32_t varint_len(uint32_t v)
     return v < (1u << 7) ? 1u : v < (1u << 14) ? 2u : v < (1u << 21) ? 3u : v < (1u << 28) ? 4u : 5u;
 }
 
-// stream header + varint tail; one wave.  meta[0] = total words.
-__global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in, int64_t n, int64_t L,
-                                                 int64_t nblk, const int64_t *__restrict__ P,
-                                                 uint32_t *__restrict__ out, int64_t *__restrict__ meta)
+constexpr int kTailBytes = 128 * 5 + 8;
+
+// stream header + varint tail; one wave (every lane calls it), pn = the
+// miniblocks' word total, bytes = kTailBytes of LDS; returns the stream's
+// word count
+__device__ int64_t enc_tail_wave(const uint32_t *__restrict__ in, int64_t n, int64_t L, int64_t nblk, int64_t pn,
+                                 uint32_t *__restrict__ out, uint8_t *bytes, int l)
 {
-    __shared__ uint8_t bytes[128 * 5 + 8];
-    const int l = threadIdx.x;
     if (l == 0) {
         out[0] = (uint32_t)L;
         out[1] = out[2] = out[3] = kCookie;
     }
-    const int64_t tailpos = 4 + 4 * ((nblk + 15) / 16) + P[nblk];
+    const int64_t tailpos = 4 + 4 * ((nblk + 15) / 16) + pn;
     const int64_t nt = n - L;
-    for (int k = l; k < 128 * 5 + 8; k += 64) bytes[k] = 0;
-    __syncthreads();
+    for (int k = l; k < kTailBytes; k += 64) bytes[k] = 0;
+    wave_sync();
     uint32_t run = 0;
     for (int r = 0; r < 2; ++r) {
         const int k = l + 64 * r;
@@ -168,12 +193,22 @@ __global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in
         }
         run += (uint32_t)__shfl(inc, 63, 64);
     }
-    __syncthreads();
+    wave_sync();
     const uint32_t nw = (run + 3) / 4;
     for (uint32_t w = l; w < nw; w += 64)
         out[tailpos + w] = (uint32_t)bytes[4 * w] | ((uint32_t)bytes[4 * w + 1] << 8) |
                            ((uint32_t)bytes[4 * w + 2] << 16) | ((uint32_t)bytes[4 * w + 3] << 24);
-    if (l == 0) meta[0] = tailpos + nw;
+    return tailpos + nw;
+}
+
+// one wave.  meta[0] = total words.
+__global__ __launch_bounds__(64) void k_enc_tail(const uint32_t *__restrict__ in, int64_t n, int64_t L,
+                                                 int64_t nblk, const int64_t *__restrict__ P,
+                                                 uint32_t *__restrict__ out, int64_t *__restrict__ meta)
+{
+    __shared__ uint8_t bytes[kTailBytes];
+    const int64_t nw = enc_tail_wave(in, n, L, nblk, P[nblk], out, bytes, threadIdx.x);
+    if (threadIdx.x == 0) meta[0] = nw;
 }
 
 // ------------------------------------------------------------------ decode
@@ -239,19 +274,12 @@ __global__ __launch_bounds__(256) void k_dec_walk(const uint32_t *__restrict__ i
     }
 }
 
-// One wave per miniblock: data offset and the (reset, lane sums) record.
-__global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ in, int64_t nblk,
-                                                   const int64_t *__restrict__ grp,
-                                                   int64_t *__restrict__ off, uint32_t *__restrict__ carry,
-                                                   const int64_t *__restrict__ meta)
+// Miniblock i of the group whose header is at hp: its data offset (*off)
+// and its (reset, lane sums) record; one wave, every lane calls it.
+__device__ __forceinline__ Carry dec_block_carry(const uint32_t *__restrict__ in, int64_t hp, int64_t i, int l,
+                                                 int64_t *off)
 {
-    // on an error, or past the blocks the stream's header names, there are no
-    // group positions to read
-    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
-    const int l = threadIdx.x & 63;
     const int64_t g = i >> 4;
-    const int64_t hp = grp[g];
     int64_t o = hp + 4;
     for (int64_t j = 16 * g; j < i; ++j) {
         const int64_t jj = j - 16 * g;
@@ -259,7 +287,6 @@ __global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ 
     }
     const int64_t ii = i - 16 * g;
     const uint32_t b = (in[hp + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
-    if (l == 0) off[i] = o;
     Carry c;
     if (b == 32) {
         c.r = 1;
@@ -287,7 +314,27 @@ __global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ 
 #pragma unroll
         for (int j = 0; j < 4; ++j) c.v[j] = __shfl(s, j, 64);
     }
-    if (l == 0) store_carry(carry, i, c);
+    *off = o;
+    return c;
+}
+
+// One wave per miniblock: data offset and the (reset, lane sums) record.
+__global__ __launch_bounds__(256) void k_dec_local(const uint32_t *__restrict__ in, int64_t nblk,
+                                                   const int64_t *__restrict__ grp,
+                                                   int64_t *__restrict__ off, uint32_t *__restrict__ carry,
+                                                   const int64_t *__restrict__ meta)
+{
+    // on an error, or past the blocks the stream's header names, there are no
+    // group positions to read
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
+    const int l = threadIdx.x & 63;
+    int64_t o = 0;
+    const Carry c = dec_block_carry(in, grp[i >> 4], i, l, &o);
+    if (l == 0) {
+        off[i] = o;
+        store_carry(carry, i, c);
+    }
 }
 
 // --- exclusive scan of Carry records (tile of 256, recursive over tiles)
@@ -487,18 +534,13 @@ hipError_t carry_scan(uint32_t *c, int64_t n, uint32_t *scratch, hipStream_t s)
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ in, int64_t nblk,
-                                                   const int64_t *__restrict__ grp,
-                                                   const int64_t *__restrict__ off,
-                                                   const uint32_t *__restrict__ carry,
-                                                   uint32_t *__restrict__ out, const int64_t *__restrict__ meta)
+// Miniblock i's 128 values from its data at o, the group header at hp and
+// the running values c before it; one wave, every lane calls it.
+__device__ __forceinline__ void dec_write_block(const uint32_t *__restrict__ in, int64_t hp, int64_t i, int64_t o,
+                                                const Carry &c, uint32_t *__restrict__ out, int l)
 {
-    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
-    const int l = threadIdx.x & 63;
     const int64_t g = i >> 4, ii = i - 16 * g;
-    const uint32_t b = (in[grp[g] + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
-    const int64_t o = off[i];
+    const uint32_t b = (in[hp + (ii >> 2)] >> (24 - 8 * (ii & 3))) & 0xFF;
     if (b == 32) {
         out[128 * i + l] = in[o + l];
         out[128 * i + l + 64] = in[o + l + 64];
@@ -526,19 +568,28 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ 
     }
     const uint32_t j = l & 3;
     const uint32_t first_half = __shfl(d[0], 60 + j, 64);
-    const Carry c = load_carry(carry, i);
     out[128 * i + l] = c.v[j] + d[0];
     out[128 * i + l + 64] = c.v[j] + first_half + d[1];
+}
+
+__global__ __launch_bounds__(256) void k_dec_write(const uint32_t *__restrict__ in, int64_t nblk,
+                                                   const int64_t *__restrict__ grp,
+                                                   const int64_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ carry,
+                                                   uint32_t *__restrict__ out, const int64_t *__restrict__ meta)
+{
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nblk || i >= meta[kMetaL] / 128 || meta[kMetaErr]) return;
+    dec_write_block(in, grp[i >> 4], i, off[i], load_carry(carry, i), out, threadIdx.x & 63);
 }
 
 // varint tail, one thread; meta[kMetaTail] = values, or kMetaErr on overflow
 // flag (round trip; may be null): set to 1 unless the stream held exactly
 // n_expect values, L of them in miniblocks
-__global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp, int64_t L, int64_t cap,
-                           uint32_t *__restrict__ out, int64_t *__restrict__ meta, int64_t n_expect,
-                           int64_t *__restrict__ flag)
+__device__ void dec_tail_thread(const uint32_t *__restrict__ in, int64_t nwords, int64_t L, int64_t cap,
+                                uint32_t *__restrict__ out, int64_t *__restrict__ meta, int64_t n_expect,
+                                int64_t *__restrict__ flag)
 {
-    const int64_t nwords = *nwp;
     meta[kMetaNw] = nwords;
     if (meta[kMetaErr]) {  // the group positions are not valid
         if (flag) *flag = 1;
@@ -568,6 +619,113 @@ __global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__res
     }
     meta[kMetaTail] = o - L;
     if (flag && (meta[kMetaL] != L || o != n_expect)) *flag = 1;
+}
+
+__global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__restrict__ nwp, int64_t L, int64_t cap,
+                           uint32_t *__restrict__ out, int64_t *__restrict__ meta, int64_t n_expect,
+                           int64_t *__restrict__ flag)
+{
+    dec_tail_thread(in, *nwp, L, cap, out, meta, n_expect, flag);
+}
+
+// ---- round trip of a short queue in one workgroup ----
+// The same stages as the multi-kernel round trip (widths, their prefix,
+// packing, header and tail; then the header walk, per-miniblock records,
+// their scan, unpacking and the tail with the count check), for at most
+// kSmallRoundTrip values, so that a short level queue costs one launch
+// instead of a dozen.  The decode reads the words the encode wrote to
+// memory, as a receiver would.
+constexpr int64_t kSmallRoundTrip = 8192;
+constexpr int kSmallBlocks = kSmallRoundTrip / 128;  // 64 miniblocks, 4 groups
+
+__global__ __launch_bounds__(256) void k_rt_small(const uint32_t *in, int64_t n, uint32_t *__restrict__ words,
+                                                  int64_t *__restrict__ nw_out, uint32_t *out,
+                                                  int64_t *__restrict__ meta, int64_t *__restrict__ flag)
+{
+    __shared__ int64_t s_P[kSmallBlocks + 1];
+    __shared__ uint32_t s_lds[4][128];
+    __shared__ uint8_t s_bytes[kTailBytes];
+    __shared__ int64_t s_grp[kSmallBlocks / 16], s_off[kSmallBlocks];
+    __shared__ Carry s_c[kSmallBlocks];
+    __shared__ int64_t s_nw, s_err, s_tp, s_L;
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t L = n / 128 * 128, nblk = L / 128;
+    if (threadIdx.x < 16) meta[threadIdx.x] = 0;
+    // ---- encode
+    for (int64_t i = wv; i < nblk; i += 4) {
+        const int64_t w = enc_block_words(in, i, l);
+        if (l == 0) s_P[i + 1] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_P[0] = 0;
+        for (int64_t i = 0; i < nblk; ++i) s_P[i + 1] += s_P[i];
+    }
+    __syncthreads();
+    for (int64_t i = wv; i < nblk; i += 4) enc_pack_block(in, nblk, i, s_P, words, s_lds[wv], l);
+    if (wv == 0) {
+        const int64_t nw = enc_tail_wave(in, n, L, nblk, s_P[nblk], words, s_bytes, l);
+        if (l == 0) {
+            s_nw = nw;
+            *nw_out = nw;
+        }
+    }
+    __threadfence();  // the words are read back below, by other waves
+    __syncthreads();
+    // ---- decode: header checks and the group-header walk
+    const int64_t nwords = s_nw;
+    if (threadIdx.x == 0) {
+        const int64_t Ls = words[0];
+        int64_t err = 0, pos = 4;
+        if (words[1] != kCookie || words[2] != kCookie || words[3] != kCookie) err = 1;
+        if (Ls % 128 != 0 || Ls > n) err = 2;
+        const int64_t nb = err ? 0 : Ls / 128, ng = (nb + 15) / 16;
+        for (int64_t g = 0; g < ng && !err; ++g) {
+            if (pos + 4 > nwords) {
+                err = 4;
+                break;
+            }
+            const int64_t cnt = min((int64_t)16, nb - 16 * g);
+            int64_t size = 4;
+            for (int64_t j = 0; j < cnt; ++j) {
+                const uint32_t bj = (words[pos + (j >> 2)] >> (24 - 8 * (j & 3))) & 0xFF;
+                if (bj > 32) err = 3;
+                size += 4 * bj;
+            }
+            s_grp[g] = pos;
+            pos += size;
+            if (pos > nwords) err = 4;
+        }
+        s_L = err ? 0 : Ls;
+        s_err = err;
+        s_tp = pos;
+        meta[kMetaL] = Ls;
+        meta[kMetaGroups] = ng;
+        meta[kMetaTailPos] = pos;
+        meta[kMetaErr] = err;
+    }
+    __syncthreads();
+    const int64_t nbd = s_L / 128;  // 0 on an error
+    for (int64_t i = wv; i < nbd; i += 4) {
+        int64_t o = 0;
+        const Carry c = dec_block_carry(words, s_grp[i >> 4], i, l, &o);
+        if (l == 0) {
+            s_off[i] = o;
+            s_c[i] = c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan of the records
+        Carry run{0, {0, 0, 0, 0}};
+        for (int64_t i = 0; i < nbd; ++i) {
+            const Carry c = s_c[i];
+            s_c[i] = run;
+            run = comb(run, c);
+        }
+    }
+    __syncthreads();
+    for (int64_t i = wv; i < nbd; i += 4) dec_write_block(words, s_grp[i >> 4], i, s_off[i], s_c[i], out, l);
+    if (threadIdx.x == 0) dec_tail_thread(words, nwords, s_L, n, out, meta, n, flag);
 }
 
 }  // namespace
@@ -731,6 +889,12 @@ int S4Codec::roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, s
                "s4 round trip: word capacity %zu below the bound for %lld values", cap_words, (long long)n);
     int rc = reserve(n);
     if (rc) return rc;
+    if (n <= kSmallRoundTrip) {
+        hipLaunchKernelGGL(k_rt_small, 1, 256, 0, s, d_in, n, d_words_out, d_nw, d_out, d_meta,
+                           d_flag ? d_flag : d_meta + kMetaFlag);
+        G5_HIP(hipGetLastError());
+        return G5_OK;
+    }
     rc = encode_launch(d_in, n, d_words_out, s);
     if (rc) return rc;
     // the stream's length stays on the device; sizes the host needs follow from n
