@@ -22,9 +22,10 @@
 // lanes are found, so one lane whose frontier is still small (a root of low
 // degree, a root that joined late) makes most vertices scan their whole
 // neighbour list.  Pull levels therefore work only on the lanes whose
-// frontier is non-empty, and when the previous level counted each lane's
-// frontier edges, the lanes with fewer than nnz / lane_ratio frontier edges
-// are pushed instead in the same level: the pull kernels run on the other
+// frontier is non-empty, and right after a push level (which counts each
+// lane's frontier edges) or after lanes joined (whose frontier is their
+// root), the lanes with fewer than nnz / lane_ratio frontier edges are
+// pushed instead in the same level: the pull kernels run on the other
 // lanes, then the push kernels, restricted to the pushed lanes, OR their
 // bits into next and set their parents with atomicMin.  Lanes are disjoint,
 // so each (vertex, root) pair gets the same level and minimum-id parent as
@@ -102,9 +103,8 @@ enum MsCnt {
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
 static_assert(kPullLanes < kCntStride, "counter stripe too small");
-// Per-lane frontier edges (counted where a push or split level builds the
-// next frontier's active bitmap): kLaneStripes stripes of 64 slots, summed
-// on the host.
+// Per-lane frontier edges (counted by push levels, from their new-vertex
+// queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
 constexpr int kHostSlots = kCntStride + 64;  // stripe 0, then the folded per-lane edges
 static_assert(kStripes * kCntStride >= kHostSlots, "per-lane edges do not fit the folded stripes");
@@ -151,10 +151,12 @@ struct g5_msbfs {
     // frontier's active vertices / edges (edges < 0: unknown, take push)
     int p = 0;
     int64_t act_verts = 0, act_edges = -1;
-    // the lanes with a non-empty frontier, and whether lane_edges holds each
-    // lane's frontier edges for the frontier the next level reads
+    // the lanes with a non-empty frontier, and the lanes whose frontier edges
+    // for the frontier the next level reads lane_edges holds (every lane
+    // after a push level, which counts them from its new-vertex queue; the
+    // lanes that joined since)
     uint64_t front_lanes = 0;
-    bool lane_edges_valid = false;
+    uint64_t lane_known = 0;
     // a split may pay: the previous level pushed (a lane may still lag
     // behind the others) or lanes joined since
     bool split_ok = false;
@@ -777,27 +779,18 @@ __global__ void k_level_begin(int64_t *cnt, int64_t *lane_edges)
 // active[w] bit i = ((next[64w + i] & lm) != 0).  A wave takes four
 // 64-vertex words per step (four independent loads in flight per lane).
 // kCount: also add the active vertices and their edges to the push unit's
-// next-frontier slots (pull levels count them in the pull kernels).  When
-// lane_edges is given, each lane's edges (the degrees of the vertices whose
-// word has that lane's bit) go to lane_edges[stripe][lane].  kOrLive: OR the
-// words into the level's live-lanes slot (kLive).
-// The per-lane edges only steer the split of the next level, so beyond the
-// first kLaneExactWords words (the highest-degree vertices) they are
-// estimated from every kLaneSample-th step of four words, weighted by
-// kLaneSample; the exact count costs a few instructions per frontier vertex.
-constexpr int64_t kLaneExactWords = 1024;
-constexpr int kLaneSample = 16;
+// next-frontier slots (pull levels count them in the pull kernels).
+// kOrLive: OR the words into the level's live-lanes slot (kLive).
 template <bool kCount, bool kOrLive>
 __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restrict__ next, int64_t nwords,
                                                          const int64_t *__restrict__ rowptr,
                                                          uint64_t *__restrict__ active, int64_t *__restrict__ cnt,
-                                                         uint64_t lm, int64_t *__restrict__ lane_edges)
+                                                         uint64_t lm)
 {
     __shared__ uint64_t s_live[kBlock / 64];
-    __shared__ int64_t s_lane[kBlock / 64][64];
     const int lane = __lane_id();
     const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-    int64_t nv = 0, ne = 0, le = 0;
+    int64_t nv = 0, ne = 0;
     uint64_t live = 0;
     for (int64_t w0 = (blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6)) * 4; w0 < nwords; w0 += waves * 4) {
         uint64_t x[4];
@@ -809,28 +802,10 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         for (int k = 0; k < 4; ++k) {
             const uint64_t bits = __ballot(x[k] != 0);
             if (lane == k) mine = bits;
-            const bool exact = w0 < kLaneExactWords;
-            const bool sampled = lane_edges && (exact || ((w0 >> 2) & (kLaneSample - 1)) == 0);
-            uint32_t d = 0;
-            if ((kCount || sampled) && x[k]) {
+            if (kCount && x[k]) {
                 const int64_t v = (w0 + k) * 64 + lane;
-                d = (uint32_t)(rowptr[v + 1] - rowptr[v]);
-                if (kCount) {
-                    ++nv;
-                    ne += d;
-                }
-            }
-            if (sampled) {
-                // lane r adds the degrees of this word's vertices whose word
-                // holds bit r (wave-uniform loop over the set words)
-                const int64_t wt = exact ? 1 : kLaneSample;
-                uint64_t todo = bits;
-                while (todo) {
-                    const int src = __ffsll((long long)todo) - 1;
-                    todo &= todo - 1;
-                    const uint64_t xs = lane_read64(x[k], src);
-                    if (xs >> lane & 1) le += wt * lane_read(d, src);
-                }
+                ++nv;
+                ne += rowptr[v + 1] - rowptr[v];
             }
         }
         if (lane < 4 && w0 + lane < nwords) active[w0 + lane] = mine;
@@ -839,7 +814,6 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
         live = g5_wave_or(live);
         if (lane == 0) s_live[threadIdx.x >> 6] = live;
     }
-    if (lane_edges) s_lane[threadIdx.x >> 6][lane] = le;
     if (kCount) block_add(cnt + kUnitSlots * kPushUnit + kNextVerts, {nv, ne});  // its barrier orders s_live
     else __syncthreads();
     if (kOrLive && threadIdx.x == 0) {
@@ -847,14 +821,6 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
         if (t) atomicOr((unsigned long long *)&cnt[(blockIdx.x % kStripes) * kCntStride + kLive], (unsigned long long)t);
-    }
-    if (lane_edges && threadIdx.x < 64) {
-        int64_t t = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) t += s_lane[w][threadIdx.x];
-        if (t)
-            atomicAdd((unsigned long long *)&lane_edges[(blockIdx.x % kLaneStripes) * 64 + threadIdx.x],
-                      (unsigned long long)t);
     }
 }
 
@@ -1201,8 +1167,7 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
     pull_bins(m, s, p, act_check, prof);
     hipLaunchKernelGGL((k_build_active<false, true>), g5::grid_for(nw, 16, 4096), kBlock, 0, s,
-                       (const uint64_t *)next, nw, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull,
-                       m->d_lane);
+                       (const uint64_t *)next, nw, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull);
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
@@ -1239,7 +1204,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     // the newly reached vertices of the lanes lm
     if (!whole)
         hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, lm, (int64_t *)nullptr);
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, lm);
     G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
     if (whole)
         hipLaunchKernelGGL(k_nq_counts, g5::grid_for(m->n, kBlock, 2048), kBlock, 0, s, (const uint32_t *)m->d_nq,
@@ -1253,7 +1218,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     // the whole next frontier: active bitmap, counts, live lanes, lane edges
     if (!whole)
         hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull, m->d_lane);
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
 
@@ -1301,7 +1266,7 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
     // the frontier vertices with a pushed lane, in act_next until push_lanes
     // rebuilds it
     hipLaunchKernelGGL((k_build_active<false, false>), g5::grid_for(nw, 16, 4096), kBlock, 0, s, front, nw,
-                       (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, pm, (int64_t *)nullptr);
+                       (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, pm);
     int rc = push_lanes(m, s, p, act_next, fqn, pm, m->listed == m->d_fq ? m->d_fq2 : m->d_fq);
     if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek_push[1], s));
@@ -1412,11 +1377,12 @@ void destroy_graphs(g5_msbfs *m)
 uint64_t split_lanes(const g5_msbfs *m)
 {
     const int64_t nnz = m->g->nnz;
-    if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->lane_edges_valid || !m->split_ok) return 0;
+    if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->split_ok) return 0;
     uint64_t pm = 0;
     int64_t pe = 0;
+    const uint64_t cand = m->front_lanes & m->lane_known;  // lanes of unknown size are pulled
     for (int r = 0; r < 64; ++r)
-        if ((m->front_lanes >> r & 1) && m->lane_edges[r] * m->tune.lane_ratio < nnz) {
+        if ((cand >> r & 1) && m->lane_edges[r] * m->tune.lane_ratio < nnz) {
             pm |= 1ull << r;
             pe += m->lane_edges[r];
         }
@@ -1464,11 +1430,12 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         next_verts += h[kUnitSlots * c + kNextVerts];
         next_edges += h[kUnitSlots * c + kNextEdges];
     }
-    m->lane_edges_valid = true;
-    for (int r = 0; r < 64; ++r) m->lane_edges[r] = h[kCntStride + r];
+    m->lane_known = push ? ~0ull : 0ull;
+    if (push)
+        for (int r = 0; r < 64; ++r) m->lane_edges[r] = h[kCntStride + r];
     m->listed_ok = push;
     m->listed_len = push ? h[kFqLen] : 0;
-    m->split_ok = push || pm;
+    m->split_ok = push;
     if (pm) ++m->split_levels;
     if (m->prof) {
         double rec[kLevelRec] = {0};
@@ -1625,7 +1592,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     m->p = 0;
     m->act_edges = -1;
     m->front_lanes = nroots == 64 ? ~0ull : (1ull << nroots) - 1;
-    m->lane_edges_valid = false;
+    m->lane_known = 0;
     m->split_levels = 0;
     m->listed_ok = true;  // k_ms_init cleared both frontier buffers
     m->listed_len = 0;
@@ -1669,7 +1636,7 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->front_lanes = 0;
     m->listed_ok = true;  // k_ms_init cleared both frontier buffers
     m->listed_len = 0;
-    m->lane_edges_valid = true;  // no lane has a frontier; joins add theirs
+    m->lane_known = 0;  // no lane has a frontier; joins add theirs
     std::fill(m->lane_edges, m->lane_edges + 64, 0);
     m->split_levels = 0;
     m->lvl.clear();
@@ -1710,6 +1677,7 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
     // neighbours): its frontier edges are the root's degree
     for (int i = 0; i < n; ++i) {
         m->lane_edges[lanes[i]] = added[2 + i];
+        m->lane_known |= 1ull << lanes[i];
         if (added[2 + i]) m->front_lanes |= 1ull << lanes[i];
     }
     m->split_ok = true;
