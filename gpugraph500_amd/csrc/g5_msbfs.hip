@@ -170,6 +170,7 @@ struct g5_msbfs {
     uint32_t *listed = nullptr;
     int64_t listed_len = 0;
     bool listed_ok = false;
+    bool last_empty = false;  // the level just run found no new (vertex, root) pair
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
     uint64_t busy = 0;
@@ -853,19 +854,22 @@ __global__ void k_level_end(int64_t *cnt, const int64_t *lane_edges)
     }
 }
 
+// visited := unused lanes, both active bitmaps := 0 and, unless the caller
+// clears them entry by entry (fronts false), both frontier buffers := 0
 __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__ fa, uint64_t *__restrict__ fb,
-                          uint64_t *__restrict__ act, int64_t n, uint64_t unused_lanes)
+                          uint64_t *__restrict__ act, int64_t n, uint64_t unused_lanes, bool fronts)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         visited[i] = unused_lanes;
-        fa[i] = 0;
-        fb[i] = 0;
+        if (fronts) {
+            fa[i] = 0;
+            fb[i] = 0;
+        }
         if (i < 2 * (n / 64)) act[i] = 0;
     }
 }
 
-// Roots: one thread; lanes of duplicate roots share the vertex.
 // One thread per root (roots may share a vertex, hence the atomics).  A root
 // without neighbours is reached and done: it never enters the frontier, so
 // no frontier word of a vertex outside the degree bins (which the pull
@@ -1225,16 +1229,16 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
 // One push level on parity p: direct launches (the frontier queue's length
 // sizes the degree scan on the host).  fqn = active vertices of the frontier.
 // next is cleared entry by entry when the level before was a push level too
-// (its frontier queue lists the only non-zero words of this level's next)
-// and the list is short; otherwise by a fill of all n words.  The two push
+// (its frontier queue lists the only non-zero words of this level's next:
+// listed_in) and the list is short; otherwise by a fill of all n words.  The two push
 // queue buffers alternate, so that list survives into this level.
-int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool prof)
+int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool listed_in, bool prof)
 {
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
-    if (m->listed_ok && m->listed_len * kListedFrac < m->n) {
+    if (listed_in && m->listed_len * kListedFrac < m->n) {
         if (m->listed_len)
             hipLaunchKernelGGL(k_zero_listed, g5::grid_for(m->listed_len, 256, 4096), 256, 0, s, next,
                                (const uint32_t *)m->listed, m->listed_len);
@@ -1398,13 +1402,17 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
 {
     hipStream_t s = m->s;
     const int p = m->p;
+    // until this level completes, the buffers' state is not known
+    const bool listed_in = m->listed_ok;
+    m->listed_ok = false;
+    m->last_empty = false;
     const int64_t act_verts = m->act_verts, act_edges = m->act_edges;
     const int64_t nnz = m->g->nnz;
     const bool push = m->mode == 2 || (m->mode == 0 && (act_edges < 0 || act_edges * m->tune.push_ratio < nnz));
     const bool act_check = act_edges < 0 || act_edges * m->tune.active_ratio < nnz;
     const uint64_t pm = push ? 0 : split_lanes(m);
     if (push) {
-        int rc = push_level(m, s, p, act_verts, m->prof);
+        int rc = push_level(m, s, p, act_verts, listed_in, m->prof);
         if (rc) return rc;
     } else {
         hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm);
@@ -1435,6 +1443,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         for (int r = 0; r < 64; ++r) m->lane_edges[r] = h[kCntStride + r];
     m->listed_ok = push;
     m->listed_len = push ? h[kFqLen] : 0;
+    m->last_empty = new_bits == 0;
     m->split_ok = push;
     if (pm) ++m->split_levels;
     if (m->prof) {
@@ -1580,8 +1589,17 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     G5_HIP(hipStreamSynchronize(s));
     G5_HIP(hipEventRecord(m->ev0, s));
     const uint64_t unused = nroots == 64 ? 0ull : ~((1ull << nroots) - 1);
+    // when the last level run was a push level that found nothing, its next
+    // buffer is clear and its frontier buffer is non-zero only at the entries
+    // of its frontier queue: clear those instead of both buffers
+    const bool listed = m->listed_ok && m->last_empty && m->listed_len * kListedFrac < m->n;
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
-                       m->d_act, m->n, unused);
+                       m->d_act, m->n, unused, !listed);
+    if (listed && m->listed_len) {
+        uint64_t *last_front = m->p == 0 ? m->d_fb : m->d_fa;  // parity p ^ 1 read it
+        hipLaunchKernelGGL(k_zero_listed, g5::grid_for(m->listed_len, 256, 4096), 256, 0, s, last_front,
+                           (const uint32_t *)m->listed, m->listed_len);
+    }
     hipLaunchKernelGGL(k_ms_roots, 1, 64, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
                        (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
     G5_HIP(hipGetLastError());
@@ -1624,7 +1642,7 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     // every lane free: free lanes count as visited everywhere, so they never
     // keep a vertex from the fully-visited early exit
     hipLaunchKernelGGL(k_ms_init, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->d_fa, m->d_fb,
-                       m->d_act, m->n, ~0ull);
+                       m->d_act, m->n, ~0ull, true);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(s));
     m->cont = true;
@@ -1678,6 +1696,7 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
     for (int i = 0; i < n; ++i) {
         m->lane_edges[lanes[i]] = added[2 + i];
         m->lane_known |= 1ull << lanes[i];
+        m->last_empty = false;  // the roots sit in the buffer the next level reads
         if (added[2 + i]) m->front_lanes |= 1ull << lanes[i];
     }
     m->split_ok = true;

@@ -168,6 +168,25 @@ def test_split_levels_match_oracle(scale):
     _check_all_lanes(m, ref, roots)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_back_to_back_runs(mode):
+    # a run after a run whose last level was a push level clears the frontier
+    # buffers entry by entry; after a pull level, whole
+    e, ref, g = kron(13)
+    roots = O.select_roots(ref, 1 << 13)
+    m = g5.MSBFS(g)
+    m.set_mode(mode)
+    for rs in (roots, roots[::-1][:40], roots[5:9], roots):
+        m.run(rs)
+        _check_all_lanes(m, ref, rs)
+    # continuous steps in between, then a batch run again
+    m.open()
+    m.join(roots[:3].tolist(), [0, 1, 2])
+    m.step()
+    m.run(roots[10:30])
+    _check_all_lanes(m, ref, roots[10:30])
+
+
 def test_autotune_keeps_results():
     e, ref, g = kron(13)
     roots = O.select_roots(ref, 1 << 13)
