@@ -302,12 +302,16 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
+    __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
-    // wave-uniform loop: the wave takes 64 consecutive vertices per step
+    uint64_t live = 0;
+    // wave-uniform loop: the wave takes 64 consecutive vertices per step, one
+    // word of the next frontier's active bitmap (the bin starts at a multiple
+    // of 64)
     for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < nlist;
          base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + lane;
@@ -345,6 +349,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
             }
             next[v] = found;
         }
+        live |= found;
+        const uint64_t word = __ballot(found != 0);
+        if (lane == 0) act_next[(first_v + base) >> 6] = word;
         // write-out: one vertex per step, lane = root; the vertex's kept
         // hits are disjoint and cover its found lanes but the spilled ones
         uint64_t todo = __ballot(nh != 0);
@@ -367,7 +374,17 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
             if (mine) __builtin_nontemporal_store((int32_t)p, &parent[(int64_t)lane_read(v, src) * 64 + lane]);
         }
     }
-    block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});
+    live = g5_wave_or(live);
+    if (lane == 0) s_live[threadIdx.x >> 6] = live;
+    block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});  // its barrier orders s_live
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t |= s_live[w];
+        if (t)
+            atomicOr((unsigned long long *)&cnt_all[(blockIdx.x % kStripes) * kCntStride + kLive],
+                     (unsigned long long)t);
+    }
 }
 
 // One wavefront per vertex (thread_max < degree <= wave_max): a step loads
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
     __shared__ int32_t s_par[kBlock / 64][64];
     int32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
@@ -476,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin)
+    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
     __shared__ uint64_t s_wave[kBlockUnroll][kBlock / 64];
     __shared__ uint64_t s_vis;
@@ -1144,12 +1161,13 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     const uint64_t *act = m->d_act + (p == 0 ? 0 : nw);
+    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     for (int k = 0; k < kBins; ++k) {
         if (prof) (void)hipEventRecord(m->ek[k], s);
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.new2old, \
-                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k
+                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
                               : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
                                        : g5::grid_for(cnt, kBlock, 16384);
@@ -1170,8 +1188,12 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
     pull_bins(m, s, p, act_check, prof);
-    hipLaunchKernelGGL((k_build_active<false, true>), g5::grid_for(nw, 16, 4096), kBlock, 0, s,
-                       (const uint64_t *)next, nw, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull);
+    // the thread kernel wrote its words of the active bitmap; the block and
+    // wave bins' words (and the live lanes among them) come from next
+    const int64_t nw01 = (m->rl.bin_first[2] + 63) / 64;
+    if (nw01)
+        hipLaunchKernelGGL((k_build_active<false, true>), g5::grid_for(nw01, 16, 4096), kBlock, 0, s,
+                           (const uint64_t *)next, nw01, (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, ~0ull);
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
@@ -1359,6 +1381,12 @@ int compute_bins(g5_msbfs *m)
     G5_HIP(hipStreamSynchronize(s));
     R.bin_first[0] = 0;
     for (int k = 0; k < kBins; ++k) R.bin_first[k + 1] = R.bin_first[k] + (int64_t)hb[k];
+    // the thread bin starts at a multiple of 64 (its waves write whole words
+    // of the active bitmap): the first bins' ends round up, moving at most 63
+    // vertices each into the bin before (any bin gives the same results)
+    const int64_t end = R.bin_first[kBins];
+    for (int k = 1; k < kBins; ++k)
+        R.bin_first[k] = std::min(end, std::max(R.bin_first[k - 1], (R.bin_first[k] + 63) / 64 * 64));
     return G5_OK;
 }
 

@@ -228,7 +228,8 @@ int g5_msbfs_stats(g5_msbfs *m, int unit, double *ms, int64_t *launches, double 
  * if the level pushed (2 if it pulled some lanes and pushed the others), the
  * frontier's vertices and edges.  *nlev = levels recorded; at most cap are copied. */
 int g5_msbfs_levels(g5_msbfs *m, double *out, int cap, int *nlev);
-/* Vertex counts of the three degree bins (units 0..2 above), then the number
+/* Vertex counts of the three degree bins (units 0..2 above; the first two
+ * end at multiples of 64, so up to 63 vertices sit one bin early), then the number
  * of split levels since the last g5_msbfs_run / g5_msbfs_open; up to cap. */
 int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
 /* Continuous mode (serving): lanes join and leave between levels, so a root
