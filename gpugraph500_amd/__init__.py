@@ -117,6 +117,8 @@ def lib():
     _sig(L, "g5_s4_roundtrip", I, [P, c_sz, P, c_sz, P, ctypes.POINTER(c_sz)])
     _sig(L, "g5_vreduce_or", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_min_pred", I, [P, I, c_i64, P])
+    _sig(L, "g5_vreduce_or_list", I, [P, I, c_i64, P])
+    _sig(L, "g5_vreduce_min_pred_list", I, [P, I, c_i64, P])
     _lib = L
     return L
 
@@ -533,6 +535,26 @@ def vreduce_min_pred(preds):
     d = DeviceArray.from_host(preds.reshape(-1))
     o = DeviceArray(n, np.int64)
     check(lib().g5_vreduce_min_pred(d.ptr, k, n, o.ptr))
+    return o.to_host()
+
+
+def vreduce_or_list(maps):
+    """The OR of k bitmaps held in k separate device buffers."""
+    bufs = [DeviceArray.from_host(np.ascontiguousarray(m, dtype=np.uint64)) for m in maps]
+    n = bufs[0].n
+    ptrs = (ctypes.c_void_p * len(bufs))(*[b.ptr for b in bufs])
+    o = DeviceArray(n, np.uint64)
+    check(lib().g5_vreduce_or_list(ptrs, len(bufs), n, o.ptr))
+    return o.to_host()
+
+
+def vreduce_min_pred_list(preds):
+    """The minimum present id over k predecessor arrays in separate buffers."""
+    bufs = [DeviceArray.from_host(np.ascontiguousarray(p, dtype=np.int64)) for p in preds]
+    n = bufs[0].n
+    ptrs = (ctypes.c_void_p * len(bufs))(*[b.ptr for b in bufs])
+    o = DeviceArray(n, np.int64)
+    check(lib().g5_vreduce_min_pred_list(ptrs, len(bufs), n, o.ptr))
     return o.to_host()
 
 

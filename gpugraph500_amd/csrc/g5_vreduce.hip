@@ -6,8 +6,10 @@
 // lists, where this build takes the minimum present id (-1 = absent) instead
 // of the reference's rank precedence, so the combined parent is the
 // canonical minimum-id parent whatever the arrival order.  On one GPU the
-// contributions a rank would receive are laid out contiguously and combined
-// in one pass: k reads + 1 write per element, 16-byte loads, grid-stride.
+// contributions a rank would receive are laid out contiguously, or named by
+// a list of up to kMaxList pointers (the _list forms, passed by value as a
+// kernel argument), and combined in one pass: k reads + 1 write per element,
+// 16-byte loads, grid-stride.
 #include "g5_internal.hpp"
 
 namespace {
@@ -71,6 +73,48 @@ __global__ __launch_bounds__(256) void k_min_pred(const int64_t *__restrict__ pr
     }
 }
 
+constexpr int kMaxList = 64;
+template <typename T>
+struct PtrList {
+    const T *p[kMaxList];
+};
+
+__global__ __launch_bounds__(256) void k_or_list(PtrList<uint64_t> m, int k, int64_t n, uint64_t *__restrict__ out)
+{
+    const bool vec = (n % 2) == 0;  // every map starts 16-byte aligned
+    if (vec) {
+        const int64_t n2 = n / 2;
+        ulonglong2 *o2 = reinterpret_cast<ulonglong2 *>(out);
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            ulonglong2 a = reinterpret_cast<const ulonglong2 *>(m.p[0])[i];
+            for (int j = 1; j < k; ++j) {
+                const ulonglong2 b = reinterpret_cast<const ulonglong2 *>(m.p[j])[i];
+                a.x |= b.x;
+                a.y |= b.y;
+            }
+            o2[i] = a;
+        }
+    } else {
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            uint64_t a = m.p[0][i];
+            for (int j = 1; j < k; ++j) a |= m.p[j][i];
+            out[i] = a;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_min_pred_list(PtrList<int64_t> m, int k, int64_t n,
+                                                       int64_t *__restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t x = pkey(m.p[0][i]);
+        for (int j = 1; j < k; ++j) x = min(x, pkey(m.p[j][i]));
+        out[i] = x == ~0ull ? -1 : (int64_t)x;
+    }
+}
+
 }  // namespace
 
 namespace g5 {
@@ -107,6 +151,40 @@ extern "C" int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int
                "g5_vreduce_min_pred: buffers must be 16-byte aligned");
     if (n == 0) return G5_OK;
     G5_HIP(g5::vreduce_min_pred(d_preds, k, n, d_out, nullptr));
+    G5_HIP(hipStreamSynchronize(nullptr));
+    return G5_OK;
+}
+
+extern "C" int g5_vreduce_or_list(const uint64_t *const *d_maps, int k, int64_t n_words, uint64_t *d_out)
+{
+    G5_REQUIRE(d_maps && d_out && k >= 1 && k <= kMaxList && n_words >= 0, G5_E_ARG,
+               "g5_vreduce_or_list: bad arguments (1 <= k <= %d)", kMaxList);
+    PtrList<uint64_t> m{};
+    for (int j = 0; j < k; ++j) {
+        G5_REQUIRE(d_maps[j] && ((uintptr_t)d_maps[j] & 15) == 0, G5_E_ARG,
+                   "g5_vreduce_or_list: map %d null or not 16-byte aligned", j);
+        m.p[j] = d_maps[j];
+    }
+    G5_REQUIRE(((uintptr_t)d_out & 15) == 0, G5_E_ARG, "g5_vreduce_or_list: output must be 16-byte aligned");
+    if (n_words == 0) return G5_OK;
+    hipLaunchKernelGGL(k_or_list, g5::grid_for(n_words / 2 + 1, 256, 8192), 256, 0, nullptr, m, k, n_words, d_out);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipStreamSynchronize(nullptr));
+    return G5_OK;
+}
+
+extern "C" int g5_vreduce_min_pred_list(const int64_t *const *d_preds, int k, int64_t n, int64_t *d_out)
+{
+    G5_REQUIRE(d_preds && d_out && k >= 1 && k <= kMaxList && n >= 0, G5_E_ARG,
+               "g5_vreduce_min_pred_list: bad arguments (1 <= k <= %d)", kMaxList);
+    PtrList<int64_t> m{};
+    for (int j = 0; j < k; ++j) {
+        G5_REQUIRE(d_preds[j], G5_E_ARG, "g5_vreduce_min_pred_list: array %d null", j);
+        m.p[j] = d_preds[j];
+    }
+    if (n == 0) return G5_OK;
+    hipLaunchKernelGGL(k_min_pred_list, g5::grid_for(n, 256, 8192), 256, 0, nullptr, m, k, n, d_out);
+    G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(nullptr));
     return G5_OK;
 }

@@ -64,7 +64,6 @@ Grid2D::Grid2D(int R, int C, g5_graph **blocks, Exchange mode, Compression<int64
         }
     if (mode_ == Exchange::Device) {
         const int64_t w = max_cols_ / 64;
-        d_colmaps_ = dalloc<uint64_t>((size_t)R * w);
         d_reduced_ = dalloc<uint64_t>((size_t)C * w);
         d_fold_ = dalloc<uint64_t>(w);
         d_preds_ = dalloc<int64_t>((size_t)R * max_cols_);
@@ -79,7 +78,7 @@ Grid2D::Grid2D(int R, int C, g5_graph **blocks, Exchange mode, Compression<int64
 
 Grid2D::~Grid2D()
 {
-    void *bufs[] = {d_colmaps_, d_reduced_, d_fold_, d_preds_, d_pred_out_, d_q_, d_words_, d_q2_};
+    void *bufs[] = {d_reduced_, d_fold_, d_preds_, d_pred_out_, d_q_, d_words_, d_q2_};
     for (void *p : bufs)
         if (p) g5_dfree(p);
 }
@@ -160,13 +159,15 @@ void Grid2D::exchange_device(GridStats &st, int r0, int c0)
     for (int c = 0; c < C_; ++c) {
         const double t0 = now();
         const int64_t w = col_len_[c] / 64;
+        // the column's out-bitmaps where the ranks keep them (an all-gather
+        // into one buffer with RCCL)
+        std::vector<const uint64_t *> maps(R_);
         for (int r = 0; r < R_; ++r) {
             int64_t nw = 0;
-            const uint64_t *bits = rank(r, c).outBitmap(nw);
-            check(g5_memcpy_d2d(d_colmaps_ + (size_t)r * w, bits, (size_t)w * 8), "g5_memcpy_d2d");
+            maps[r] = rank(r, c).outBitmap(nw);
         }
         uint64_t *red = d_reduced_ + (size_t)c * wmax;
-        check(g5_vreduce_or(d_colmaps_, R_, w, red), "g5_vreduce_or");
+        check(g5_vreduce_or_list(maps.data(), R_, w, red), "g5_vreduce_or_list");
         if (bottom_up_) {
             int64_t k = 0;
             check(g5_bitmap_count(red, w, &k), "g5_bitmap_count");

@@ -20,7 +20,7 @@
 //                         every message through a Compression schema
 //                         (compress -> decompress) as it would cross the wire
 //   Exchange::Device      bitmaps stay in HBM: OR of the column's
-//                         out-bitmaps with g5_vreduce_or, the fold by device
+//                         out-bitmaps with g5_vreduce_or_list, the fold by device
 //                         bitmap, optionally through the GPU codec (bitmap ->
 //                         queue -> s4-bp128-d4 encode -> decode -> bitmap),
 //                         and g5_vreduce_min_pred for the predecessors
@@ -90,7 +90,6 @@ private:
     int64_t n_total_ = 0, max_cols_ = 0;
     std::vector<int64_t> gdeg_;  // generated degree per global vertex (sum over a row of blocks)
     // device scratch for Exchange::Device
-    uint64_t *d_colmaps_ = nullptr;   // R x max_cols/64 words
     uint64_t *d_reduced_ = nullptr;   // C x max_cols/64 words
     uint64_t *d_fold_ = nullptr;      // max_cols/64 words
     int64_t *d_preds_ = nullptr;      // R x max_cols

@@ -292,9 +292,16 @@ int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t ca
 /* ---- vreduce combines (vreduce.hpp:41-544, globalbfs.hh:235-524) ----
  * d_maps: k contiguous bitmaps of n_words uint64 each -> d_out = OR. */
 int g5_vreduce_or(const uint64_t *d_maps, int k, int64_t n_words, uint64_t *d_out);
+/* The same over k <= 64 bitmaps wherever they lie: d_maps is a host array of
+ * k device pointers (16-byte aligned), e.g. the ranks' g5_bfs_out_bitmap
+ * buffers, so nothing is copied together first.  The reduce_fq_out functor's
+ * inputs (globalbfs.hh:617-622) map onto it one to one. */
+int g5_vreduce_or_list(const uint64_t *const *d_maps, int k, int64_t n_words, uint64_t *d_out);
 /* d_preds: k contiguous int64[n] predecessor arrays (-1 absent) -> d_out =
  * minimum present id (the canonical rule replacing rank precedence). */
 int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out);
+/* The same over k <= 64 arrays named by a host array of device pointers. */
+int g5_vreduce_min_pred_list(const int64_t *const *d_preds, int k, int64_t n, int64_t *d_out);
 
 /* ---- result validation (validate_bfs_result, validate/validate.h:249-491) ----
  * For a whole (1x1) graph.  *bad is 0 for a valid predecessor map, else the
