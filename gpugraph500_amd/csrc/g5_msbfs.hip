@@ -138,6 +138,7 @@ struct g5_msbfs {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nroots = 0;
     int64_t roots[64];
+    int32_t lane_of[64];  // batch run: root r's lane (continuous mode: lane r itself)
     // accounting
     bool prof = false;
     hipEvent_t ek[kBins + 1] = {};
@@ -887,22 +888,36 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
     }
 }
 
-// One thread per root (roots may share a vertex, hence the atomics).  A root
-// without neighbours is reached and done: it never enters the frontier, so
-// no frontier word of a vertex outside the degree bins (which the pull
-// kernels never rewrite) is ever non-zero.
+// One thread per root (roots may share a vertex, hence the atomics).  Root r
+// takes lane lane_of[r] = its rank by descending degree (ties by r): roots of
+// similar reach then sit in neighbouring lanes, so the lanes a vertex gains
+// at one level tend to share the 64 B sectors of its parent record, and a
+// pull level writes fewer partial sectors.  A root without neighbours is
+// reached and done: it never enters the frontier, so no frontier word of a
+// vertex outside the degree bins (which the pull kernels never rewrite) is
+// ever non-zero.
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
                            const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
                            uint64_t *__restrict__ frontier, uint64_t *__restrict__ active,
-                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt)
+                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt, int32_t *__restrict__ lane_of)
 {
+    __shared__ int64_t s_deg[64];
     const int r = threadIdx.x;
+    int64_t v = 0, d = -1;
     if (r < nroots) {
-        const int64_t v = old2new[roots[r]];
-        atomicOr((unsigned long long *)&visited[v], 1ull << r);
-        parent[v * 64 + r] = (int32_t)roots[r];
-        if (rowptr[v + 1] != rowptr[v]) {
-            atomicOr((unsigned long long *)&frontier[v], 1ull << r);
+        v = old2new[roots[r]];
+        d = rowptr[v + 1] - rowptr[v];
+    }
+    s_deg[r] = d;
+    __syncthreads();
+    if (r < nroots) {
+        int lane = 0;
+        for (int j = 0; j < nroots; ++j) lane += (s_deg[j] > d || (s_deg[j] == d && j < r)) ? 1 : 0;
+        lane_of[r] = lane;
+        atomicOr((unsigned long long *)&visited[v], 1ull << lane);
+        parent[v * 64 + lane] = (int32_t)roots[r];
+        if (d != 0) {
+            atomicOr((unsigned long long *)&frontier[v], 1ull << lane);
             atomicOr((unsigned long long *)&active[v >> 6], 1ull << (v & 63));
         }
     }
@@ -1557,7 +1572,7 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kHostSlots * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_lane, kLaneStripes * 64 * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_scratch, (g5::scan_scratch_words(n + 1) + 16) * 8);
-    if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_roots, 64 * 8 + 64 * 4);  // roots, then lane_of
     if (e == hipSuccess) e = hipMalloc(&m->d_join, 194 * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_sums, 128 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
@@ -1629,7 +1644,8 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
                            (const uint32_t *)m->listed, m->listed_len);
     }
     hipLaunchKernelGGL(k_ms_roots, 1, 64, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
-                       (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt);
+                       (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt,
+                       (int32_t *)(m->d_roots + 64));
     G5_HIP(hipGetLastError());
     int levels = 1;
     m->lvl.clear();
@@ -1657,7 +1673,9 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         ++levels;
     }
     G5_HIP(hipEventRecord(m->ev1, s));
+    G5_HIP(hipMemcpyAsync(m->lane_of, m->d_roots + 64, 64 * 4, hipMemcpyDeviceToHost, s));
     G5_HIP(hipEventSynchronize(m->ev1));
+    G5_HIP(hipStreamSynchronize(s));
     if (ms) G5_HIP(hipEventElapsedTime(ms, m->ev0, m->ev1));
     if (nlevels) *nlevels = levels;
     return G5_OK;
@@ -1776,6 +1794,11 @@ extern "C" int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint6
     return G5_OK;
 }
 
+namespace {
+// the lane that holds the caller's lane `lane` (batch runs rank the roots)
+int internal_lane(const g5_msbfs *m, int lane) { return m->cont ? lane : m->lane_of[lane]; }
+}  // namespace
+
 extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level)
 {
     G5_REQUIRE(m && lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
@@ -1798,7 +1821,8 @@ extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *le
     if (e == hipSuccess && level) e = hipMalloc(&dflag, sizeof(int));
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_ms_extract, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint32_t *)m->rl.old2new,
-                           (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, n, lane, dp);
+                           (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, n, internal_lane(m, lane),
+                           dp);
         e = hipGetLastError();
     }
     if (e == hipSuccess && pred) e = hipMemcpyAsync(pred, dp, (size_t)n * 8, hipMemcpyDeviceToHost, s);
@@ -1838,7 +1862,8 @@ extern "C" int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred)
     G5_REQUIRE(lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
                "g5_msbfs_pred_device: lane %d not in use", lane);
     hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s, (const uint32_t *)m->rl.old2new,
-                       (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, m->n, lane, d_pred);
+                       (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, m->n, internal_lane(m, lane),
+                       d_pred);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(m->s));
     return G5_OK;
@@ -1860,7 +1885,7 @@ extern "C" int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge)
     if (e == hipSuccess) e = hipStreamSynchronize(m->s);
     (void)hipFree(d);
     G5_HIP(e);
-    for (int r = 0; r < m->nroots; ++r) nedge[r] = (int64_t)(h[r] / 2);
+    for (int r = 0; r < m->nroots; ++r) nedge[r] = (int64_t)(h[internal_lane(m, r)] / 2);
     return G5_OK;
 }
 
