@@ -175,7 +175,9 @@ struct g5_msbfs {
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
     uint64_t busy = 0;
-    int64_t *d_join = nullptr;  // roots [0, 64), lanes [64, 128), added frontier [128, 130), root degrees [130, 194)
+    // join: roots [0, 64), lanes [64, 128), added frontier [128, 130), root
+    // degrees [130, 194); batch run: the roots' reach [0, 64)
+    int64_t *d_join = nullptr;
     unsigned long long *d_sums = nullptr;  // leave: checksums [0, 64), generated-degree sums [64, 128)
 };
 
@@ -888,9 +890,26 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
     }
 }
 
+// reach[r] = the edges of root r's neighbours (its second level's frontier
+// edges); one workgroup per root
+__global__ __launch_bounds__(256) void k_root_reach(const int64_t *__restrict__ roots, const uint32_t *__restrict__ old2new,
+                                                    const int64_t *__restrict__ rowptr,
+                                                    const uint32_t *__restrict__ cols, int64_t *__restrict__ reach)
+{
+    const int64_t v = old2new[roots[blockIdx.x]];
+    int64_t t = 0;
+    for (int64_t j = rowptr[v] + threadIdx.x; j < rowptr[v + 1]; j += blockDim.x) {
+        const uint32_t u = cols[j];
+        t += rowptr[u + 1] - rowptr[u];
+    }
+    t = g5_block_sum256(t);
+    if (threadIdx.x == 0) reach[blockIdx.x] = t;
+}
+
 // One thread per root (roots may share a vertex, hence the atomics).  Root r
-// takes lane lane_of[r] = its rank by descending degree (ties by r): roots of
-// similar reach then sit in neighbouring lanes, so the lanes a vertex gains
+// takes lane lane_of[r] = its rank by descending reach (the edges of its
+// neighbours; ties by r): roots of similar reach then sit in neighbouring
+// lanes, so the lanes a vertex gains
 // at one level tend to share the 64 B sectors of its parent record, and a
 // pull level writes fewer partial sectors.  A root without neighbours is
 // reached and done: it never enters the frontier, so no frontier word of a
@@ -899,20 +918,22 @@ __global__ void k_ms_init(uint64_t *__restrict__ visited, uint64_t *__restrict__
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
                            const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
                            uint64_t *__restrict__ frontier, uint64_t *__restrict__ active,
-                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt, int32_t *__restrict__ lane_of)
+                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt, int32_t *__restrict__ lane_of,
+                           const int64_t *__restrict__ reach)
 {
-    __shared__ int64_t s_deg[64];
+    __shared__ int64_t s_key[64];
     const int r = threadIdx.x;
-    int64_t v = 0, d = -1;
+    int64_t v = 0, d = -1, key = -1;
     if (r < nroots) {
         v = old2new[roots[r]];
         d = rowptr[v + 1] - rowptr[v];
+        key = reach[r];
     }
-    s_deg[r] = d;
+    s_key[r] = key;
     __syncthreads();
     if (r < nroots) {
         int lane = 0;
-        for (int j = 0; j < nroots; ++j) lane += (s_deg[j] > d || (s_deg[j] == d && j < r)) ? 1 : 0;
+        for (int j = 0; j < nroots; ++j) lane += (s_key[j] > key || (s_key[j] == key && j < r)) ? 1 : 0;
         lane_of[r] = lane;
         atomicOr((unsigned long long *)&visited[v], 1ull << lane);
         parent[v * 64 + lane] = (int32_t)roots[r];
@@ -1643,9 +1664,11 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         hipLaunchKernelGGL(k_zero_listed, g5::grid_for(m->listed_len, 256, 4096), 256, 0, s, last_front,
                            (const uint32_t *)m->listed, m->listed_len);
     }
+    hipLaunchKernelGGL(k_root_reach, nroots, 256, 0, s, (const int64_t *)d_roots, (const uint32_t *)m->rl.old2new,
+                       (const int64_t *)m->rl.rowptr, (const uint32_t *)m->rl.cols, m->d_join);
     hipLaunchKernelGGL(k_ms_roots, 1, 64, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
                        (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt,
-                       (int32_t *)(m->d_roots + 64));
+                       (int32_t *)(m->d_roots + 64), (const int64_t *)m->d_join);
     G5_HIP(hipGetLastError());
     int levels = 1;
     m->lvl.clear();
