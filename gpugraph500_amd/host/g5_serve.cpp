@@ -38,6 +38,9 @@ Engine::Engine(int scale, int edgefactor, int lanes, int window, int small)
     keys_.resize(found);
     check(g5_msbfs_create(g_, &m_), "g5_msbfs_create");
     check(g5_msbfs_open(m_), "g5_msbfs_open");
+    // allocated up front: a device allocation in the middle of serving can
+    // stall the other engines' streams
+    if (small_ > 0) check(g5_bfs_create(g_, &b_), "g5_bfs_create");
 }
 
 Engine::~Engine()
@@ -127,7 +130,6 @@ bool Engine::run_small(std::deque<Query *> &from)
     const size_t k = q->roots.size();
     q->nedge.assign(k, 0);
     q->checksum.assign(k, 0);
-    if (!b_) check(g5_bfs_create(g_, &b_), "g5_bfs_create");
     for (size_t i = 0; i < k; ++i) {
         int32_t nl = 0;
         float ms = 0;

@@ -86,7 +86,7 @@ private:
     int64_t n_ = 0;
     g5_graph *g_ = nullptr;
     g5_msbfs *m_ = nullptr;
-    g5_bfs *b_ = nullptr;  // single-root search for small queries (created on first use)
+    g5_bfs *b_ = nullptr;  // single-root search for small queries (small > 0)
     std::vector<int64_t> keys_;
     std::thread th_;
     std::mutex mu_;
