@@ -635,7 +635,7 @@ __global__ void k_dec_tail(const uint32_t *__restrict__ in, const int64_t *__res
 // kSmallRoundTrip values, so that a short level queue costs one launch
 // instead of a dozen.  The decode reads the words the encode wrote to
 // memory, as a receiver would.
-constexpr int64_t kSmallRoundTrip = 8192;
+constexpr int64_t kSmallRoundTrip = 8192;  // 32768: C2 53-56 -> 51 GTEPS (one workgroup is slower there)
 constexpr int kSmallBlocks = kSmallRoundTrip / 128;  // 64 miniblocks, 4 groups
 
 __global__ __launch_bounds__(256) void k_rt_small(const uint32_t *in, int64_t n, uint32_t *__restrict__ words,

@@ -136,7 +136,8 @@ def test_serial_parallel_switch():
         assert np.array_equal(g5.s4_decode(w, n), x)
 
 
-@pytest.mark.parametrize("n", [0, 1, 5, 127, 128, 129, 2047, 2048, 2049, 4173, 8191, 8192, 8193, 20000])
+@pytest.mark.parametrize("n", [0, 1, 5, 127, 128, 129, 2047, 2048, 2049, 4173, 8191, 8192, 8193, 20000, 32769,
+                               50000])
 def test_roundtrip_stream_is_the_reference_stream(n):
     """Short queues (up to 8192 values) take a one-workgroup round trip,
     longer ones the multi-kernel path: either way the stream written is the

@@ -65,6 +65,28 @@ def test_duplicate_roots_share_a_vertex():
         assert np.array_equal(lev, olev) and np.array_equal(pred, opred)
 
 
+def test_mixed_roots_keep_the_callers_lane_order():
+    # lanes are ranked by the roots' reach inside a batch run; results come
+    # back in the caller's order, also with an edgeless root, duplicates and
+    # ties in reach
+    e, ref, g = kron(12)
+    deg = np.diff(ref.rowptr)
+    iso = np.flatnonzero(deg == 0)
+    sel = O.select_roots(ref, 1 << 12)
+    rs = [int(sel[5]), int(sel[0]), int(sel[5])] + ([int(iso[0])] if len(iso) else []) + [int(x) for x in sel[10:40]]
+    m = g5.MSBFS(g)
+    m.run(rs)
+    ne = m.nedge()
+    for lane, r in enumerate(rs):
+        pred, lev = m.result(lane)
+        opred, olev, _ = O.bfs_bin(ref, r)
+        assert np.array_equal(lev, olev) and np.array_equal(pred, opred), (lane, r)
+        assert ne[lane] == O.nedge(ref, olev)
+        d = g5.DeviceArray(g.n_cols, np.int64)
+        m.pred_device(lane, d)
+        assert np.array_equal(d.to_host(), opred)
+
+
 def test_profile_path_equals_graph_path():
     e, ref, g = kron(14)
     roots = O.select_roots(ref, 1 << 14)
