@@ -118,6 +118,7 @@ def lib():
     _sig(L, "g5_vreduce_or", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_min_pred", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_or_list", I, [P, I, c_i64, P])
+    _sig(L, "g5_msbfs_reach", I, [P, P, I, P])
     _sig(L, "g5_vreduce_min_pred_list", I, [P, I, c_i64, P])
     _lib = L
     return L
@@ -477,6 +478,13 @@ class MSBFS:
         out = np.zeros(3, np.int64)
         check(lib().g5_msbfs_info(self.h, _p(out), 3))
         return out.tolist()
+
+    def reach(self, roots):
+        """Per root: the edges of its neighbours (at most 64 roots)."""
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        out = np.zeros(max(r.size, 1), np.int64)
+        check(lib().g5_msbfs_reach(self.h, _p(r), r.size, _p(out)))
+        return out[:r.size]
 
     def split_levels(self):
         """Levels since the last run/open that pulled some lanes and pushed the others."""

@@ -2078,3 +2078,21 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     if (trials) *trials = n_trials;
     return G5_OK;
 }
+
+extern "C" int g5_msbfs_reach(g5_msbfs *m, const int64_t *roots, int n, int64_t *reach)
+{
+    G5_REQUIRE(m && n >= 0 && n <= 64 && (n == 0 || (roots && reach)), G5_E_ARG, "g5_msbfs_reach: bad arguments");
+    if (n == 0) return G5_OK;
+    for (int i = 0; i < n; ++i)
+        G5_REQUIRE(roots[i] >= 0 && roots[i] < m->n, G5_E_ARG, "g5_msbfs_reach: root %lld out of range",
+                   (long long)roots[i]);
+    hipStream_t s = m->s;
+    G5_HIP(hipStreamSynchronize(s));  // d_roots and d_sums are free between calls
+    G5_HIP(hipMemcpyAsync(m->d_roots, roots, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_root_reach, n, 256, 0, s, (const int64_t *)m->d_roots, (const uint32_t *)m->rl.old2new,
+                       (const int64_t *)m->rl.rowptr, (const uint32_t *)m->rl.cols, (int64_t *)m->d_sums);
+    G5_HIP(hipGetLastError());
+    G5_HIP(hipMemcpyAsync(reach, m->d_sums, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    G5_HIP(hipStreamSynchronize(s));
+    return G5_OK;
+}

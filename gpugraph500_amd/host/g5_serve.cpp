@@ -37,6 +37,8 @@ Engine::Engine(int scale, int edgefactor, int lanes, int window, int small)
     check(g5_select_roots(g_, (int64_t)1 << scale, 64, keys_.data(), &found), "g5_select_roots");
     keys_.resize(found);
     check(g5_msbfs_create(g_, &m_), "g5_msbfs_create");
+    key_reach_.assign(keys_.size(), 0);
+    check(g5_msbfs_reach(m_, keys_.data(), (int)keys_.size(), key_reach_.data()), "g5_msbfs_reach");
     check(g5_msbfs_open(m_), "g5_msbfs_open");
     // allocated up front: a device allocation in the middle of serving can
     // stall the other engines' streams
@@ -88,31 +90,51 @@ bool Engine::in_window() const
     return true;
 }
 
+int64_t Engine::reach_of(int64_t root) const
+{
+    for (size_t i = 0; i < keys_.size(); ++i)
+        if (keys_[i] == root) return key_reach_[i];
+    return 0;
+}
+
 int Engine::admit(std::deque<Query *> &from)
 {
     if (!in_window()) return 0;  // wait for the running cohort
-    std::vector<int64_t> roots;
-    std::vector<int> lanes;
     const uint64_t allowed = max_lanes_ == 64 ? ~0ull : ((1ull << max_lanes_) - 1);
+    // the queries that fit, FIFO, and their roots
+    struct Pending {
+        Query *q;
+        int idx;
+        int64_t reach;
+    };
+    std::vector<Pending> adm;
+    int free_lanes = popc(allowed & ~busy_);
     while (!from.empty()) {
         Query *q = from.front();
         const int need = (int)q->roots.size();
         if (need > max_lanes_) throw std::invalid_argument("query with more roots than lanes");
-        if (popc(allowed & ~busy_) < need) break;  // FIFO: later queries wait behind it
+        if (free_lanes < need) break;  // FIFO: later queries wait behind it
         from.pop_front();
+        free_lanes -= need;
         q->admitted = clock();
         q->remaining = need;
         q->nedge.assign(need, 0);
         q->checksum.assign(need, 0);
-        for (int i = 0; i < need; ++i) {
-            const int lane = __builtin_ctzll(allowed & ~busy_);
-            busy_ |= 1ull << lane;
-            slots_[lane].q = q;
-            slots_[lane].idx = i;
-            slots_[lane].joined = level_;
-            roots.push_back(q->roots[i]);
-            lanes.push_back(lane);
-        }
+        for (int i = 0; i < need; ++i) adm.push_back({q, i, reach_of(q->roots[i])});
+    }
+    // lanes in ascending order by descending reach, as a batch run assigns
+    // them: roots of similar reach share parent-record sectors
+    std::stable_sort(adm.begin(), adm.end(), [](const Pending &a, const Pending &b) { return a.reach > b.reach; });
+    std::vector<int64_t> roots;
+    std::vector<int> lanes;
+    for (const Pending &a : adm) {
+        const int lane = __builtin_ctzll(allowed & ~busy_);
+        busy_ |= 1ull << lane;
+        slots_[lane].q = a.q;
+        slots_[lane].idx = a.idx;
+        slots_[lane].joined = level_;
+        roots.push_back(a.q->roots[a.idx]);
+        lanes.push_back(lane);
     }
     if (!roots.empty()) {
         check(g5_msbfs_join(m_, roots.data(), lanes.data(), (int)roots.size()), "g5_msbfs_join");

@@ -88,6 +88,8 @@ private:
     g5_msbfs *m_ = nullptr;
     g5_bfs *b_ = nullptr;  // single-root search for small queries (small > 0)
     std::vector<int64_t> keys_;
+    std::vector<int64_t> key_reach_;  // g5_msbfs_reach of each key
+    int64_t reach_of(int64_t root) const;
     std::thread th_;
     std::mutex mu_;
     std::condition_variable cv_;

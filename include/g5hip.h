@@ -270,6 +270,11 @@ int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value);
 int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, int passes, double *best_ms, int *trials);
 int g5_msbfs_open(g5_msbfs *m);
 int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes, int n);
+/* reach[i] = the edges of roots[i]'s neighbours (the frontier edges of its
+ * second level), n <= 64.  Batch runs give roots lanes by descending reach,
+ * so that the lanes a vertex gains at one level share parent-record sectors;
+ * a server joining several roots can do the same. */
+int g5_msbfs_reach(g5_msbfs *m, const int64_t *roots, int n, int64_t *reach);
 int g5_msbfs_step(g5_msbfs *m, uint64_t *live, float *ms);
 int g5_msbfs_leave(g5_msbfs *m, uint64_t lanes, int64_t *nedge, uint64_t *checksum);
 

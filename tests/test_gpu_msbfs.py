@@ -87,6 +87,17 @@ def test_mixed_roots_keep_the_callers_lane_order():
         assert np.array_equal(d.to_host(), opred)
 
 
+def test_reach_is_the_neighbours_edges():
+    e, ref, g = kron(12)
+    deg = np.diff(ref.rowptr)
+    roots = np.concatenate([O.select_roots(ref, 1 << 12)[:40], np.flatnonzero(deg == 0)[:2]])
+    m = g5.MSBFS(g)
+    want = [int(deg[ref.cols[ref.rowptr[r]:ref.rowptr[r + 1]]].sum()) for r in roots.tolist()]
+    assert m.reach(roots).tolist() == want
+    with pytest.raises(g5.G5Error):
+        m.reach(np.arange(65))
+
+
 def test_profile_path_equals_graph_path():
     e, ref, g = kron(14)
     roots = O.select_roots(ref, 1 << 14)
