@@ -128,6 +128,9 @@ struct g5_msbfs {
     uint64_t *d_act = nullptr;           // two n/64-word active bitmaps (parity a, b)
     uint32_t *d_fq = nullptr, *d_nq = nullptr;  // push: frontier queue, newly reached queue
     int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
+    // parent records [n][64] by relabelled vertex: slot r of v holds root r's
+    // parent of v, as an original id (roots, push levels) or, from a pull
+    // level, as ~u for its relabelled id u (see parent_orig)
     int32_t *d_parent = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
     int64_t *d_lane = nullptr;  // [kLaneStripes][64] per-lane frontier edges
@@ -183,7 +186,20 @@ struct g5_msbfs {
 
 namespace {
 
-// parent slots of vertex v for the roots in `bits` := u (an original id)
+// A pull level stores a parent as ~u, u its relabelled id (a negative slot),
+// and needs no lookup of the original id, a random 4 B gather per hit; push
+// levels and the roots store original ids, which the push's atomicMin pass
+// compares (a slot is written by one level only, the level that reaches v in
+// that lane, and within a level the pulled and pushed lanes are disjoint).
+__device__ __forceinline__ int32_t pulled(uint32_t u) { return ~(int32_t)u; }
+
+// the original id of a parent slot's value
+__device__ __forceinline__ int32_t parent_orig(int32_t p, const uint32_t *__restrict__ new2old)
+{
+    return p < 0 ? (int32_t)new2old[~p] : p;
+}
+
+// parent slots of vertex v for the roots in `bits` := u
 __device__ __forceinline__ void write_parents(int32_t *__restrict__ parent, int64_t v, uint64_t bits, uint32_t u)
 {
     while (bits) {
@@ -264,7 +280,6 @@ template <int K, bool kActive>
 __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32_t v, uint64_t unv, uint64_t &found,
                                                     int &nh, uint32_t (&hu)[kHits], uint64_t (&hf)[kHits],
                                                     int64_t &ngath, const uint32_t *__restrict__ cols,
-                                                    const uint32_t *__restrict__ new2old,
                                                     const uint64_t *__restrict__ frontier,
                                                     const uint64_t *__restrict__ active, int32_t *__restrict__ parent)
 {
@@ -284,7 +299,7 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
         const uint64_t f = fr[k] & unv & ~found;
         if (!f) continue;
         found |= f;
-        const uint32_t uo = new2old[u[k]];
+        const uint32_t uo = (uint32_t)pulled(u[k]);
         if (nh < kHits) {
 #pragma unroll
             for (int h = 0; h < kHits; ++h)
@@ -303,7 +318,7 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -335,11 +350,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
                 if (j < e)
-                    j = thread_scan_step<kThreadFirst, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols, new2old,
+                    j = thread_scan_step<kThreadFirst, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
                                                                 frontier, active, parent);
                 while (j < e && found != unv)
                     j = thread_scan_step<kThreadUnroll, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
-                                                                 new2old, frontier, active, parent);
+                                                                 frontier, active, parent);
                 j = j < e ? j : e;
                 if (found) {
                     visited[v] = vis | found;
@@ -407,7 +422,7 @@ static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -454,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
                 const uint64_t prev = lanes_excl_or(fresh);
                 uint64_t first = fresh & ~prev;
                 if (first) {
-                    const int32_t uo = (int32_t)new2old[u[k]];
+                    const int32_t uo = pulled(u[k]);
                     while (first) {
                         sp[__ffsll((long long)first) - 1] = uo;
                         first &= first - 1;
@@ -494,7 +509,7 @@ constexpr int kBlockUnroll = 1;
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ new2old, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -559,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
                 }
                 uint64_t first = f[k] & ~(in_wave[k] | before);
                 if (first) {
-                    const int32_t uo = (int32_t)new2old[u[k]];
+                    const int32_t uo = pulled(u[k]);
                     while (first) {
                         s_par[__ffsll((long long)first) - 1] = uo;
                         first &= first - 1;
@@ -1004,13 +1019,14 @@ __global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ r
 
 // per-lane extraction: pred of root `lane` as int64[n] in original ids (-1
 // where the lane's visited bit is clear)
-__global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const int32_t *__restrict__ parent,
-                             const uint64_t *__restrict__ visited, int64_t n, int lane, int64_t *__restrict__ pred)
+__global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const uint32_t *__restrict__ new2old,
+                             const int32_t *__restrict__ parent, const uint64_t *__restrict__ visited, int64_t n,
+                             int lane, int64_t *__restrict__ pred)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = old2new[i];
-        pred[i] = (visited[v] >> lane & 1) ? (int64_t)parent[v * 64 + lane] : -1;
+        pred[i] = (visited[v] >> lane & 1) ? (int64_t)parent_orig(parent[v * 64 + lane], new2old) : -1;
     }
 }
 
@@ -1171,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
             for (int k = 0; k < kLeaveGroup; ++k) {
                 const uint32_t oj = lane_read(o, j0 + k), dj = lane_read(d, j0 + k);
                 if (rj[k] >> lane & 1) {
-                    ck += g5_mix64(((uint64_t)oj << 32) | (uint32_t)(pj[k] + 1));
+                    ck += g5_mix64(((uint64_t)oj << 32) | (uint32_t)(parent_orig(pj[k], new2old) + 1));
                     dg += dj;
                 }
             }
@@ -1202,8 +1218,8 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
         if (prof) (void)hipEventRecord(m->ek[k], s);
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
-#define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.new2old, \
-                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k, act_next
+#define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, m->d_visited, front, act, \
+                     next, m->d_parent, m->d_cnt, k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
                               : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
                                        : g5::grid_for(cnt, kBlock, 16384);
@@ -1571,6 +1587,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     G5_REQUIRE(g && out, G5_E_ARG, "g5_msbfs_create: null argument");
     G5_REQUIRE(g->n_rows == g->n_cols && g->row_start == g->col_start, G5_E_ARG,
                "g5_msbfs_create: needs a diagonal (1x1) block");
+    G5_REQUIRE(g->n_rows <= INT32_MAX, G5_E_ARG, "g5_msbfs_create: %lld vertices, at most 2^31 - 1 (int32 parents)",
+               (long long)g->n_rows);
     g5_msbfs *m = new g5_msbfs();
     m->g = g;
     m->n = g->n_rows;
@@ -1844,7 +1862,8 @@ extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *le
     if (e == hipSuccess && level) e = hipMalloc(&dflag, sizeof(int));
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_ms_extract, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint32_t *)m->rl.old2new,
-                           (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, n, internal_lane(m, lane),
+                           (const uint32_t *)m->rl.new2old, (const int32_t *)m->d_parent,
+                           (const uint64_t *)m->d_visited, n, internal_lane(m, lane),
                            dp);
         e = hipGetLastError();
     }
@@ -1885,7 +1904,8 @@ extern "C" int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred)
     G5_REQUIRE(lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
                "g5_msbfs_pred_device: lane %d not in use", lane);
     hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s, (const uint32_t *)m->rl.old2new,
-                       (const int32_t *)m->d_parent, (const uint64_t *)m->d_visited, m->n, internal_lane(m, lane),
+                       (const uint32_t *)m->rl.new2old, (const int32_t *)m->d_parent,
+                       (const uint64_t *)m->d_visited, m->n, internal_lane(m, lane),
                        d_pred);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(m->s));
