@@ -114,6 +114,10 @@ struct Relabeled {
     uint32_t *new2old = nullptr, *old2new = nullptr;
     int64_t *rowptr = nullptr;
     uint32_t *cols = nullptr;  // new ids, rows in ascending original-id order
+    // per vertex, its row's first two neighbours (low word first; 0 past the
+    // row's end): the thread kernel's first step reads them from here, 64
+    // vertices' in 512 contiguous bytes, instead of from 64 row starts
+    uint64_t *head = nullptr;
     uint32_t *gdeg = nullptr;
     int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
 };
@@ -267,6 +271,9 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // gathers are in flight per thread; its first step loads only kThreadFirst,
 // because on the dense levels most vertices are done after one or two
 // neighbours and every gather past the early exit is a wasted line fill.
+// That first step takes the two neighbours from the head array (one 8 B word
+// per vertex, contiguous across the wave) rather than from the row starts,
+// each in a line of its own once rows exceed 32 neighbours.
 // kActive (all pull kernels): skip neighbours whose bit in the active bitmap
 // is clear before gathering their frontier word (pays only when few sources
 // are active).  All pull kernels work on the lanes in cnt[kPullLanes] only:
@@ -275,18 +282,27 @@ constexpr int kHits = 4;
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4; 12 after the short first step: the same
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
 
-// One step of a thread's scan: neighbours [j, j + K) of the row ending at e.
-template <int K, bool kActive>
+// One step of a thread's scan: neighbours [j, j + K) of the row ending at e
+// (kHead: the row's first two, j at the row's start, from the head array).
+template <int K, bool kActive, bool kHead>
 __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32_t v, uint64_t unv, uint64_t &found,
                                                     int &nh, uint32_t (&hu)[kHits], uint64_t (&hf)[kHits],
                                                     int64_t &ngath, const uint32_t *__restrict__ cols,
+                                                    const uint64_t *__restrict__ head,
                                                     const uint64_t *__restrict__ frontier,
                                                     const uint64_t *__restrict__ active, int32_t *__restrict__ parent)
 {
+    static_assert(!kHead || K == 2, "the head array holds two neighbours per vertex");
     uint32_t u[K];
     uint64_t fr[K];
+    if (kHead) {
+        const uint64_t h = head[v];
+        u[0] = (uint32_t)h;
+        u[K - 1] = (uint32_t)(h >> 32);
+    } else {
 #pragma unroll
-    for (int k = 0; k < K; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
+        for (int k = 0; k < K; ++k) u[k] = j + k < e ? cols[j + k] : 0u;
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const bool take = j + k < e && (!kActive || (active[u[k] >> 6] >> (u[k] & 63) & 1));
@@ -318,7 +334,7 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -350,11 +366,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
                 if (j < e)
-                    j = thread_scan_step<kThreadFirst, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
-                                                                frontier, active, parent);
+                    j = thread_scan_step<kThreadFirst, kActive, true>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
+                                                                      head, frontier, active, parent);
                 while (j < e && found != unv)
-                    j = thread_scan_step<kThreadUnroll, kActive>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
-                                                                 frontier, active, parent);
+                    j = thread_scan_step<kThreadUnroll, kActive, false>(j, e, v, unv, found, nh, hu, hf, ngath,
+                                                                        cols, head, frontier, active, parent);
                 j = j < e ? j : e;
                 if (found) {
                     visited[v] = vis | found;
@@ -422,7 +438,7 @@ static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -509,7 +525,7 @@ constexpr int kBlockUnroll = 1;
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
 {
@@ -1017,6 +1033,18 @@ __global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ r
     }
 }
 
+// head[v] = the first two neighbours of relabelled row v (0 past its end)
+__global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols, int64_t n,
+                            uint64_t *__restrict__ head)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = rowptr[v], d = rowptr[v + 1] - b;
+        const uint64_t lo = d > 0 ? cols[b] : 0u, hi = d > 1 ? cols[b + 1] : 0u;
+        head[v] = lo | hi << 32;
+    }
+}
+
 // per-lane extraction: pred of root `lane` as int64[n] in original ids (-1
 // where the lane's visited bit is clear)
 __global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const uint32_t *__restrict__ new2old,
@@ -1218,8 +1246,8 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
         if (prof) (void)hipEventRecord(m->ek[k], s);
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
-#define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, m->d_visited, front, act, \
-                     next, m->d_parent, m->d_cnt, k, act_next
+#define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
+                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
                               : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
                                        : g5::grid_for(cnt, kBlock, 16384);
@@ -1392,6 +1420,7 @@ int build_relabel(g5_msbfs *m)
     if (e == hipSuccess) e = hipMalloc(&R.gdeg, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&R.head, (size_t)std::max<int64_t>(n, 1) * 8);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_degree_keys, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)g->d_rowptr, n,
                            keys);
@@ -1409,6 +1438,11 @@ int build_relabel(g5_msbfs *m)
         hipLaunchKernelGGL(k_perm_rows, g5::grid_for((n + 3) / 4, 1, 65536), 256, 0, s,
                            (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_cols, (const uint32_t *)R.new2old,
                            (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_row_heads, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)R.rowptr,
+                           (const uint32_t *)R.cols, n, R.head);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1639,7 +1673,7 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
-                   m->d_lane,    m->d_fq2};
+                   m->d_lane,    m->d_fq2,     m->rl.head};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
