@@ -1234,6 +1234,7 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
 
 // The three pull kernels of parity p (act: they consult the active bitmap;
 // prof: events between them).  They work on the lanes in cnt[kPullLanes].
+// They share one argument list; only the thread kernel reads `head`.
 void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
 {
     const Relabeled &R = m->rl;
