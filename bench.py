@@ -18,18 +18,27 @@ roots (cpu_2d/main.cpp:944-950), in GTEPS.
     One rank's share of a 4 x 4 grid: the g500_hip driver's emulated grid
     (device exchange, codec on the fold), rank (0, 0)'s share timed.
 
-Every measured search is then validated on the device (g5_validate_device:
-the checks of validate/validate.h:249-491 with its TEPS numerator), and four
-batched lanes' predecessor maps are compared bit for bit with the single-root
-search of the same root; a failure aborts the run.
+The batched time runs from the start to the 64 final predecessor and level
+maps in HBM (parent int32[n][64], level uint8[n][64], original ids), as the
+reference's runBFS window ends with the predecessors final
+(cpu_2d/main.cpp:419-462); the single-root time likewise ends with its
+parent and level arrays final.  Every measured search is then validated on
+the device (g5_validate_device: the checks of validate/validate.h:249-491
+with its TEPS numerator), and every batched lane's predecessor and level
+maps are compared word for word, on the device, with the single-root search
+of the same root; a failure aborts the run.
+
+Shapes: the timed roots are the first 64 of the benchmark's draw; the
+batched search's shapes come from tuned_shapes.json for this graph's scale
+and degree histogram, or from an autotune on the next 64 draws (disjoint
+from the timed set), which is then persisted.
 
 roofline.traffic: HBM bytes per launch of the dominant kernel from two
 rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run as child processes before
 this process touches the GPU, each scaled by what the counter reads for a
 plain copy kernel of known size; null when rocprofv3 is unavailable.  A
-first child (no profiler) autotunes the batched search's shapes; the counter
-passes and the timed run all use those shapes, so the kernels measured are
-the kernels timed.
+first child (no profiler) finds the shapes; the counter passes and the timed
+run all use those shapes, so the kernels measured are the kernels timed.
 
     python bench.py [--config c4|c2|c3] [--steps K] [--warmup W] [--scale S]
 """
@@ -160,17 +169,41 @@ def cpu_baseline_codec(g, scale, roots, nedge_gpu, budget_s=12.0):
 
 # ---------------------------------------------------------------- PMC traffic
 
+def root_sets(g, scale, n):
+    """The timed search keys (the first n of the benchmark's draw,
+    cpu_2d/main.cpp:351-411) and a disjoint set for tuning (the next n draws
+    of the same generator); the timed set again when the graph has too few
+    eligible vertices."""
+    allr = g.select_roots(1 << scale, 2 * n)
+    timed = allr[:n]
+    tune = allr[n:2 * n] if len(allr) >= 2 * n else timed
+    return timed, tune
+
+
 def tune_probe(args):
-    """Child process (no profiler): autotune the batched search's shapes and
-    print them as one JSON line."""
+    """Child process (no profiler): the batched search's shapes for this
+    graph -- the persisted ones for its scale and degree histogram, or an
+    autotune on the tuning root set (disjoint from the timed set), then
+    persisted -- printed as one JSON line."""
     import gpugraph500_amd as g5
+    from gpugraph500_amd import shapes
     g5.init(0)
     g = g5.Graph.kronecker(args.scale)
-    roots = g.select_roots(1 << args.scale, args.roots)
+    timed, tune = root_sets(g, args.scale, args.roots)
+    key = shapes.graph_key(g, args.scale)
+    saved = shapes.load(key)
+    if saved is not None:
+        print(json.dumps({"params": saved["params"], "candidates_run": 0, "best_ms": saved.get("best_ms"),
+                          "source": f"persisted ({shapes.PATH}, key {key})", "tuned_on": saved.get("tuned_on")}),
+              flush=True)
+        return 0
     m = g5.MSBFS(g)
-    m.run(roots)
-    best_ms, trials, params = m.autotune(roots)
-    print(json.dumps({"params": params, "candidates_run": trials, "best_ms": best_ms}), flush=True)
+    m.run(tune)
+    best_ms, trials, params = m.autotune(tune)
+    shapes.save(key, {"params": params, "best_ms": best_ms, "tuned_on": "roots 65-128 of the benchmark's draw"})
+    print(json.dumps({"params": params, "candidates_run": trials, "best_ms": best_ms,
+                      "source": f"autotuned now, persisted under key {key}",
+                      "tuned_on": "roots 65-128 of the benchmark's draw"}), flush=True)
     return 0
 
 
@@ -280,8 +313,8 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     g = g5.Graph.kronecker(args.scale)
     log(f"graph: scale {args.scale}, {g.nglobal} vertices, {g.nnz} nonzeros, "
         f"built on device in {time.perf_counter() - t0:.2f} s")
-    roots = g.select_roots(1 << args.scale, args.roots)
-    log(f"roots: {len(roots)}")
+    roots, tune_roots = root_sets(g, args.scale, args.roots)
+    log(f"roots: {len(roots)} timed, {len(tune_roots)} for tuning (disjoint: {len(set(roots) & set(tune_roots)) == 0})")
 
     # ---- single-root searches (each root once after warmup) ----
     b = g5.BFS(g)
@@ -319,14 +352,18 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     tuned = None
     if probe_tuned:  # the shapes the counter passes measured
         m.tune(**probe_tuned["params"])
-        tuned = dict(probe_tuned, source="tune probe (child process, before the counter passes)")
-        log(f"shapes from the tune probe: {probe_tuned['params']}")
+        tuned = dict(probe_tuned, probe="tune probe (child process, before the counter passes)")
+        log(f"shapes from the tune probe: {probe_tuned['params']} ({probe_tuned.get('source')})")
     elif args.autotune == "on":
         t0 = time.perf_counter()
-        best_ms, trials, params = m.autotune(roots)
+        best_ms, trials, params = m.autotune(tune_roots)
         tuned = {"params": params, "candidates_run": trials, "best_ms": best_ms,
-                 "seconds": time.perf_counter() - t0, "source": "this process"}
+                 "seconds": time.perf_counter() - t0, "source": "this process",
+                 "tuned_on": "roots 65-128 of the benchmark's draw"}
         log(f"autotune: {params} after {trials} candidates, {best_ms:.2f} ms")
+    if tuned is not None:
+        tuned["timed_on"] = "roots 1-64 of the benchmark's draw"
+        tuned["disjoint"] = bool(len(set(roots.tolist()) & set(tune_roots.tolist())) == 0)
     log(f"msbfs degree bins (workgroup, wavefront, thread per vertex): {m.bins()}")
     m.profile(True)            # one run as direct launches with per-kernel events
     m.run(roots)
@@ -351,8 +388,9 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
                                     "gbs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else 0.0}
                                 for k, v in kst.items()}}
 
-    # ---- timed steps: one batched search of all roots per step ----
-    step_ms, dev_ms = [], []
+    # ---- timed steps: one batched search of all roots per step, from the
+    # start to the 64 final predecessor and level maps in HBM ----
+    step_ms, dev_ms, search_ms, fin_ms = [], [], [], []
     for i in range(args.warmup + args.steps):
         g5.lib().g5_device_sync()
         t = time.perf_counter()
@@ -361,6 +399,10 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         if i >= args.warmup:
             step_ms.append((time.perf_counter() - t) * 1e3)
             dev_ms.append(ms)
+            a, b_ = m.times()
+            search_ms.append(a)
+            fin_ms.append(b_)
+    levels_final = m.maps()[3]
     ne_b = m.nedge().astype(np.float64)
     assert np.array_equal(ne_b, nedge), "batched and single-root TEPS numerators differ"
     validation = None
@@ -373,27 +415,34 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         validation = {"searches": len(val_ms), "failures": val_fail, "mean_ms": float(np.mean(val_ms)),
                       "what": "every single-root and batched result, device validator (validate.h:249-491)"}
         log(f"validation: {len(val_ms)} results, {val_fail} failures, {np.mean(val_ms):.1f} ms each")
-        # bit-exact agreement of batched lanes with the single-root search
-        # (predecessor maps, minimum-id parents) on four lanes
+        # bit-exact agreement of every batched lane with the single-root
+        # search of the same root: predecessor and level maps, compared on
+        # the device
         b2 = g5.BFS(g)
         d_single = g5.DeviceArray(g.n_rows, np.int64)
-        same_lanes = []
-        for lane in sorted({0, len(roots) // 3, (2 * len(roots)) // 3, len(roots) - 1}):
-            b2.run(int(roots[lane]))
+        l_single, l_batch = g5.DeviceArray(g.n_rows, np.int32), g5.DeviceArray(g.n_rows, np.int32)
+        differ = []
+        for lane, r in enumerate(roots.tolist()):
+            b2.run(r)
             b2.pred_device(d_single)
+            b2.level_device(l_single)
             m.pred_device(lane, d_pred)
-            same_lanes.append((lane, bool(np.array_equal(d_single.to_host(), d_pred.to_host()))))
-        b2.free()
-        d_single.free()
-        validation["batched_equals_single_root"] = {str(k): v for k, v in same_lanes}
-        log(f"batched == single-root predecessor maps on lanes {[k for k, _ in same_lanes]}: "
-            f"{all(v for _, v in same_lanes)}")
+            m.level_device(lane, l_batch)
+            if g5.diff_u32(d_single, d_pred, 2 * g.n_rows) or g5.diff_u32(l_single, l_batch, g.n_rows):
+                differ.append(lane)
+        for a in (b2, d_single, l_single, l_batch):
+            a.free()
+        validation["batched_equals_single_root"] = {
+            "lanes_compared": len(roots), "lanes_differing": differ,
+            "what": "every lane's predecessor and level maps against the single-root search of its root, "
+                    "compared word for word on the device"}
+        log(f"batched == single-root predecessor and level maps: {len(roots) - len(differ)} of {len(roots)} lanes")
         val.free()
         d_pred.free()
         if val_fail:
             raise SystemExit(f"validation failed for {val_fail} of {len(val_ms)} results")
-        if not all(v for _, v in same_lanes):
-            raise SystemExit("batched and single-root predecessor maps differ")
+        if differ:
+            raise SystemExit(f"batched and single-root maps differ on lanes {differ}")
     t_batch = float(np.mean(dev_ms)) / 1e3
     batched_teps = ne_b / (t_batch / len(roots))
     value = hmean(batched_teps) / 1e9
@@ -421,6 +470,13 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         "batched_over_single": value / single_h,
         "aggregate_gteps": aggregate,
         "batched_ms": t_batch * 1e3,
+        "batched_ms_split": {"search": float(np.mean(search_ms)), "finalize": float(np.mean(fin_ms)),
+                             "what": "search: start to the last level (the kernels that reach a pair write its "
+                                     "parent, as an original id at the vertex's original index, and its level); "
+                                     "finalize: the pass that marks the used lanes' unreached slots"},
+        "batched_output": {"layout": "parent int32[n][64] and level uint8[n][64] by original vertex id, "
+                                     "slot = lane, in HBM (g5_msbfs_maps)",
+                           "levels_final": bool(levels_final)},
         "single_root_mean_ms": float(np.mean(single_ms)),
         "single_root_kernel": single_kernel,
         "roofline": roofline,

@@ -57,6 +57,7 @@ def lib():
     _sig(L, "g5_block_bounds", I, [c_i64, I, I, I, I, P, P, P, P])
     _sig(L, "g5_graph_bounds", I, [P, P, P])
     _sig(L, "g5_graph_gdeg", I, [P, P])
+    _sig(L, "g5_graph_degree_hist", I, [P, P, I])
     _sig(L, "g5_memcpy_d2d", I, [P, P, c_sz])
     _sig(L, "g5_fill_u64", I, [P, c_i64, c_u64])
     _sig(L, "g5_bitmap_to_queue", I, [P, c_i64, P, ctypes.POINTER(c_i64)])
@@ -92,6 +93,11 @@ def lib():
     _sig(L, "g5_msbfs_run", I, [P, P, I, ctypes.POINTER(c_i32), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_msbfs_result", I, [P, I, P, P])
     _sig(L, "g5_msbfs_pred_device", I, [P, I, P])
+    _sig(L, "g5_msbfs_level_device", I, [P, I, P])
+    _sig(L, "g5_msbfs_maps", I, [P, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), P, ctypes.POINTER(I)])
+    _sig(L, "g5_msbfs_times", I, [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_bfs_level_device", I, [P, P])
+    _sig(L, "g5_diff_u32", I, [P, P, c_i64, ctypes.POINTER(c_i64)])
     _sig(L, "g5_validator_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_validator_free", I, [P])
     _sig(L, "g5_validate", I, [P, c_i64, c_i64, P, ctypes.POINTER(I), ctypes.POINTER(c_i64), P,
@@ -137,6 +143,16 @@ def _p(a):
 
 def init(device=0):
     check(lib().g5_init(device))
+
+
+def diff_u32(a, b, n_words):
+    """Positions where two device buffers (DeviceArray or raw device
+    addresses) differ, compared as n_words uint32 words on the device."""
+    pa = a.ptr if hasattr(a, "ptr") else c_vp(a)
+    pb = b.ptr if hasattr(b, "ptr") else c_vp(b)
+    out = c_i64()
+    check(lib().g5_diff_u32(pa, pb, int(n_words), ctypes.byref(out)))
+    return out.value
 
 
 class DeviceArray:
@@ -220,6 +236,12 @@ class Graph:
         found = ctypes.c_int()
         check(lib().g5_select_roots(self.h, vertices, nroots, _p(roots), ctypes.byref(found)))
         return roots[:found.value].copy()
+
+    def degree_hist(self, nbins=40):
+        """Rows per degree class: [0] no neighbours, [k] 2^(k-1) <= degree < 2^k."""
+        out = np.zeros(nbins, np.int64)
+        check(lib().g5_graph_degree_hist(self.h, _p(out), nbins))
+        return out
 
     def download(self):
         rp = np.empty(self.n_rows + 1, np.int64)
@@ -305,6 +327,10 @@ class BFS:
     def pred_device(self, out):
         """The predecessor map (global ids) into the DeviceArray `out` (int64[n_cols])."""
         check(lib().g5_bfs_pred_device(self.h, out.ptr))
+
+    def level_device(self, out):
+        """The level map into the DeviceArray `out` (int32[n_cols])."""
+        check(lib().g5_bfs_level_device(self.h, out.ptr))
 
     RUN_CODEC, RUN_TOPDOWN = 1, 2
     KERNELS = ("k_expand", "k_bottom_up")
@@ -394,6 +420,25 @@ class MSBFS:
     def pred_device(self, lane, out):
         """Lane `lane`'s predecessor map into the DeviceArray `out` (int64[n])."""
         check(lib().g5_msbfs_pred_device(self.h, lane, out.ptr))
+
+    def level_device(self, lane, out):
+        """Lane `lane`'s level map into the DeviceArray `out` (int32[n])."""
+        check(lib().g5_msbfs_level_device(self.h, lane, out.ptr))
+
+    def maps(self):
+        """The last batch run's final maps as they lie in HBM: (device pointer
+        of parent int32[n][64], device pointer of level uint8[n][64], lane of
+        each root, levels_final)."""
+        dp, dl, fin = c_vp(), c_vp(), ctypes.c_int()
+        lane_of = np.zeros(max(self.nroots, 1), np.int32)
+        check(lib().g5_msbfs_maps(self.h, ctypes.byref(dp), ctypes.byref(dl), _p(lane_of), ctypes.byref(fin)))
+        return dp.value, dl.value, lane_of[:self.nroots], bool(fin.value)
+
+    def times(self):
+        """(search ms, finalising pass ms) of the last batch run."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        check(lib().g5_msbfs_times(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     # ---- shape parameters ----
     TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3, "lane_ratio": 4}

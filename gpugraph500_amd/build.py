@@ -78,15 +78,27 @@ def drivers():
             if f.startswith("g500_") and f.endswith(".cpp")}
 
 
+TESTS = os.path.join(ROOT, "tests")
+HIPSIMD_TEST = os.path.join(TESTS, "bin", "hipsimd_gpu")
+
+
 def build_host():
     """The C++ host side (HipBFS, the emulated 2D grid, the serving engine and
     the g500_* drivers): plain g++ against the C ABI of libg5hip.so, found
-    next to the binaries."""
+    next to the binaries.  Also the GPU tests' C++ driver of the codec schema
+    (tests/hipsimd_gpu.cpp -> tests/bin/hipsimd_gpu), so that the tests run
+    on a GPU box without building anything there."""
+    cxx = os.environ.get("CXX", "g++")
     for exe, main in drivers().items():
         tmp = exe + ".tmp"
-        _run([os.environ.get("CXX", "g++")] + HOST_FLAGS + host_sources() + [main] +
-             ["-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN"])
+        _run([cxx] + HOST_FLAGS + host_sources() + [main] + ["-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, exe)
+    src = os.path.join(TESTS, "hipsimd_gpu.cpp")
+    if os.path.exists(src):
+        os.makedirs(os.path.dirname(HIPSIMD_TEST), exist_ok=True)
+        tmp = HIPSIMD_TEST + ".tmp"
+        _run([cxx] + HOST_FLAGS + [src, "-o", tmp, "-L" + HERE, "-lg5hip", "-Wl,-rpath,$ORIGIN/../../gpugraph500_amd"])
+        os.replace(tmp, HIPSIMD_TEST)
     return DRIVER
 
 

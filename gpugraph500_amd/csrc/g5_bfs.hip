@@ -711,6 +711,15 @@ extern "C" int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred)
     return G5_OK;
 }
 
+extern "C" int g5_bfs_level_device(g5_bfs *b, int32_t *d_level)
+{
+    G5_REQUIRE(b && d_level, G5_E_ARG, "g5_bfs_level_device: null argument");
+    G5_REQUIRE(b->depth >= 0, G5_E_STATE, "g5_bfs_level_device: no search");
+    G5_HIP(hipMemcpyAsync(d_level, b->d_level, (size_t)b->g->n_cols * 4, hipMemcpyDeviceToDevice, b->s));
+    G5_HIP(hipStreamSynchronize(b->s));
+    return G5_OK;
+}
+
 extern "C" int g5_bfs_nedge(g5_bfs *b, int64_t *nedge)
 {
     G5_REQUIRE(b && nedge, G5_E_ARG, "g5_bfs_nedge: null argument");

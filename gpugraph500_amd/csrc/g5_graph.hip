@@ -675,6 +675,45 @@ extern "C" int g5_graph_gdeg(const g5_graph *g, int64_t *gdeg)
     return G5_OK;
 }
 
+namespace {
+// hist[0] = rows without neighbours; hist[k] = rows with 2^(k-1) <= degree
+// < 2^k, the last bin open-ended.  Per-workgroup counts in LDS, then one
+// atomic per non-empty bin.
+__global__ __launch_bounds__(256) void k_degree_hist(const int64_t *__restrict__ rowptr, int64_t n, int nbins,
+                                                     unsigned long long *__restrict__ hist)
+{
+    __shared__ unsigned long long s_h[64];
+    if (threadIdx.x < 64) s_h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = rowptr[v + 1] - rowptr[v];
+        const int k = d == 0 ? 0 : min(nbins - 1, 64 - __clzll((long long)d));
+        atomicAdd(&s_h[k], 1ull);
+    }
+    __syncthreads();
+    if (threadIdx.x < nbins && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
+}
+}  // namespace
+
+extern "C" int g5_graph_degree_hist(const g5_graph *g, int64_t *hist, int nbins)
+{
+    G5_REQUIRE(g && hist && nbins >= 2 && nbins <= 64, G5_E_ARG, "g5_graph_degree_hist: bad arguments");
+    unsigned long long *d = nullptr;
+    G5_HIP(hipMalloc(&d, 64 * 8));
+    hipError_t e = g5::fill_u64((uint64_t *)d, 64, 0, nullptr);
+    if (e == hipSuccess && g->n_rows) {
+        hipLaunchKernelGGL(k_degree_hist, g5::grid_for(g->n_rows, 256, 4096), 256, 0, nullptr,
+                           (const int64_t *)g->d_rowptr, g->n_rows, nbins, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h[64];
+    if (e == hipSuccess) e = hipMemcpy(h, d, 64 * 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    G5_HIP(e);
+    for (int k = 0; k < nbins; ++k) hist[k] = (int64_t)h[k];
+    return G5_OK;
+}
+
 extern "C" int g5_graph_bounds(const g5_graph *g, int64_t *row_start, int64_t *col_start)
 {
     G5_REQUIRE(g && row_start && col_start, G5_E_ARG, "g5_graph_bounds: null argument");

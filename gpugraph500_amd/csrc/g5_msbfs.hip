@@ -45,15 +45,24 @@
 // sequence is captured once into a hipGraph per frontier parity and
 // replayed; the level number lives in device memory.
 //
-// Output in HBM: parent int32[n][64] (original ids), vertex-major in the
-// relabelled order, valid where the vertex's visited bit for that lane is
-// set.  The pull kernels store each vertex's new parents with one masked,
-// non-temporal store of its 256 B record: the records are not read again
-// during the search, so they should not displace frontier words from the
-// caches.  The search stores no levels: like the reference validator, which
-// derives depths from the predecessor map (validate/validate.h:107-232),
-// g5_msbfs_result computes a root's levels from its parent tree (pointer
-// jumping) when it gathers the per-root arrays in original ids.
+// Output in HBM, final when g5_msbfs_run returns: the 64 predecessor maps
+// as one vertex-major array parent int32[n][64] and the 64 level maps as
+// level uint8[n][64], both indexed by the ORIGINAL vertex id, slot = lane.
+// A parent is an original id (root -> root, -1 unreached); a level byte is
+// the distance from the root (255 unreached).  The kernels that find a
+// (vertex, root) pair write both at once: the pull kernels translate a
+// parent's relabelled id through new2old once per distinct parent (a hit),
+// not once per slot, and store each vertex's new slots with one masked,
+// non-temporal store of its 256 B record and one of its 64 B level record
+// (the records are not read again during the search, so they should not
+// displace frontier words from the caches).  Slots of pairs the search did
+// not reach still hold an earlier search's values until k_ms_finalize, the
+// last pass of a batch run, sets them; edgeless vertices' records stay at
+// the unreached values except where a root sat, which the next run clears.
+// Levels need one byte: a search deeper than 254 levels reports its level
+// bytes as not final, and g5_msbfs_result then derives levels from the
+// parent tree by pointer jumping, as the reference validator derives
+// depths from the predecessor map (validate/validate.h:107-232).
 #include <algorithm>
 #include <vector>
 
@@ -107,6 +116,9 @@ static_assert(kPullLanes < kCntStride, "counter stripe too small");
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
 constexpr int kHostSlots = kCntStride + 64;  // stripe 0, then the folded per-lane edges
+// edgeless roots remembered between batch runs (beyond: every edgeless
+// vertex's record is cleared)
+constexpr int64_t kDirtyCap = 1 << 16;
 static_assert(kStripes * kCntStride >= kHostSlots, "per-lane edges do not fit the folded stripes");
 
 // The degree-ordered copy of the block.
@@ -132,10 +144,10 @@ struct g5_msbfs {
     uint64_t *d_act = nullptr;           // two n/64-word active bitmaps (parity a, b)
     uint32_t *d_fq = nullptr, *d_nq = nullptr;  // push: frontier queue, newly reached queue
     int64_t *d_qoff = nullptr;                  // push: frontier degree prefix
-    // parent records [n][64] by relabelled vertex: slot r of v holds root r's
-    // parent of v, as an original id (roots, push levels) or, from a pull
-    // level, as ~u for its relabelled id u (see parent_orig)
+    // records [n][64] by original vertex id: slot r of v holds lane r's
+    // parent of v (original id) and its level byte
     int32_t *d_parent = nullptr;
+    uint8_t *d_level = nullptr;
     int64_t *d_cnt = nullptr, *h_cnt = nullptr;
     int64_t *d_lane = nullptr;  // [kLaneStripes][64] per-lane frontier edges
     int64_t *d_scratch = nullptr;
@@ -186,32 +198,45 @@ struct g5_msbfs {
     // degrees [130, 194); batch run: the roots' reach [0, 64)
     int64_t *d_join = nullptr;
     unsigned long long *d_sums = nullptr;  // leave: checksums [0, 64), generated-degree sums [64, 128)
+    // the last batch run: device time of the search and of its finalising
+    // pass (ev_fin between them), and whether its level bytes are final
+    // (depth below 255)
+    hipEvent_t ev_fin = nullptr;
+    float search_ms = 0, finalize_ms = 0;
+    bool levels_final = false;
+    int32_t run_levels = 0;
+    // roots since the last batch run's start (batch roots and joined roots,
+    // original ids): an edgeless one left its slot in a record that must read
+    // "unreached" again; d_dirty holds them on the device for the clearing pass
+    std::vector<int64_t> dirty;
+    int64_t *d_dirty = nullptr;
+    // continuous mode: each lane's root and the levels run since it joined
+    int64_t lane_root[64] = {};
+    int64_t lane_steps[64] = {};
 };
 
 namespace {
 
-// A pull level stores a parent as ~u, u its relabelled id (a negative slot),
-// and needs no lookup of the original id, a random 4 B gather per hit; push
-// levels and the roots store original ids, which the push's atomicMin pass
-// compares (a slot is written by one level only, the level that reaches v in
-// that lane, and within a level the pulled and pushed lanes are disjoint).
-__device__ __forceinline__ int32_t pulled(uint32_t u) { return ~(int32_t)u; }
+// Every slot holds an original id and is written by one level only, the
+// level that reaches its (vertex, root) pair; within a level the pulled and
+// pushed lanes are disjoint, and the push's atomicMin pass compares
+// original ids.
 
-// the original id of a parent slot's value
-__device__ __forceinline__ int32_t parent_orig(int32_t p, const uint32_t *__restrict__ new2old)
-{
-    return p < 0 ? (int32_t)new2old[~p] : p;
-}
-
-// parent slots of vertex v for the roots in `bits` := u
-__device__ __forceinline__ void write_parents(int32_t *__restrict__ parent, int64_t v, uint64_t bits, uint32_t u)
+// slots `bits` of the record of vertex o (original id): parent uo, level lv
+__device__ __forceinline__ void write_slots(int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t o,
+                                            uint64_t bits, uint32_t uo, uint8_t lv)
 {
     while (bits) {
         const int r = __ffsll((long long)bits) - 1;
-        parent[v * 64 + r] = (int32_t)u;
+        parent[o * 64 + r] = (int32_t)uo;
+        level[o * 64 + r] = lv;
         bits &= bits - 1;
     }
 }
+
+// the level byte of the pairs the running level reaches (the level number
+// lives in device memory, so captured level graphs replay unchanged)
+__device__ __forceinline__ uint8_t level_byte(const int64_t *__restrict__ cnt) { return (uint8_t)cnt[kDepth]; }
 
 // lane-to-wave broadcast; l must be wave-uniform
 __device__ __forceinline__ uint32_t lane_read(uint32_t x, int l)
@@ -261,10 +286,12 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 }
 
 // One thread per vertex (degree <= thread_max): a sequential ascending scan.  Each
-// thread keeps its first kHits hits (parent id, new lanes) in registers; once
-// the wave's 64 vertices are scanned, the wave writes them out vertex by
-// vertex, lane r storing root r's parent, so every store instruction covers
-// one vertex's contiguous 256 B parent record instead of 64 scattered slots.
+// thread keeps its first kHits hits (parent's relabelled id, new lanes) in
+// registers; once the wave's 64 vertices are scanned, each thread translates
+// its kept parents to original ids (one gather per hit, not per slot), and
+// the wave writes them out vertex by vertex, lane r storing root r's parent
+// and level, so every store instruction covers one vertex's contiguous 256 B
+// parent record (or 64 B level record) instead of 64 scattered slots.
 // Hits beyond kHits (rare: they need more than kHits distinct first-hit
 // neighbours) are stored directly.  The scan loads kThreadUnroll neighbours
 // and their frontier words before using any of them, so that several
@@ -282,6 +309,15 @@ constexpr int kHits = 4;
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4; 12 after the short first step: the same
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
 
+// the record outputs of a pull level: parent and level records by original
+// id, the level byte of the pairs it reaches, and new2old
+struct PullOut {
+    int32_t *__restrict__ parent;
+    uint8_t *__restrict__ level;
+    const uint32_t *__restrict__ new2old;
+    uint8_t lv;
+};
+
 // One step of a thread's scan: neighbours [j, j + K) of the row ending at e
 // (kHead: the row's first two, j at the row's start, from the head array).
 template <int K, bool kActive, bool kHead>
@@ -290,7 +326,7 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
                                                     int64_t &ngath, const uint32_t *__restrict__ cols,
                                                     const uint64_t *__restrict__ head,
                                                     const uint64_t *__restrict__ frontier,
-                                                    const uint64_t *__restrict__ active, int32_t *__restrict__ parent)
+                                                    const uint64_t *__restrict__ active, const PullOut &out)
 {
     static_assert(!kHead || K == 2, "the head array holds two neighbours per vertex");
     uint32_t u[K];
@@ -315,17 +351,16 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
         const uint64_t f = fr[k] & unv & ~found;
         if (!f) continue;
         found |= f;
-        const uint32_t uo = (uint32_t)pulled(u[k]);
         if (nh < kHits) {
 #pragma unroll
             for (int h = 0; h < kHits; ++h)
                 if (h == nh) {
-                    hu[h] = uo;
+                    hu[h] = u[k];
                     hf[h] = f;
                 }
             ++nh;
         } else {
-            write_parents(parent, v, f, uo);
+            write_slots(out.parent, out.level, out.new2old[v], f, out.new2old[u[k]], out.lv);
         }
     }
     return j + K;
@@ -336,11 +371,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
+    uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
+    uint64_t *__restrict__ act_next)
 {
     __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const PullOut out{parent, level, new2old, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
     // wave-uniform loop: the wave takes 64 consecutive vertices per step, one
@@ -367,10 +404,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                 int64_t j = b;
                 if (j < e)
                     j = thread_scan_step<kThreadFirst, kActive, true>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
-                                                                      head, frontier, active, parent);
+                                                                      head, frontier, active, out);
                 while (j < e && found != unv)
                     j = thread_scan_step<kThreadUnroll, kActive, false>(j, e, v, unv, found, nh, hu, hf, ngath,
-                                                                        cols, head, frontier, active, parent);
+                                                                        cols, head, frontier, active, out);
                 j = j < e ? j : e;
                 if (found) {
                     visited[v] = vis | found;
@@ -386,6 +423,15 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
         live |= found;
         const uint64_t word = __ballot(found != 0);
         if (lane == 0) act_next[(first_v + base) >> 6] = word;
+        // the kept parents' original ids and the vertex's own, all gathers
+        // issued before the write-out
+        uint32_t o = 0;
+        if (nh) {
+            o = new2old[v];
+#pragma unroll
+            for (int k = 0; k < kHits; ++k)
+                if (k < nh) hu[k] = new2old[hu[k]];
+        }
         // write-out: one vertex per step, lane = root; the vertex's kept
         // hits are disjoint and cover its found lanes but the spilled ones
         uint64_t todo = __ballot(nh != 0);
@@ -405,7 +451,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                     mine = true;
                 }
             }
-            if (mine) __builtin_nontemporal_store((int32_t)p, &parent[(int64_t)lane_read(v, src) * 64 + lane]);
+            if (mine) {
+                const int64_t slot = (int64_t)lane_read(o, src) * 64 + lane;
+                __builtin_nontemporal_store((int32_t)p, &parent[slot]);
+                __builtin_nontemporal_store(out.lv, &level[slot]);
+            }
         }
     }
     live = g5_wave_or(live);
@@ -440,10 +490,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
+    uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
+    uint64_t *__restrict__ act_next)
 {
-    __shared__ int32_t s_par[kBlock / 64][64];
-    int32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
+    __shared__ uint32_t s_par[kBlock / 64][64];
+    uint32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
+    const uint8_t lv = level_byte(cnt_all);
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
@@ -484,12 +536,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
                 if (__ballot(fresh != 0) == 0) continue;
                 const uint64_t prev = lanes_excl_or(fresh);
                 uint64_t first = fresh & ~prev;
-                if (first) {
-                    const int32_t uo = pulled(u[k]);
-                    while (first) {
-                        sp[__ffsll((long long)first) - 1] = uo;
-                        first &= first - 1;
-                    }
+                while (first) {
+                    sp[__ffsll((long long)first) - 1] = u[k];
+                    first &= first - 1;
                 }
                 found |= __shfl(prev | fresh, 63, 64);
             }
@@ -500,7 +549,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (found >> lane & 1) __builtin_nontemporal_store(sp[lane], &parent[(int64_t)v * 64 + lane]);
+            if (found >> lane & 1) {
+                // original ids: lanes with the same parent gather the same word
+                const int64_t slot = (int64_t)new2old[v] * 64 + lane;
+                __builtin_nontemporal_store((int32_t)new2old[sp[lane]], &parent[slot]);
+                __builtin_nontemporal_store(lv, &level[slot]);
+            }
             __builtin_amdgcn_wave_barrier();
         }
         if (lane == 0) {
@@ -527,11 +581,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
     const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt_all, int bin, uint64_t *__restrict__ act_next)
+    uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
+    uint64_t *__restrict__ act_next)
 {
     __shared__ uint64_t s_wave[kBlockUnroll][kBlock / 64];
     __shared__ uint64_t s_vis;
-    __shared__ int32_t s_par[64];
+    __shared__ uint32_t s_par[64];
+    const uint8_t lv = level_byte(cnt_all);
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
@@ -589,12 +645,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
                     all |= t;
                 }
                 uint64_t first = f[k] & ~(in_wave[k] | before);
-                if (first) {
-                    const int32_t uo = pulled(u[k]);
-                    while (first) {
-                        s_par[__ffsll((long long)first) - 1] = uo;
-                        first &= first - 1;
-                    }
+                while (first) {
+                    s_par[__ffsll((long long)first) - 1] = u[k];
+                    first &= first - 1;
                 }
                 earlier |= all;
             }
@@ -602,7 +655,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             __syncthreads();
         }
         // s_par is complete: the round's second barrier follows its writes
-        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) __builtin_nontemporal_store(s_par[threadIdx.x], &parent[(int64_t)v * 64 + threadIdx.x]);
+        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) {
+            const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
+            __builtin_nontemporal_store((int32_t)new2old[s_par[threadIdx.x]], &parent[slot]);
+            __builtin_nontemporal_store(lv, &level[slot]);
+        }
         if (threadIdx.x == 0) {
             next[v] = found;
             if (found) {
@@ -630,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 // level).
 // a push level clears next entry by entry when the list is below n / kListedFrac
 constexpr int64_t kListedFrac = 32;
-constexpr int kPushTile = 1024;  // 16 KB of LDS per block (2048: 32 KB, 4 waves per SIMD)
+constexpr int kPushTile = 1024;  // 20 KB of LDS per block (2048: 40 KB, 4 waves per SIMD)
 constexpr int kPushGrid = 4096;
 constexpr int kPushVertGrid = 2048;
 
@@ -662,14 +719,17 @@ __global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__r
     }
 }
 
-// one pushed edge: u (relabelled id) -> v.  mark (first pass only, when
-// given): the edge that makes next[v] non-zero sets v's bit in mark, the
-// next frontier's active bitmap.
+// one pushed edge: u (relabelled id, uo its original id) -> v.  mark (first
+// pass only, when given): the edge that makes next[v] non-zero sets v's bit
+// in mark, the next frontier's active bitmap.  The first pass writes the
+// level bytes of the pairs it sets (every candidate parent of a pair is of
+// the same level).
 template <bool kMin>
-__device__ __forceinline__ void push_edge(uint32_t u, uint32_t v, const uint32_t *__restrict__ new2old,
+__device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, const uint32_t *__restrict__ new2old,
                                           const uint64_t *__restrict__ visited,
                                           const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
-                                          int32_t *__restrict__ parent, uint64_t lm, uint64_t *__restrict__ mark)
+                                          int32_t *__restrict__ parent, uint8_t *__restrict__ level, uint8_t lv,
+                                          uint64_t lm, uint64_t *__restrict__ mark)
 {
     const uint64_t f = frontier[u] & lm & ~visited[v];
     if (!kMin) {
@@ -680,36 +740,38 @@ __device__ __forceinline__ void push_edge(uint32_t u, uint32_t v, const uint32_t
         const uint64_t fresh = f & ~old;
         // the edge that sets a (v, root) bit first stores its source as the
         // pair's first parent candidate; the kMin pass lowers it
-        if (fresh) write_parents(parent, v, fresh, new2old[u]);
+        if (fresh) write_slots(parent, level, new2old[v], fresh, uo, lv);
         if (mark && old == 0) atomicOr((unsigned long long *)&mark[v >> 6], 1ull << (v & 63));
     } else if (f) {
         uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
-        const int32_t uo = (int32_t)new2old[u];
+        int32_t *rec = &parent[(int64_t)new2old[v] * 64];
         while (bits) {
             const int r = __ffsll((long long)bits) - 1;
-            int32_t *slot = &parent[(int64_t)v * 64 + r];
-            if (*slot > uo) atomicMin(slot, uo);
+            int32_t *slot = &rec[r];
+            if (*slot > (int32_t)uo) atomicMin(slot, (int32_t)uo);
             bits &= bits - 1;
         }
     }
 }
 
 // A tile's sources (at most kPushTile: every queue vertex has an edge) are
-// staged in LDS first -- their first edge's offset within the tile, their id
-// and their row start less their queue offset -- so that each edge finds its
-// source by a binary search in LDS rather than in the global prefix.
+// staged in LDS first -- their first edge's offset within the tile, their
+// relabelled and original ids and their row start less their queue offset --
+// so that each edge finds its source by a binary search in LDS rather than
+// in the global prefix.
 template <bool kMin>
 __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    int64_t *__restrict__ cnt, uint64_t lm, uint64_t *__restrict__ mark)
+    uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm, uint64_t *__restrict__ mark)
 {
     __shared__ int64_t s_lo, s_hi;
     __shared__ int32_t s_off[kPushTile];
-    __shared__ uint32_t s_u[kPushTile];
+    __shared__ uint32_t s_u[kPushTile], s_uo[kPushTile];
     __shared__ int64_t s_rb[kPushTile];
     const int64_t qn = cnt[kFqLen], E = cnt[kFqEdges];
+    const uint8_t lv = level_byte(cnt);
     for (int64_t t0 = blockIdx.x * (int64_t)kPushTile; t0 < E; t0 += (int64_t)gridDim.x * kPushTile) {
         const int64_t t1 = min(E, t0 + (int64_t)kPushTile);
         if (threadIdx.x == 0) {
@@ -724,6 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                 const uint32_t u = fq[lo + j];
                 const int64_t q = qoff[lo + j];
                 s_u[j] = u;
+                s_uo[j] = new2old[u];
                 s_rb[j] = rowptr[u] - q;
                 s_off[j] = (int32_t)max(q - t0, (int64_t)0);
             }
@@ -738,14 +801,15 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                     else b = mid - 1;
                 }
                 j = a;
-                push_edge<kMin>(s_u[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, lm, mark);
+                push_edge<kMin>(s_u[j], s_uo[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, level,
+                                lv, lm, mark);
             }
         } else {
             for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
                 const int64_t si = fq_src(qoff, lo, hi, e);
                 const uint32_t u = fq[si];
-                push_edge<kMin>(u, cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next, parent, lm,
-                                mark);
+                push_edge<kMin>(u, new2old[u], cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next,
+                                parent, level, lv, lm, mark);
             }
         }
         __syncthreads();
@@ -949,8 +1013,8 @@ __global__ __launch_bounds__(256) void k_root_reach(const int64_t *__restrict__ 
 __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const uint32_t *__restrict__ old2new,
                            const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
                            uint64_t *__restrict__ frontier, uint64_t *__restrict__ active,
-                           int32_t *__restrict__ parent, int64_t *__restrict__ cnt, int32_t *__restrict__ lane_of,
-                           const int64_t *__restrict__ reach)
+                           int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t *__restrict__ cnt,
+                           int32_t *__restrict__ lane_of, const int64_t *__restrict__ reach)
 {
     __shared__ int64_t s_key[64];
     const int r = threadIdx.x;
@@ -967,7 +1031,8 @@ __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const 
         for (int j = 0; j < nroots; ++j) lane += (s_key[j] > key || (s_key[j] == key && j < r)) ? 1 : 0;
         lane_of[r] = lane;
         atomicOr((unsigned long long *)&visited[v], 1ull << lane);
-        parent[v * 64 + lane] = (int32_t)roots[r];
+        parent[roots[r] * 64 + lane] = (int32_t)roots[r];
+        level[roots[r] * 64 + lane] = 0;
         if (d != 0) {
             atomicOr((unsigned long long *)&frontier[v], 1ull << lane);
             atomicOr((unsigned long long *)&active[v >> 6], 1ull << (v & 63));
@@ -1045,16 +1110,82 @@ __global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *
     }
 }
 
-// per-lane extraction: pred of root `lane` as int64[n] in original ids (-1
-// where the lane's visited bit is clear)
-__global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const uint32_t *__restrict__ new2old,
-                             const int32_t *__restrict__ parent, const uint64_t *__restrict__ visited, int64_t n,
-                             int lane, int64_t *__restrict__ pred)
+// Records of edgeless vertices that held a root (original ids; roots whose
+// relabelled id is below bin_end have edges and are left alone): every slot
+// back to "unreached".  One wavefront per listed vertex, lane = slot.
+__global__ __launch_bounds__(256) void k_ms_clear_roots(const int64_t *__restrict__ list, int64_t len,
+                                                        const uint32_t *__restrict__ old2new, int64_t bin_end,
+                                                        int32_t *__restrict__ parent, uint8_t *__restrict__ level)
 {
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6); i < len; i += waves) {
+        const int64_t o = list[i];
+        if ((int64_t)old2new[o] < bin_end) continue;
+        parent[o * 64 + __lane_id()] = -1;
+        level[o * 64 + __lane_id()] = 0xFF;
+    }
+}
+
+// The same for every edgeless vertex (relabelled ids [from, to)).
+__global__ __launch_bounds__(256) void k_ms_clear_range(const uint32_t *__restrict__ new2old, int64_t from,
+                                                        int64_t to, int32_t *__restrict__ parent,
+                                                        uint8_t *__restrict__ level)
+{
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t v = from + blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6); v < to; v += waves) {
+        const int64_t o = new2old[v];
+        parent[o * 64 + __lane_id()] = -1;
+        level[o * 64 + __lane_id()] = 0xFF;
+    }
+}
+
+// The last pass of a batch run: slots of the used lanes that the search did
+// not reach := unreached (-1, 255), for every vertex with edges (relabelled
+// ids [0, bin_end)); edgeless vertices' records already read unreached.  A
+// wave loads 64 vertices' visited words (lane = vertex), then writes the
+// records of those with an unreached lane, one vertex per step (lane = slot).
+__global__ __launch_bounds__(256) void k_ms_finalize(const uint64_t *__restrict__ visited,
+                                                     const uint32_t *__restrict__ new2old, int64_t bin_end,
+                                                     uint64_t used, int32_t *__restrict__ parent,
+                                                     uint8_t *__restrict__ level)
+{
+    const int lane = __lane_id();
+    for (int64_t base = blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); base < bin_end;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int64_t v = base + lane;
+        const uint64_t miss = v < bin_end ? used & ~visited[v] : 0ull;
+        const uint32_t o = miss ? new2old[v] : 0u;
+        uint64_t todo = __ballot(miss != 0);
+        while (todo) {
+            const int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            if (lane_read64(miss, src) >> lane & 1) {
+                const int64_t slot = (int64_t)lane_read(o, src) * 64 + lane;
+                parent[slot] = -1;
+                level[slot] = 0xFF;
+            }
+        }
+    }
+}
+
+// Lane `lane`'s maps as the single-root search returns them, by original id:
+// pred int64[n] (-1 unreached) and, when given, level int32[n] (-1
+// unreached).  root < 0: the records were finalised by a batch run (they
+// are the answer as they stand); otherwise (continuous mode, root = the
+// lane's root) a slot counts only where the lane's visited bit is set, and
+// levels are taken relative to the root's level byte (the engine's level
+// counter when the root joined).
+__global__ void k_ms_extract(const uint32_t *__restrict__ old2new, const int32_t *__restrict__ parent,
+                             const uint8_t *__restrict__ level, const uint64_t *__restrict__ visited, int64_t n,
+                             int lane, int64_t root, int64_t *__restrict__ pred, int32_t *__restrict__ lev)
+{
+    const bool final_maps = root < 0;
+    const uint8_t root_byte = final_maps ? 0 : level[root * 64 + lane];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = old2new[i];
-        pred[i] = (visited[v] >> lane & 1) ? (int64_t)parent_orig(parent[v * 64 + lane], new2old) : -1;
+        const bool reached = final_maps ? parent[i * 64 + lane] >= 0 : (visited[old2new[i]] >> lane & 1) != 0;
+        pred[i] = reached ? (int64_t)parent[i * 64 + lane] : -1;
+        if (lev) lev[i] = reached ? (int32_t)(uint8_t)(level[i * 64 + lane] - root_byte) : -1;
     }
 }
 
@@ -1072,7 +1203,9 @@ __global__ void k_jump_init(const int64_t *__restrict__ pred, int64_t n, uint64_
     }
 }
 
-__global__ void k_jump(uint64_t *__restrict__ w, int64_t n, int *__restrict__ changed)
+// *flags |= 1 when a word changed, 2 when a vertex's ancestor is unreached
+// (a broken tree: its levels cannot be derived)
+__global__ void k_jump(uint64_t *__restrict__ w, int64_t n, int *__restrict__ flags)
 {
     int ch = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -1081,12 +1214,17 @@ __global__ void k_jump(uint64_t *__restrict__ w, int64_t n, int *__restrict__ ch
         if (x == ~0ull) continue;
         const uint32_t a = (uint32_t)x;
         const uint64_t y = __hip_atomic_load(&w[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((y >> 32) == 0 || y == ~0ull) continue;  // a is the root (an unreached a: broken tree)
+        if (y == ~0ull) {
+            ch |= 2;
+            continue;
+        }
+        if ((y >> 32) == 0) continue;  // a is the root
         __hip_atomic_store(&w[i], ((x >> 32) + (y >> 32)) << 32 | (uint32_t)y, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        ch = 1;
+        ch |= 1;
     }
-    if (__ballot(ch) && __lane_id() == 0) atomicOr(changed, 1);
+    ch = (int)g5_wave_or32((uint32_t)ch);
+    if (ch && __lane_id() == 0) atomicOr(flags, ch);
 }
 
 __global__ void k_jump_levels(const uint64_t *__restrict__ w, int64_t n, int32_t *__restrict__ lev)
@@ -1137,8 +1275,10 @@ __global__ void k_lane_masks(uint64_t *__restrict__ visited, int64_t n, uint64_t
 __global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__restrict__ old2new,
                        const int64_t *__restrict__ rowptr, uint64_t *__restrict__ visited,
                        uint64_t *__restrict__ front, uint64_t *__restrict__ act, int32_t *__restrict__ parent,
-                       int64_t *__restrict__ added)
+                       uint8_t *__restrict__ level, const int64_t *__restrict__ cnt, int64_t *__restrict__ added)
 {
+    // the next level finds the root's neighbours with byte cnt[kDepth]
+    const uint8_t root_byte = (uint8_t)(cnt[kDepth] - 1);
     int64_t nv = 0, ne = 0;
     for (int i = 0; i < n; ++i) {
         const int64_t root = jr[i];
@@ -1147,7 +1287,8 @@ __global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__
         const uint64_t bit = 1ull << lane;
         const int64_t deg = rowptr[v + 1] - rowptr[v];
         visited[v] |= bit;
-        parent[v * 64 + lane] = (int32_t)root;
+        parent[root * 64 + lane] = (int32_t)root;
+        level[root * 64 + lane] = root_byte;
         added[2 + i] = deg;
         if (deg == 0) continue;
         if (front[v] == 0) {
@@ -1166,8 +1307,9 @@ __global__ void k_join(const int64_t *__restrict__ jr, int n, const uint32_t *__
 // mix64(orig(v) << 32 | (pred(v) + 1)) mod 2^64, in original ids; then the
 // lanes are freed (visited set everywhere).  A wave takes 64 consecutive
 // vertices (lane = vertex: visited words, ids and degrees in coalesced
-// loads), then reads the parent records of the reached ones 8 vertices at a
-// time with lane = root, all 8 loads issued before any is used.
+// loads), then reads the parent records of the reached ones (by original
+// id) 8 vertices at a time with lane = root, all 8 loads issued before any
+// is used.
 constexpr int kLeaveGroup = 8;
 
 __device__ __forceinline__ uint64_t bcast64(uint64_t x, int l)
@@ -1205,17 +1347,19 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
         for (int j0 = 0; j0 < 64; j0 += kLeaveGroup) {
             if (((todo >> j0) & ((1ull << kLeaveGroup) - 1)) == 0) continue;
             uint64_t rj[kLeaveGroup];
+            uint32_t oj[kLeaveGroup];
             int32_t pj[kLeaveGroup];
 #pragma unroll
             for (int k = 0; k < kLeaveGroup; ++k) {
                 rj[k] = bcast64(reach, j0 + k);
-                pj[k] = (rj[k] >> lane & 1) ? parent[(base + j0 + k) * 64 + lane] : 0;
+                oj[k] = lane_read(o, j0 + k);
+                pj[k] = (rj[k] >> lane & 1) ? parent[(int64_t)oj[k] * 64 + lane] : 0;
             }
 #pragma unroll
             for (int k = 0; k < kLeaveGroup; ++k) {
-                const uint32_t oj = lane_read(o, j0 + k), dj = lane_read(d, j0 + k);
+                const uint32_t dj = lane_read(d, j0 + k);
                 if (rj[k] >> lane & 1) {
-                    ck += g5_mix64(((uint64_t)oj << 32) | (uint32_t)(parent_orig(pj[k], new2old) + 1));
+                    ck += g5_mix64(((uint64_t)oj[k] << 32) | (uint32_t)(pj[k] + 1));
                     dg += dj;
                 }
             }
@@ -1248,7 +1392,8 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
-                     m->d_visited, front, act, next, m->d_parent, m->d_cnt, k, act_next
+                     m->d_visited, front, act, next, m->d_parent, m->d_level, (const uint32_t *)R.new2old, m->d_cnt, \
+                     k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
                               : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
                                        : g5::grid_for(cnt, kBlock, 16384);
@@ -1307,7 +1452,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr);
+                       m->d_level, m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr);
     // the newly reached vertices of the lanes lm
     if (!whole)
         hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
@@ -1319,7 +1464,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
-                       m->d_cnt, lm, (uint64_t *)nullptr);
+                       m->d_level, m->d_cnt, lm, (uint64_t *)nullptr);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
     // the whole next frontier: active bitmap, counts, live lanes, lane edges
@@ -1642,6 +1787,8 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_nq, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_qoff, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_parent, (size_t)n * 64 * 4);
+    if (e == hipSuccess) e = hipMalloc(&m->d_level, (size_t)n * 64);
+    if (e == hipSuccess) e = hipMalloc(&m->d_dirty, (size_t)kDirtyCap * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_cnt, kStripes * kCntStride * 8);
     if (e == hipSuccess) e = hipHostMalloc(&m->h_cnt, kHostSlots * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->d_lane, kLaneStripes * 64 * 8);
@@ -1651,8 +1798,13 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_sums, 128 * 8);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev_fin);
     for (int k = 0; k <= kBins && e == hipSuccess; ++k) e = hipEventCreate(&m->ek[k]);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&m->ek_push[k]);
+    // every record starts unreached; a batch run keeps it so wherever it does
+    // not reach
+    if (e == hipSuccess) e = g5::fill_i32(m->d_parent, n * 64, -1, m->s);
+    if (e == hipSuccess) e = g5::fill_u32((uint32_t *)m->d_level, n * 16, 0xFFFFFFFFu, m->s);
     if (e != hipSuccess) return fail(g5::fail_hip(e, "g5_msbfs_create allocation", __FILE__, __LINE__));
     int rc = build_relabel(m);
     if (rc) return fail(rc);
@@ -1674,11 +1826,11 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
-                   m->d_lane,    m->d_fq2,     m->rl.head};
+                   m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
-    hipEvent_t evs[] = {m->ev0, m->ev1};
+    hipEvent_t evs[] = {m->ev0, m->ev1, m->ev_fin};
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : m->ek)
@@ -1703,8 +1855,24 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     std::copy(roots, roots + nroots, m->roots);
     int64_t *d_roots = m->d_roots;
     G5_HIP(hipMemcpyAsync(d_roots, roots, (size_t)nroots * 8, hipMemcpyHostToDevice, s));
+    const bool dirty_listed = (int64_t)m->dirty.size() <= kDirtyCap;
+    if (dirty_listed && !m->dirty.empty())
+        G5_HIP(hipMemcpyAsync(m->d_dirty, m->dirty.data(), m->dirty.size() * 8, hipMemcpyHostToDevice, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_HIP(hipEventRecord(m->ev0, s));
+    // records of edgeless vertices that held an earlier root read unreached
+    // again (every edgeless vertex's, when too many roots came since)
+    const int64_t bin_end = m->rl.bin_first[kBins];
+    if (!dirty_listed) {
+        if (m->n > bin_end)
+            hipLaunchKernelGGL(k_ms_clear_range, g5::grid_for(m->n - bin_end, 4, 16384), 256, 0, s,
+                               (const uint32_t *)m->rl.new2old, bin_end, m->n, m->d_parent, m->d_level);
+    } else if (!m->dirty.empty()) {
+        hipLaunchKernelGGL(k_ms_clear_roots, g5::grid_for((int64_t)m->dirty.size(), 4, 4096), 256, 0, s,
+                           (const int64_t *)m->d_dirty, (int64_t)m->dirty.size(), (const uint32_t *)m->rl.old2new,
+                           bin_end, m->d_parent, m->d_level);
+    }
+    m->dirty.assign(roots, roots + nroots);
     const uint64_t unused = nroots == 64 ? 0ull : ~((1ull << nroots) - 1);
     // when the last level run was a push level that found nothing, its next
     // buffer is clear and its frontier buffer is non-zero only at the entries
@@ -1720,8 +1888,8 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     hipLaunchKernelGGL(k_root_reach, nroots, 256, 0, s, (const int64_t *)d_roots, (const uint32_t *)m->rl.old2new,
                        (const int64_t *)m->rl.rowptr, (const uint32_t *)m->rl.cols, m->d_join);
     hipLaunchKernelGGL(k_ms_roots, 1, 64, 0, s, (const int64_t *)d_roots, nroots, (const uint32_t *)m->rl.old2new,
-                       (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_cnt,
-                       (int32_t *)(m->d_roots + 64), (const int64_t *)m->d_join);
+                       (const int64_t *)m->rl.rowptr, m->d_visited, m->d_fa, m->d_act, m->d_parent, m->d_level,
+                       m->d_cnt, (int32_t *)(m->d_roots + 64), (const int64_t *)m->d_join);
     G5_HIP(hipGetLastError());
     int levels = 1;
     m->lvl.clear();
@@ -1748,11 +1916,23 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
         if (new_bits == 0) break;
         ++levels;
     }
+    // the maps are final once the used lanes' unreached slots say so
+    G5_HIP(hipEventRecord(m->ev_fin, s));
+    if (bin_end)
+        hipLaunchKernelGGL(k_ms_finalize, g5::grid_for(bin_end, kBlock, 8192), kBlock, 0, s,
+                           (const uint64_t *)m->d_visited, (const uint32_t *)m->rl.new2old, bin_end, ~unused,
+                           m->d_parent, m->d_level);
+    G5_HIP(hipGetLastError());
     G5_HIP(hipEventRecord(m->ev1, s));
     G5_HIP(hipMemcpyAsync(m->lane_of, m->d_roots + 64, 64 * 4, hipMemcpyDeviceToHost, s));
     G5_HIP(hipEventSynchronize(m->ev1));
     G5_HIP(hipStreamSynchronize(s));
-    if (ms) G5_HIP(hipEventElapsedTime(ms, m->ev0, m->ev1));
+    G5_HIP(hipEventElapsedTime(&m->search_ms, m->ev0, m->ev_fin));
+    G5_HIP(hipEventElapsedTime(&m->finalize_ms, m->ev_fin, m->ev1));
+    // level L is stored as the byte L: final up to level 254 (255: unreached)
+    m->run_levels = levels;
+    m->levels_final = levels <= 255;
+    if (ms) *ms = m->search_ms + m->finalize_ms;
     if (nlevels) *nlevels = levels;
     return G5_OK;
 }
@@ -1805,12 +1985,18 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
     G5_HIP(hipMemcpyAsync(m->d_join, h, sizeof(h), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_lane_masks, g5::grid_for(m->n, 256, 16384), 256, 0, s, m->d_visited, m->n, 0ull, mask);
     hipLaunchKernelGGL(k_join, 1, 1, 0, s, (const int64_t *)m->d_join, n, (const uint32_t *)m->rl.old2new,
-                       (const int64_t *)m->rl.rowptr, m->d_visited, front, act, m->d_parent, m->d_join + 128);
+                       (const int64_t *)m->rl.rowptr, m->d_visited, front, act, m->d_parent, m->d_level,
+                       (const int64_t *)m->d_cnt, m->d_join + 128);
     G5_HIP(hipGetLastError());
     int64_t added[66] = {};
     G5_HIP(hipMemcpyAsync(added, m->d_join + 128, (size_t)(2 + n) * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     m->busy |= mask;
+    for (int i = 0; i < n; ++i) {
+        m->lane_root[lanes[i]] = roots[i];
+        m->lane_steps[lanes[i]] = 0;
+        if ((int64_t)m->dirty.size() <= kDirtyCap) m->dirty.push_back(roots[i]);
+    }
     m->act_verts += added[0];
     m->act_edges += added[1];
     // a joined lane's frontier is its root alone (none if the root has no
@@ -1837,6 +2023,8 @@ extern "C" int g5_msbfs_step(g5_msbfs *m, uint64_t *live, float *ms)
     uint64_t l = 0;
     int rc = run_level(m, &new_bits, &l);
     if (rc) return rc;
+    for (int r = 0; r < 64; ++r)
+        if (m->busy >> r & 1) ++m->lane_steps[r];
     G5_HIP(hipEventRecord(m->ev1, m->s));
     G5_HIP(hipEventSynchronize(m->ev1));
     if (ms) G5_HIP(hipEventElapsedTime(ms, m->ev0, m->ev1));
@@ -1875,12 +2063,35 @@ namespace {
 int internal_lane(const g5_msbfs *m, int lane) { return m->cont ? lane : m->lane_of[lane]; }
 }  // namespace
 
+namespace {
+// the lane's level bytes hold its levels: a finalised batch run of depth
+// below 255, or a continuous-mode lane that has run fewer than 255 levels
+bool lane_levels_exact(const g5_msbfs *m, int lane)
+{
+    return m->cont ? m->lane_steps[lane] < 255 : m->levels_final;
+}
+
+// extraction of internal lane il into device arrays (lev may be null)
+int extract_lane(g5_msbfs *m, int il, int64_t *d_pred, int32_t *d_lev)
+{
+    hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s, (const uint32_t *)m->rl.old2new,
+                       (const int32_t *)m->d_parent, (const uint8_t *)m->d_level, (const uint64_t *)m->d_visited,
+                       m->n, il, m->cont ? m->lane_root[il] : (int64_t)-1, d_pred, d_lev);
+    G5_HIP(hipGetLastError());
+    return G5_OK;
+}
+}  // namespace
+
 extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level)
 {
     G5_REQUIRE(m && lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
                "g5_msbfs_result: lane %d not in use", lane);
     const int64_t n = m->n;
     hipStream_t s = m->s;
+    const int il = internal_lane(m, lane);
+    // levels from the level bytes, or (a search deeper than they hold) from
+    // the parent tree by pointer jumping
+    const bool jump = level && !lane_levels_exact(m, il);
     int64_t *dp = nullptr;
     uint64_t *dw = nullptr;
     int32_t *dl = nullptr;
@@ -1892,44 +2103,49 @@ extern "C" int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *le
         if (dflag) (void)hipFree(dflag);
     };
     hipError_t e = hipMalloc(&dp, (size_t)n * 8);
-    if (e == hipSuccess && level) e = hipMalloc(&dw, (size_t)n * 8);
     if (e == hipSuccess && level) e = hipMalloc(&dl, (size_t)n * 4);
-    if (e == hipSuccess && level) e = hipMalloc(&dflag, sizeof(int));
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_ms_extract, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint32_t *)m->rl.old2new,
-                           (const uint32_t *)m->rl.new2old, (const int32_t *)m->d_parent,
-                           (const uint64_t *)m->d_visited, n, internal_lane(m, lane),
-                           dp);
-        e = hipGetLastError();
+    if (e == hipSuccess && jump) e = hipMalloc(&dw, (size_t)n * 8);
+    if (e == hipSuccess && jump) e = hipMalloc(&dflag, sizeof(int));
+    if (e != hipSuccess) {
+        cleanup();
+        G5_HIP(e);
     }
-    if (e == hipSuccess && pred) e = hipMemcpyAsync(pred, dp, (size_t)n * 8, hipMemcpyDeviceToHost, s);
-    bool converged = level == nullptr;
-    if (e == hipSuccess && level) {
+    int rc = extract_lane(m, il, dp, jump ? nullptr : dl);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    if (pred) e = hipMemcpyAsync(pred, dp, (size_t)n * 8, hipMemcpyDeviceToHost, s);
+    int flags = 0;
+    if (e == hipSuccess && jump) {
         hipLaunchKernelGGL(k_jump_init, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)dp, n, dw);
         e = hipGetLastError();
         // each pass at least doubles every vertex's jump: 40 passes cover
         // any depth below 2^32; a parent cycle never converges
+        bool converged = false;
         for (int pass = 0; pass < 40 && e == hipSuccess && !converged; ++pass) {
-            int changed = 0;
             e = g5::fill_i32(dflag, 1, 0, s);
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(k_jump, g5::grid_for(n, 256, 16384), 256, 0, s, dw, n, dflag);
                 e = hipGetLastError();
             }
-            if (e == hipSuccess) e = hipMemcpyAsync(&changed, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(&flags, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
-            converged = changed == 0;
+            converged = (flags & 1) == 0;
+            if (flags & 2) break;
         }
+        if (e == hipSuccess && !converged) flags |= 4;
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_jump_levels, g5::grid_for(n, 256, 16384), 256, 0, s, (const uint64_t *)dw, n, dl);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(level, dl, (size_t)n * 4, hipMemcpyDeviceToHost, s);
     }
+    if (e == hipSuccess && level) e = hipMemcpyAsync(level, dl, (size_t)n * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     G5_HIP(e);
-    G5_REQUIRE(converged, G5_E_STATE, "g5_msbfs_result: parent tree of lane %d has a cycle", lane);
+    G5_REQUIRE(!(flags & 2), G5_E_STATE, "g5_msbfs_result: lane %d has a vertex whose parent is unreached", lane);
+    G5_REQUIRE(!(flags & 4), G5_E_STATE, "g5_msbfs_result: parent tree of lane %d has a cycle", lane);
     return G5_OK;
 }
 
@@ -1938,12 +2154,48 @@ extern "C" int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred)
     G5_REQUIRE(m && d_pred, G5_E_ARG, "g5_msbfs_pred_device: null argument");
     G5_REQUIRE(lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
                "g5_msbfs_pred_device: lane %d not in use", lane);
-    hipLaunchKernelGGL(k_ms_extract, g5::grid_for(m->n, 256, 16384), 256, 0, m->s, (const uint32_t *)m->rl.old2new,
-                       (const uint32_t *)m->rl.new2old, (const int32_t *)m->d_parent,
-                       (const uint64_t *)m->d_visited, m->n, internal_lane(m, lane),
-                       d_pred);
-    G5_HIP(hipGetLastError());
+    int rc = extract_lane(m, internal_lane(m, lane), d_pred, nullptr);
+    if (rc) return rc;
     G5_HIP(hipStreamSynchronize(m->s));
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_level_device(g5_msbfs *m, int lane, int32_t *d_level)
+{
+    G5_REQUIRE(m && d_level, G5_E_ARG, "g5_msbfs_level_device: null argument");
+    G5_REQUIRE(lane >= 0 && lane < 64 && (m->cont ? (m->busy >> lane & 1) != 0 : lane < m->nroots), G5_E_ARG,
+               "g5_msbfs_level_device: lane %d not in use", lane);
+    const int il = internal_lane(m, lane);
+    G5_REQUIRE(lane_levels_exact(m, il), G5_E_STATE,
+               "g5_msbfs_level_device: lane %d searched 255 levels or more; use g5_msbfs_result", lane);
+    int64_t *dp = nullptr;
+    G5_HIP(hipMalloc(&dp, (size_t)m->n * 8));
+    int rc = extract_lane(m, il, dp, d_level);
+    hipError_t e = hipStreamSynchronize(m->s);
+    (void)hipFree(dp);
+    if (rc) return rc;
+    G5_HIP(e);
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_maps(g5_msbfs *m, const int32_t **d_pred, const uint8_t **d_level, int32_t *lane_of,
+                             int *levels_final)
+{
+    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_maps: null handle");
+    G5_REQUIRE(!m->cont && m->nroots > 0, G5_E_STATE, "g5_msbfs_maps: no batch run");
+    if (d_pred) *d_pred = m->d_parent;
+    if (d_level) *d_level = m->d_level;
+    if (lane_of) std::copy(m->lane_of, m->lane_of + m->nroots, lane_of);
+    if (levels_final) *levels_final = m->levels_final ? 1 : 0;
+    return G5_OK;
+}
+
+extern "C" int g5_msbfs_times(g5_msbfs *m, float *search_ms, float *finalize_ms)
+{
+    G5_REQUIRE(m, G5_E_ARG, "g5_msbfs_times: null handle");
+    G5_REQUIRE(!m->cont && m->nroots > 0, G5_E_STATE, "g5_msbfs_times: no batch run");
+    if (search_ms) *search_ms = m->search_ms;
+    if (finalize_ms) *finalize_ms = m->finalize_ms;
     return G5_OK;
 }
 

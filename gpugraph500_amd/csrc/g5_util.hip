@@ -282,6 +282,38 @@ extern "C" int g5_fill_u64(uint64_t *d, int64_t n, uint64_t value)
     return G5_OK;
 }
 
+namespace {
+// *ndiff += the positions where a and b differ
+__global__ __launch_bounds__(256) void k_diff_u32(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                                  int64_t n, unsigned long long *__restrict__ ndiff)
+{
+    int64_t c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += a[i] != b[i];
+    c = g5_block_sum256(c);
+    if (threadIdx.x == 0 && c) atomicAdd(ndiff, (unsigned long long)c);
+}
+}  // namespace
+
+extern "C" int g5_diff_u32(const uint32_t *d_a, const uint32_t *d_b, int64_t n, int64_t *ndiff)
+{
+    G5_REQUIRE(n >= 0 && ndiff && (n == 0 || (d_a && d_b)), G5_E_ARG, "g5_diff_u32: bad arguments");
+    *ndiff = 0;
+    if (n == 0) return G5_OK;
+    unsigned long long *d = nullptr, h = 0;
+    G5_HIP(hipMalloc(&d, 8));
+    hipError_t e = g5::fill_u64((uint64_t *)d, 1, 0, nullptr);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_diff_u32, g5::grid_for(n, 256, 8192), 256, 0, nullptr, d_a, d_b, n, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    G5_HIP(e);
+    *ndiff = (int64_t)h;
+    return G5_OK;
+}
+
 extern "C" int g5_bitmap_to_queue(const uint64_t *d_bits, int64_t n_words, uint32_t *d_queue, int64_t *n)
 {
     G5_REQUIRE(n_words >= 0 && n && (n_words == 0 || (d_bits && d_queue)), G5_E_ARG,

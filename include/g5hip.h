@@ -52,6 +52,10 @@ int g5_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
 int g5_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
 int g5_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
 int g5_fill_u64(uint64_t *d, int64_t n, uint64_t value);
+/* *ndiff = positions i < n where d_a[i] != d_b[i] (device arrays; e.g. two
+ * predecessor maps viewed as 2n words), so results are compared where they
+ * lie. */
+int g5_diff_u32(const uint32_t *d_a, const uint32_t *d_b, int64_t n, int64_t *ndiff);
 /* Device bitmap <-> sorted queue (the frontier representations the 2D
  * exchange moves; cuda/cuda_bfs.cu:380-475 sorts and uniques on the host).
  * d_queue needs room for every set bit (n_words * 64 at most); *n = length.
@@ -92,6 +96,11 @@ int g5_graph_info(const g5_graph *g, int64_t *n_rows, int64_t *n_cols, int64_t *
                   int64_t *nglobal, int64_t *n_gen_edges);
 /* global ids of the block's local row 0 and column 0 */
 int g5_graph_bounds(const g5_graph *g, int64_t *row_start, int64_t *col_start);
+/* Degree histogram of the block's rows: hist[0] = rows without neighbours,
+ * hist[k] = rows with 2^(k-1) <= degree < 2^k, hist[nbins-1] open-ended
+ * (2 <= nbins <= 64).  The batched search's shapes are kept per scale and
+ * histogram (its degree skew). */
+int g5_graph_degree_hist(const g5_graph *g, int64_t *hist, int nbins);
 /* generated directed degree per local row, gdeg[n_rows] */
 int g5_graph_gdeg(const g5_graph *g, int64_t *gdeg);
 int g5_graph_download(const g5_graph *g, int64_t *rowptr, uint32_t *cols, int64_t *gdeg);
@@ -144,6 +153,9 @@ int g5_bfs_out_bitmap(g5_bfs *b, const uint64_t **d_bits, int64_t *n_words);
 int g5_bfs_set_out_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_set_in_bitmap(g5_bfs *b, const uint64_t *d_bits);
 int g5_bfs_pred_device(g5_bfs *b, int64_t *d_pred);
+/* the level map int32[n_cols] of the last search (-1 unreached) copied to
+ * device memory d_level */
+int g5_bfs_level_device(g5_bfs *b, int32_t *d_level);
 /* A bottom-up level of the hook path, the alternative to g5_bfs_expand for
  * dense levels (no counterpart in the reference, whose CUDA path is
  * top-down only): every unvisited column vertex of the block scans its
@@ -196,19 +208,43 @@ int g5_bfs_summary(g5_bfs *b, int64_t *nedge, uint64_t *checksum);
  * beside the single-root plug-in; results equal nroots separate searches
  * with the minimum-id parent rule of cpu_2d/cpubfs_bin.cpp:150-173) ----
  * Needs a diagonal (1x1) block.  Up to 64 roots per run, one uint64 lane mask
- * per vertex; parents are kept as int32[n][64] in HBM (levels are derived
- * from the parent tree on extraction, as validate/validate.h:107-232 does). */
+ * per vertex. */
 int g5_msbfs_create(g5_graph *g, g5_msbfs **out);
 int g5_msbfs_free(g5_msbfs *m);
-/* Searches all roots; *nlevels = levels of the deepest search, *ms = device time. */
+/* Searches all roots; *nlevels = levels of the deepest search, *ms = device
+ * time from the start to the final maps (g5_msbfs_maps), as the reference's
+ * runBFS window ends with the predecessors final (cpu_2d/main.cpp:419-462,
+ * globalbfs.hh:985-1082).  The search and its last (finalising) pass are
+ * split by g5_msbfs_times. */
 int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32_t *nlevels, float *ms);
+/* The result of the last batch run as it lies in HBM (borrowed; valid until
+ * the next run, open or free on m):
+ *   d_pred   int32[n][64]: row v (original vertex id), slot lane_of[i] holds
+ *            root i's parent of v as an original id (root -> root, -1
+ *            unreached); slots of lanes >= nroots are undefined
+ *   d_level  uint8[n][64], the same slots: distance from the root (255
+ *            unreached) -- the level map, when *levels_final (a search of
+ *            fewer than 255 levels; deeper ones take g5_msbfs_result)
+ *   lane_of  int32[nroots]: root i's lane (roots are ranked by reach)
+ * Any pointer may be NULL. */
+int g5_msbfs_maps(g5_msbfs *m, const int32_t **d_pred, const uint8_t **d_level, int32_t *lane_of,
+                  int *levels_final);
+/* Device time of the last batch run's search (start to the last level) and
+ * of its finalising pass (unreached slots of the used lanes). */
+int g5_msbfs_times(g5_msbfs *m, float *search_ms, float *finalize_ms);
 /* Root `lane`'s result as the single-root search returns it: pred int64[n]
  * (-1 unreached, root -> root) and level int32[n] (-1 unreached); either may
- * be NULL.  Allocates device scratch; G5_E_STATE if the tree has a cycle. */
+ * be NULL.  Allocates device scratch.  Levels come from the level bytes, or,
+ * for a search of 255 levels or more, from the parent tree by pointer
+ * jumping (G5_E_STATE if that tree is broken: a cycle, or a vertex whose
+ * parent is unreached). */
 int g5_msbfs_result(g5_msbfs *m, int lane, int64_t *pred, int32_t *level);
 /* The same pred int64[n] written to device memory d_pred (no allocation), for
  * g5_validate_device without a host round trip. */
 int g5_msbfs_pred_device(g5_msbfs *m, int lane, int64_t *d_pred);
+/* The same level int32[n] written to device memory d_level (G5_E_STATE for
+ * a search of 255 levels or more: use g5_msbfs_result). */
+int g5_msbfs_level_device(g5_msbfs *m, int lane, int32_t *d_level);
 /* TEPS numerators nedge[nroots] of the last run (validate/validate.h:439-485). */
 int g5_msbfs_nedge(g5_msbfs *m, int64_t *nedge);
 /* Direction per level: 0 (default) push while the frontier's edges are

@@ -20,6 +20,7 @@ import hashlib
 import json
 import os
 import subprocess
+import tempfile
 import sys
 
 import numpy as np
@@ -130,13 +131,13 @@ def main():
     np.savez_compressed(os.path.join(HERE, "codec_s4bp128d4.npz"), **arrays)
     record("codec_s4bp128d4.npz")
 
-    exe = os.path.join(ROOT, "_work", "roots_draws")
-    os.makedirs(os.path.dirname(exe), exist_ok=True)
-    subprocess.check_call(["g++", "-O2", "-std=c++11", "-o", exe, os.path.join(HERE, "roots_draws.cpp")])
     draws = {}
-    for s in (5, 10, 16, 20, 22, 24, 26):
-        out = subprocess.check_output([exe, str(1 << s), "4096"]).decode().split()
-        draws[str(s)] = [int(v) for v in out]
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "roots_draws")
+        subprocess.check_call(["g++", "-O2", "-std=c++11", "-o", exe, os.path.join(HERE, "roots_draws.cpp")])
+        for s in (5, 10, 16, 20, 22, 24, 26):
+            out = subprocess.check_output([exe, str(1 << s), "4096"]).decode().split()
+            draws[str(s)] = [int(v) for v in out]
     with open(os.path.join(HERE, "roots_draws.json"), "w") as f:
         json.dump(draws, f)
     record("roots_draws.json")

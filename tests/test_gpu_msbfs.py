@@ -241,3 +241,90 @@ def test_tune_rejects_bad_values():
             m2.tune(**bad)
     with pytest.raises(g5.G5Error):
         g5.check(g5.lib().g5_msbfs_tune(m.h, 9, 1))
+
+
+def _download_maps(m, n):
+    dp, dl, lane_of, final = m.maps()
+    par = np.empty((n, 64), np.int32)
+    lev = np.empty((n, 64), np.uint8)
+    g5.check(g5.lib().g5_memcpy_d2h(par.ctypes.data_as(g5.c_vp), g5.c_vp(dp), par.nbytes))
+    g5.check(g5.lib().g5_memcpy_d2h(lev.ctypes.data_as(g5.c_vp), g5.c_vp(dl), lev.nbytes))
+    return par, lev, lane_of, final
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_final_maps_as_they_lie_in_hbm(mode):
+    # the run's own output, with no extraction: row v (original id), slot
+    # lane_of[i] = root i's parent and level, -1 / 255 where unreached
+    e, ref, g = kron(12)
+    roots = O.select_roots(ref, 1 << 12)
+    m = g5.MSBFS(g)
+    m.set_mode(mode)
+    for rs in (roots, roots[7:20]):
+        m.run(rs)
+        par, lev, lane_of, final = _download_maps(m, g.n_rows)
+        assert final
+        assert sorted(lane_of.tolist()) == list(range(len(rs)))
+        for i, r in enumerate(rs.tolist()):
+            opred, olev, _ = O.bfs_bin(ref, r)
+            assert np.array_equal(par[:, lane_of[i]].astype(np.int64), opred), (mode, i)
+            want = np.where(olev < 0, 255, olev).astype(np.uint8)
+            assert np.array_equal(lev[:, lane_of[i]], want), (mode, i)
+
+
+def test_edgeless_root_leaves_no_trace():
+    # an edgeless root's own slot holds the root; the next runs, with other
+    # roots in that lane, read the vertex as unreached again (also after
+    # continuous-mode joins of edgeless roots)
+    e, ref, g = kron(12)
+    deg = np.diff(ref.rowptr)
+    iso = np.flatnonzero(deg == 0)
+    assert len(iso) >= 3
+    sel = O.select_roots(ref, 1 << 12)
+    m = g5.MSBFS(g)
+    m.run([int(iso[0])] + sel[:10].tolist())
+    par, lev, lane_of, _ = _download_maps(m, g.n_rows)
+    assert par[iso[0], lane_of[0]] == iso[0] and lev[iso[0], lane_of[0]] == 0
+    m.open()
+    m.join([int(iso[1]), int(sel[3])], [0, 5])
+    m.step()
+    m.run(sel[20:60])
+    par, lev, lane_of, _ = _download_maps(m, g.n_rows)
+    used = lane_of.tolist()
+    for v in iso[:3]:
+        assert (par[v, used] == -1).all() and (lev[v, used] == 255).all()
+    _check_all_lanes(m, ref, sel[20:60])
+
+
+def test_deep_search_levels_from_the_parent_tree():
+    # a path of 600 vertices: 599 levels do not fit the level bytes, so the
+    # run reports them as not final and the per-root result derives levels
+    # from the parent tree; a star beside it (centre 600) is shallow
+    n = 600
+    path = np.stack([np.arange(n - 1), np.arange(1, n)], 1)
+    star = np.stack([np.full(39, 600), np.arange(601, 640)], 1)
+    ref = O.build_csr(np.concatenate([path, star]), 640)
+    g = g5.Graph.upload(ref.rowptr, ref.cols, 640, gdeg=ref.gdeg)
+    m = g5.MSBFS(g)
+    roots = [0, 299, 599]
+    nl, _ = m.run(roots)
+    assert nl == 600
+    _, _, _, final = m.maps()
+    assert not final
+    for lane, r in enumerate(roots):
+        pred, lev = m.result(lane)
+        opred, olev, _ = O.bfs_bin(ref, r)
+        assert np.array_equal(pred, opred) and np.array_equal(lev, olev), r
+        d = g5.DeviceArray(640, np.int32)
+        with pytest.raises(g5.G5Error):
+            m.level_device(lane, d)
+    # a shallow run afterwards has final level bytes again
+    m.run([600, 605])
+    assert m.maps()[3]
+    for lane, r in enumerate([600, 605]):
+        pred, lev = m.result(lane)
+        opred, olev, _ = O.bfs_bin(ref, r)
+        assert np.array_equal(lev, olev) and np.array_equal(pred, opred)
+        d = g5.DeviceArray(640, np.int32)
+        m.level_device(lane, d)
+        assert np.array_equal(d.to_host(), olev)
