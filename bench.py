@@ -230,6 +230,50 @@ def pmc_probe(args):
     return 0
 
 
+def probe(args):
+    """Workloads for profiler runs (rocprofv3 before this script):
+    single    the 64 single-root searches (profiled) and the copy calibration kernel
+    exchange  the c3 and c2 sub-records by themselves
+    batched   one profiled and six timed batched searches with the given shapes
+    Each prints one JSON line."""
+    import gpugraph500_amd as g5
+    g5.init(0)
+    g = g5.Graph.kronecker(args.scale)
+    roots = g.select_roots(1 << args.scale, args.roots)
+    if args.probe == "single":
+        b = g5.BFS(g)
+        for r in roots[:2].tolist():
+            b.run(r)
+        b.profile(True)
+        ms = [b.run(r)[1] for r in roots.tolist()]
+        out = {"mean_ms": float(np.mean(ms)), "kernels": b.kernel_stats()}
+        b.free()
+        g.free()
+        src = g5.DeviceArray(COPY_WORDS, np.uint64)
+        dst = g5.DeviceArray(COPY_WORDS, np.uint64)
+        for _ in range(3):
+            g5.check(g5.lib().g5_vreduce_or(src.ptr, 1, COPY_WORDS, dst.ptr))
+        g5.check(g5.lib().g5_device_sync())
+    elif args.probe == "exchange":
+        out = {"c3": sub_c3(g5, g, roots)}
+        g.free()
+        out["c2"] = sub_c2(g5)
+    else:
+        m = g5.MSBFS(g)
+        if args.tune_json:
+            m.tune(**json.loads(args.tune_json))
+        m.run(roots)
+        m.profile(True)
+        m.run(roots)
+        st, lv = m.stats(), m.levels()
+        m.profile(False)
+        ts = [m.run(roots)[1] for _ in range(6)]
+        out = {"ms": float(np.mean(ts[1:])), "times": m.times(), "kernels": st,
+               "levels": [[round(x, 3) for x in r[:4]] + [int(r[9])] for r in lv]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _pmc_pass(exe, counter, args, tune_json):
     """{pull kernel: mean counter per dispatch}, the copy kernel's mean and the
     dispatches sampled."""
@@ -486,7 +530,130 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
     m.free()
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(g, args.scale, roots, nedge)
+    if args.subconfigs == "on":
+        out["c3"] = sub_c3(g5, g, roots)
+        g.free()
+        out["c2"] = sub_c2(g5)
     return out
+
+
+def _rates(stats):
+    """GB/s per exchange unit from g5_exchange_stats (algorithmic bytes over
+    the units' device time)."""
+    return {k: {"gbs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else 0.0,
+                "frac": (v["bytes"] / (v["ms"] / 1e3) / 1e9) / HBM_PEAK_GBS if v["ms"] > 0 else 0.0,
+                "ms": v["ms"], "calls": v["calls"], "bytes": v["bytes"]}
+            for k, v in stats.items() if v["calls"]}
+
+
+def sub_c2(g5, scale=22, nroots=64):
+    """C2 beside the headline: the single-root search with every level's queue
+    through s4-bp128-d4 encode -> decode on the device at scale 22 (hmean
+    GTEPS over the 64 search keys, each checked against the plain search's
+    TEPS numerator), and the device codec's own rates, event-timed through
+    g5_s4_encode / g5_s4_decode on the level queues of four of those
+    searches (encode 4n + 4*csize bytes, decode 4*csize + 4n)."""
+    t0 = time.perf_counter()
+    g = g5.Graph.kronecker(scale)
+    roots = g.select_roots(1 << scale, nroots)
+    b = g5.BFS(g)
+    b.run(int(roots[0]), codec=True)
+    ms, teps = [], []
+    for r in roots.tolist():
+        b.run(r)
+        ne = b.nedge()
+        nl, t = b.run(r, codec=True)
+        assert b.nedge() == ne, "the codec changed a search"
+        ms.append(t)
+        teps.append(ne / (t / 1e3))
+    queues = []
+    for r in roots[:4].tolist():
+        b.run(r)
+        _, lev = b.result()
+        queues += [np.flatnonzero(lev == d).astype(np.uint32) for d in range(1, int(lev.max()) + 1)]
+    b.free()
+    g.free()
+    words, sizes = 0, 0
+    g5.s4_decode(g5.s4_encode(queues[0]), queues[0].size)  # code objects loaded before timing
+    g5.exchange_profile(True)
+    for q in queues:
+        w = g5.s4_encode(q)
+        back = g5.s4_decode(w, q.size)
+        assert np.array_equal(back, q), "codec round trip lost values"
+        words += w.size
+        sizes += q.size
+    rates = _rates(g5.exchange_stats())
+    g5.exchange_profile(False)
+    log(f"c2: hmean {hmean(teps) / 1e9:.2f} GTEPS with the codec on every level; codec encode "
+        f"{rates['encode']['gbs']:.0f} GB/s, decode {rates['decode']['gbs']:.0f} GB/s "
+        f"({time.perf_counter() - t0:.1f} s)")
+    return {"workload": f"single-root BFS + s4-bp128-d4 encode/decode of every level queue, 64 roots, "
+                        f"Kronecker scale {scale}, 1x1 block",
+            "hmean_gteps": hmean(teps) / 1e9, "mean_ms_per_root": float(np.mean(ms)),
+            "codec": {"queues": len(queues), "values": sizes, "words": words,
+                      "what": "level queues of 4 of the searches, each encoded and decoded once on the device",
+                      **rates}}
+
+
+def sub_c3(g5, g, roots):
+    """C3 beside the headline: one rank's share of a 4 x 4 grid at scale 26
+    (its 2^24 columns).  Per level of the seeded search of the first key, 4
+    column-frontier bitmaps of that level's density within the rank's
+    columns (262,144 words each) are OR-combined on the device, and the
+    level's queue within the rank's columns goes through the codec; then 4
+    predecessor arrays of 2^24 are min-reduced.  Rates from the device's own
+    event timing of g5_vreduce_or / g5_vreduce_min_pred / g5_s4_encode /
+    g5_s4_decode (OR (k+1)*8*W, min-pred (k+1)*8*n)."""
+    t0 = time.perf_counter()
+    n, k = 1 << 24, 4
+    w = n // 64
+    b = g5.BFS(g)
+    b.run(int(roots[0]))
+    pred, lev = b.result()
+    b.free()
+    rng = np.random.default_rng(26)
+    maps = g5.DeviceArray(k * w, np.uint64)
+    out = g5.DeviceArray(w, np.uint64)
+    dens = []
+    g5.check(g5.lib().g5_vreduce_or(maps.ptr, k, w, out.ptr))  # code objects loaded before timing
+    g5.exchange_profile(True)
+    for d in range(1, int(lev.max()) + 1):
+        p = float(np.mean(lev[:n] == d))
+        dens.append(p)
+        bits = rng.random(k * n, dtype=np.float32) < p
+        maps_h = np.packbits(bits, bitorder="little").view(np.uint64)
+        g5.check(g5.lib().g5_memcpy_h2d(maps.ptr, g5._p(maps_h), maps_h.nbytes))
+        g5.check(g5.lib().g5_vreduce_or(maps.ptr, k, w, out.ptr))
+        q = np.flatnonzero(lev[:n] == d).astype(np.uint32)
+        if q.size:
+            back = g5.s4_decode(g5.s4_encode(q), q.size)
+            assert np.array_equal(back, q), "codec round trip lost values"
+    want = np.bitwise_or.reduce(maps_h.reshape(k, w), axis=0)
+    assert np.array_equal(out.to_host(), want), "vreduce OR differs from numpy"
+    # the predecessor combine: each of the 4 contributions holds the parents
+    # of a random quarter of the reached columns (-1 elsewhere)
+    own = rng.integers(0, k, size=n)
+    preds_h = np.full((k, n), -1, np.int64)
+    for j in range(k):
+        sel = (own == j) & (pred[:n] >= 0)
+        preds_h[j, sel] = pred[:n][sel]
+    dp = g5.DeviceArray.from_host(preds_h.reshape(-1))
+    dmin = g5.DeviceArray(n, np.int64)
+    for _ in range(3):
+        g5.check(g5.lib().g5_vreduce_min_pred(dp.ptr, k, n, dmin.ptr))
+    assert np.array_equal(dmin.to_host(), pred[:n]), "min-pred combine differs from the search's parents"
+    rates = _rates(g5.exchange_stats())
+    g5.exchange_profile(False)
+    for a in (maps, out, dp, dmin):
+        a.free()
+    log(f"c3: OR {rates['or']['gbs']:.0f} GB/s over {rates['or']['calls']} levels, min-pred "
+        f"{rates['min_pred']['gbs']:.0f} GB/s, codec encode {rates['encode']['gbs']:.0f} / decode "
+        f"{rates['decode']['gbs']:.0f} GB/s ({time.perf_counter() - t0:.1f} s)")
+    return {"workload": "one rank's share of a 4x4 grid at Kronecker scale 26: 4 x 262,144-word column "
+                        "bitmaps OR-combined per level (densities of the seeded search's levels within the "
+                        "rank's 2^24 columns), the level queue through s4-bp128-d4, 4 x int64[2^24] "
+                        "predecessors min-reduced",
+            "level_densities": dens, **rates}
 
 
 def bench_c2(args, g5):
@@ -607,9 +774,13 @@ def main():
                     help="validate every measured search on the device (c4)")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="c4: pick the batched search's shape parameters before timing")
+    ap.add_argument("--subconfigs", choices=["on", "off"], default="on",
+                    help="c4: also measure the C2 (codec) and C3 (vreduce) sub-records")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--tune-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--tune-json", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--probe", choices=["single", "exchange", "batched"], default=None,
+                    help="a workload for a profiler run (see probe())")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = DEFAULT_SCALE[args.config]
@@ -622,6 +793,8 @@ def main():
         return pmc_probe(args)
     if args.tune_probe:
         return tune_probe(args)
+    if args.probe:
+        return probe(args)
     if args.config == "c3":
         print(json.dumps(bench_c3(args)))
         return 0

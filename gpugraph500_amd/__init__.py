@@ -96,8 +96,12 @@ def lib():
     _sig(L, "g5_msbfs_level_device", I, [P, I, P])
     _sig(L, "g5_msbfs_maps", I, [P, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), P, ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_times", I, [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "g5_msbfs_footprint", I, [P, P, P, P])
+    _sig(L, "g5_mem_info", I, [P, P])
     _sig(L, "g5_bfs_level_device", I, [P, P])
     _sig(L, "g5_diff_u32", I, [P, P, c_i64, ctypes.POINTER(c_i64)])
+    _sig(L, "g5_exchange_profile", I, [I])
+    _sig(L, "g5_exchange_stats", I, [I, P, P, P])
     _sig(L, "g5_validator_create", I, [P, ctypes.POINTER(c_vp)])
     _sig(L, "g5_validator_free", I, [P])
     _sig(L, "g5_validate", I, [P, c_i64, c_i64, P, ctypes.POINTER(I), ctypes.POINTER(c_i64), P,
@@ -143,6 +147,25 @@ def _p(a):
 
 def init(device=0):
     check(lib().g5_init(device))
+
+
+EXCHANGE_UNITS = ("or", "min_pred", "encode", "decode")
+
+
+def exchange_profile(enable=True):
+    """Device-time accounting of the vreduce and codec entry points (resets
+    the totals)."""
+    check(lib().g5_exchange_profile(1 if enable else 0))
+
+
+def exchange_stats():
+    """{unit: {"ms", "bytes", "calls"}} since exchange_profile(True)."""
+    out = {}
+    for u, name in enumerate(EXCHANGE_UNITS):
+        ms, by, calls = ctypes.c_double(), ctypes.c_double(), c_i64()
+        check(lib().g5_exchange_stats(u, ctypes.byref(ms), ctypes.byref(by), ctypes.byref(calls)))
+        out[name] = {"ms": ms.value, "bytes": by.value, "calls": calls.value}
+    return out
 
 
 def diff_u32(a, b, n_words):
@@ -433,6 +456,13 @@ class MSBFS:
         lane_of = np.zeros(max(self.nroots, 1), np.int32)
         check(lib().g5_msbfs_maps(self.h, ctypes.byref(dp), ctypes.byref(dl), _p(lane_of), ctypes.byref(fin)))
         return dp.value, dl.value, lane_of[:self.nroots], bool(fin.value)
+
+    @staticmethod
+    def footprint(graph):
+        """(graph bytes, fixed bytes, bytes per lane) of a batched search over graph."""
+        a, b, c = c_i64(), c_i64(), c_i64()
+        check(lib().g5_msbfs_footprint(graph.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def times(self):
         """(search ms, finalising pass ms) of the last batch run."""

@@ -732,46 +732,81 @@ __global__ __launch_bounds__(256) void k_rt_small(const uint32_t *in, int64_t n,
 
 namespace g5 {
 
+size_t S4Codec::bind(uint8_t *base, int64_t n)
+{
+    const int64_t nblk = n / 128 + 2;
+    const int64_t ngrp = nblk / 16 + 2;
+    const int64_t ncand = (int64_t)g5_s4_max_words((size_t)n) / 4 + 2;
+    size_t off = 0;
+    // pieces start 256 B apart from each other
+    auto take = [&](size_t bytes) -> uint8_t * {
+        uint8_t *p = base ? base + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    uint8_t *words = take((size_t)(nblk + 1) * 8);
+    uint8_t *scan = take((scan_scratch_words(nblk + 1) + 8) * 8);
+    uint8_t *offs = take((size_t)nblk * 8);
+    uint8_t *carry = take((size_t)nblk * 20);
+    uint8_t *carry_scan = take((size_t)(carry_scratch_records(nblk) + 2) * 20);
+    uint8_t *grp = take((size_t)ngrp * 8);
+    uint8_t *nxt = take((size_t)ncand * 4);
+    uint8_t *jmp = take((size_t)ncand * 4);
+    uint8_t *jtmp = take((size_t)ncand * 4);
+    uint8_t *anchor = take((size_t)(ngrp / 32 + 2) * 4);
+    uint8_t *meta = take(16 * 8);
+    uint8_t *nw = take(2 * 8);
+    if (base) {
+        d_base = base;
+        d_words = (int64_t *)words;
+        d_scan = (int64_t *)scan;
+        d_off = (int64_t *)offs;
+        d_carry = (uint32_t *)carry;
+        d_carry_scan = (uint32_t *)carry_scan;
+        d_grp = (int64_t *)grp;
+        d_nxt = (int32_t *)nxt;
+        d_jmp = (int32_t *)jmp;
+        d_jtmp = (int32_t *)jtmp;
+        d_anchor = (int32_t *)anchor;
+        d_meta = (int64_t *)meta;
+        d_nw = (int64_t *)nw;
+        ncand_cap = ncand;
+        cap = n;
+    }
+    return off;
+}
+
 int S4Codec::reserve(int64_t n)
 {
     if (n <= cap && d_meta) return G5_OK;
     release();
-    const int64_t nblk = n / 128 + 2;
-    const int64_t ngrp = nblk / 16 + 2;
-    hipError_t e = hipMalloc(&d_words, (size_t)(nblk + 1) * 8);
-    if (e == hipSuccess) e = hipMalloc(&d_scan, (scan_scratch_words(nblk + 1) + 8) * 8);
-    if (e == hipSuccess) e = hipMalloc(&d_off, (size_t)nblk * 8);
-    if (e == hipSuccess) e = hipMalloc(&d_carry, (size_t)nblk * 20);
-    if (e == hipSuccess) e = hipMalloc(&d_carry_scan, (size_t)(carry_scratch_records(nblk) + 2) * 20);
-    if (e == hipSuccess) e = hipMalloc(&d_grp, (size_t)ngrp * 8);
-    ncand_cap = (int64_t)g5_s4_max_words((size_t)n) / 4 + 2;
-    if (e == hipSuccess) e = hipMalloc(&d_nxt, (size_t)ncand_cap * 4);
-    if (e == hipSuccess) e = hipMalloc(&d_jmp, (size_t)ncand_cap * 4);
-    if (e == hipSuccess) e = hipMalloc(&d_jtmp, (size_t)ncand_cap * 4);
-    if (e == hipSuccess) e = hipMalloc(&d_anchor, (size_t)(ngrp / 32 + 2) * 4);
-    if (e == hipSuccess) e = hipMalloc(&d_meta, 16 * 8);
-    if (e == hipSuccess) e = hipMalloc(&d_nw, 2 * 8);
+    uint8_t *base = nullptr;
+    hipError_t e = hipMalloc(&base, bind(nullptr, n));
     if (e == hipSuccess) e = hipHostMalloc(&h_meta, 16 * 8, hipHostMallocDefault);
     if (e != hipSuccess) {
-        release();
+        if (base) (void)hipFree(base);
+        h_meta = nullptr;
         return fail_hip(e, "S4Codec::reserve", __FILE__, __LINE__);
     }
-    cap = n;
+    bind(base, n);
+    owned = true;
     return G5_OK;
 }
 
 void S4Codec::release()
 {
-    void *dev[] = {d_words, d_scan, d_off, d_carry, d_carry_scan, d_grp, d_meta, d_nxt, d_jmp, d_jtmp, d_anchor, d_nw};
-    for (void *p : dev)
-        if (p) (void)hipFree(p);
-    if (h_meta) (void)hipHostFree(h_meta);
+    if (owned) {
+        if (d_base) (void)hipFree(d_base);
+        if (h_meta) (void)hipHostFree(h_meta);
+        h_meta = nullptr;
+    }
+    owned = false;
+    d_base = nullptr;
     d_words = d_scan = d_off = d_grp = d_meta = nullptr;
     d_carry = d_carry_scan = nullptr;
     d_nxt = d_jmp = d_jtmp = d_anchor = nullptr;
     d_nw = nullptr;
     ncand_cap = 0;
-    h_meta = nullptr;
     cap = 0;
 }
 
@@ -801,11 +836,14 @@ int S4Codec::encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap
                "s4 encode: output capacity %zu below the bound for %lld values", cap_words, (long long)n);
     int rc = reserve(n);
     if (rc) return rc;
+    if (acct) G5_HIP(acct->start());
     rc = encode_launch(d_in, n, d_out, s);
     if (rc) return rc;
+    if (acct) G5_HIP(acct->stop());
     G5_HIP(hipMemcpyAsync(h_meta, d_nw, 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     *nwords = (size_t)h_meta[0];
+    if (acct) G5_HIP(acct->commit(G5_UNIT_ENCODE, 4.0 * (double)n + 4.0 * (double)*nwords, 1));
     return G5_OK;
 }
 
@@ -866,19 +904,25 @@ int S4Codec::decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t
     int rc = reserve((int64_t)cap_values);
     if (rc) return rc;
     G5_HIP(fill_i64(d_nw + 1, 1, (int64_t)nwords, s));
+    if (acct) G5_HIP(acct->start());
     rc = search_launch(d_in, d_nw + 1, (int64_t)nwords, (int64_t)cap_values, (int64_t)nwords <= kSerialWords, s);
     if (rc) return rc;
+    if (acct) G5_HIP(acct->stop());
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
+    if (acct) G5_HIP(acct->commit(G5_UNIT_DECODE, 0, 0));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_STREAM, "s4 decode: malformed stream (code %lld)",
                (long long)h_meta[kMetaErr]);
     const int64_t L = h_meta[kMetaL];
+    if (acct) G5_HIP(acct->start());
     rc = unpack_launch(d_in, d_nw + 1, L, (int64_t)cap_values, d_out, nullptr, s);
     if (rc) return rc;
+    if (acct) G5_HIP(acct->stop());
     G5_HIP(hipMemcpyAsync(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
     G5_REQUIRE(h_meta[kMetaErr] == 0, G5_E_OVERFLOW, "s4 decode: output capacity %zu exceeded", cap_values);
     *n = (size_t)(L + h_meta[kMetaTail]);
+    if (acct) G5_HIP(acct->commit(G5_UNIT_DECODE, 4.0 * (double)nwords + 4.0 * (double)*n, 1));
     return G5_OK;
 }
 
@@ -906,6 +950,14 @@ int S4Codec::roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, s
     return unpack_launch(d_words_out, d_nw, L, n, d_out, d_flag ? d_flag : d_meta + kMetaFlag, s);
 }
 
+int S4Codec::roundtrip_result(size_t *nwords, const char *who)
+{
+    G5_HIP(hipMemcpy(h_meta, d_meta, 8 * 8, hipMemcpyDeviceToHost));
+    G5_REQUIRE(h_meta[kMetaFlag] == 0, G5_E_STREAM, "%s: the decoded stream lost values", who);
+    *nwords = (size_t)h_meta[kMetaNw];
+    return G5_OK;
+}
+
 }  // namespace g5
 
 namespace {
@@ -926,13 +978,23 @@ extern "C" int g5_s4_encode(const uint32_t *d_in, size_t n, uint32_t *d_out, siz
 {
     G5_REQUIRE((d_in || n == 0) && d_out && nwords, G5_E_ARG, "g5_s4_encode: null argument");
     G5_REQUIRE(((uintptr_t)d_out & 15) == 0, G5_E_ARG, "g5_s4_encode: output must be 16-byte aligned");
-    return abi_codec().encode(d_in, (int64_t)n, d_out, out_cap, nwords, nullptr);
+    g5::ExchangeTimer t(nullptr);
+    g5::S4Codec &c = abi_codec();
+    c.acct = &t;
+    const int rc = c.encode(d_in, (int64_t)n, d_out, out_cap, nwords, nullptr);
+    c.acct = nullptr;
+    return rc;
 }
 
 extern "C" int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t out_cap, size_t *n)
 {
     G5_REQUIRE(d_in && (d_out || out_cap == 0) && n, G5_E_ARG, "g5_s4_decode: null argument");
-    return abi_codec().decode(d_in, nwords, d_out, out_cap, n, nullptr);
+    g5::ExchangeTimer t(nullptr);
+    g5::S4Codec &c = abi_codec();
+    c.acct = &t;
+    const int rc = c.decode(d_in, nwords, d_out, out_cap, n, nullptr);
+    c.acct = nullptr;
+    return rc;
 }
 
 extern "C" int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t cap_words, uint32_t *d_out,
@@ -945,8 +1007,5 @@ extern "C" int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words
     // the flag lives in the decoder's record, which the header search clears
     rc = c.roundtrip(d_in, (int64_t)n, d_words, cap_words, d_out, nullptr, nullptr);
     if (rc) return rc;
-    G5_HIP(hipMemcpy(c.h_meta, c.d_meta, 8 * 8, hipMemcpyDeviceToHost));
-    G5_REQUIRE(c.h_meta[kMetaFlag] == 0, G5_E_STREAM, "g5_s4_roundtrip: the decoded stream lost values");
-    *nwords = (size_t)c.h_meta[kMetaNw];
-    return G5_OK;
+    return c.roundtrip_result(nwords, "g5_s4_roundtrip");
 }

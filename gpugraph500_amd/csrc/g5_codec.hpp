@@ -6,8 +6,13 @@
 namespace g5 {
 
 // Scratch for one stream at a time on one stream; sized for up to cap values.
+// All device arrays live in one buffer: reserve() allocates it (owned), or
+// bind() lays them out in a buffer the caller provides (borrowed: never
+// freed here).
 struct S4Codec {
     int64_t cap = 0;               // values
+    uint8_t *d_base = nullptr;     // the buffer holding every device array below
+    bool owned = false;            // d_base and h_meta were allocated by reserve()
     int64_t *d_words = nullptr;    // per-miniblock word counts -> prefix, nblk + 1
     int64_t *d_scan = nullptr;     // int64 scan scratch
     int64_t *d_off = nullptr;      // decode: per-miniblock data offset
@@ -21,9 +26,16 @@ struct S4Codec {
     int64_t *d_meta = nullptr;     // small device record
     int64_t *d_nw = nullptr;       // [0] the last encode's stream length, [1] decode's input length
     int64_t *h_meta = nullptr;     // pinned mirror
+    // accounting of encode() and decode() (g5_exchange_profile), when set
+    ExchangeTimer *acct = nullptr;
 
     int reserve(int64_t n_values);
     void release();
+    // Lays the device arrays for up to n_values out from base (nothing is
+    // set when base is null) and returns the bytes they take; h_meta must be
+    // set by the caller of a borrowed layout.
+    size_t bind(uint8_t *base, int64_t n_values);
+    static size_t scratch_bytes(int64_t n_values) { return S4Codec().bind(nullptr, n_values); }
     // d_in[n] -> d_out; requires cap_words >= g5_s4_max_words(n).
     int encode(const uint32_t *d_in, int64_t n, uint32_t *d_out, size_t cap_words, size_t *nwords,
                hipStream_t s);
@@ -37,6 +49,10 @@ struct S4Codec {
     // the round trip clears first).
     int roundtrip(const uint32_t *d_in, int64_t n, uint32_t *d_words_out, size_t cap_words, uint32_t *d_out,
                   int64_t *d_flag, hipStream_t s);
+    // After a roundtrip with d_flag null, on the null stream: waits for it,
+    // fails with G5_E_STREAM unless exactly its n values came back, and sets
+    // *nwords to the stream's length (who names the caller in the error).
+    int roundtrip_result(size_t *nwords, const char *who);
 
 private:
     int encode_launch(const uint32_t *d_in, int64_t n, uint32_t *d_out, hipStream_t s);

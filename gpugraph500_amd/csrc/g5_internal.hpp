@@ -74,6 +74,32 @@ hipError_t fill_u32(uint32_t *d, int64_t n, uint32_t v, hipStream_t s);
 hipError_t fill_i32(int32_t *d, int64_t n, int32_t v, hipStream_t s);
 hipError_t fill_i64(int64_t *d, int64_t n, int64_t v, hipStream_t s);
 int grid_for(int64_t n, int block, int max_grid = 1 << 20);
+// Device-time accounting of the exchange entry points (g5_exchange_profile).
+// An ExchangeTimer on stream s is live only while accounting is on (else
+// every call does nothing and begin()/end() are null): start() opens an
+// interval (an event on s), stop() closes it (a second event, no wait), and
+// commit(unit, bytes, calls) waits for the closing event and adds the
+// interval, the algorithmic bytes and the calls to the unit's totals.  A
+// single kernel is timed exactly by passing begin() and end() to
+// hipExtLaunchKernelGGL instead of start() / stop() (events on an idle
+// stream would also count the launch's host latency).  An interval that
+// spans a host round trip is split in two: start, stop, commit(unit, 0, 0),
+// start, stop, commit.
+class ExchangeTimer
+{
+public:
+    explicit ExchangeTimer(hipStream_t s);
+    ~ExchangeTimer();
+    hipError_t start();
+    hipError_t stop();
+    hipError_t commit(int unit, double bytes, int64_t calls);
+    hipEvent_t begin() const { return e0_; }
+    hipEvent_t end() const { return e1_; }
+
+private:
+    hipStream_t s_;
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+};
 // Ascending radix sort of n keys on bits [0, end_bit); tmp holds n keys.
 // The result ends in keys.
 hipError_t sort_u64(uint64_t *keys, uint64_t *tmp, int64_t n, int end_bit, hipStream_t s);

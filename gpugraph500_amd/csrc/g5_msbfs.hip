@@ -97,13 +97,15 @@ struct Tune {
 // words (the lanes still searching).  Then set-once slots: the level number,
 // the newly reached queue length (push), the frontier queue length and its
 // edge total (push), and the lanes the pull kernels work on (written by the
-// host before each pull level).
+// host before each pull level) and the lanes in use (batch run: its roots'
+// lanes; continuous mode: the busy lanes), also set before each pull level.
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
+    kUsed = kAccum + 5,
     kCntStride = 32
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -111,7 +113,7 @@ enum MsCnt {
 // kStripes (stripes 256 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kPullLanes < kCntStride, "counter stripe too small");
+static_assert(kUsed < kCntStride, "counter stripe too small");
 // Per-lane frontier edges (counted by push levels, from their new-vertex
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
@@ -130,6 +132,10 @@ struct Relabeled {
     // row's end): the thread kernel's first step reads them from here, 64
     // vertices' in 512 contiguous bytes, instead of from 64 row starts
     uint64_t *head = nullptr;
+    // the same two neighbours' original ids: the parent a hit among them
+    // names, read with the vertex's other coalesced words instead of by a
+    // random new2old gather per hit
+    uint64_t *head_orig = nullptr;
     uint32_t *gdeg = nullptr;
     int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
 };
@@ -320,10 +326,13 @@ struct PullOut {
 
 // One step of a thread's scan: neighbours [j, j + K) of the row ending at e
 // (kHead: the row's first two, j at the row's start, from the head array).
+// A kept hit from the head step keeps its head slot (0, 1) in hu and sets its
+// bit in from_head: its original id comes from head_orig, not from new2old.
 template <int K, bool kActive, bool kHead>
 __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32_t v, uint64_t unv, uint64_t &found,
                                                     int &nh, uint32_t (&hu)[kHits], uint64_t (&hf)[kHits],
-                                                    int64_t &ngath, const uint32_t *__restrict__ cols,
+                                                    uint32_t &from_head, int64_t &ngath,
+                                                    const uint32_t *__restrict__ cols,
                                                     const uint64_t *__restrict__ head,
                                                     const uint64_t *__restrict__ frontier,
                                                     const uint64_t *__restrict__ active, const PullOut &out)
@@ -355,9 +364,10 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 #pragma unroll
             for (int h = 0; h < kHits; ++h)
                 if (h == nh) {
-                    hu[h] = u[k];
+                    hu[h] = kHead ? (uint32_t)k : u[k];
                     hf[h] = f;
                 }
+            if (kHead) from_head |= 1u << nh;
             ++nh;
         } else {
             write_slots(out.parent, out.level, out.new2old[v], f, out.new2old[u[k]], out.lv);
@@ -366,10 +376,79 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
     return j + K;
 }
 
+// The hits a wave step keeps for its write-out: per thread (vertex) the
+// number kept, their parents (after translate(): original ids), their new
+// lanes, the vertex's original id, all lanes found at this level (spilled
+// hits included), and whether the vertex had no lane in use visited before
+// (its first event: the write-out then stores the whole record).
+struct KeptHits {
+    int nh = 0;
+    uint32_t hu[kHits] = {};
+    uint64_t hf[kHits] = {};
+    uint32_t o = 0;
+    uint64_t found = 0;
+    bool first = false;
+};
+
+// Issues the gathers of the kept parents' original ids (head-step hits from
+// head_orig, coalesced; the others from new2old) and of the vertex's own.
+__device__ __forceinline__ void translate(KeptHits &k, uint32_t v, uint32_t from_head,
+                                          const uint32_t *__restrict__ new2old,
+                                          const uint64_t *__restrict__ head_orig)
+{
+    if (!k.nh) return;
+    k.o = new2old[v];
+    const uint64_t ho = from_head ? head_orig[v] : 0ull;
+#pragma unroll
+    for (int h = 0; h < kHits; ++h)
+        if (h < k.nh) k.hu[h] = (from_head >> h & 1) ? (uint32_t)(ho >> (32 * k.hu[h])) : new2old[k.hu[h]];
+}
+
+// The wave writes the kept hits of its 64 vertices out, one vertex per step,
+// lane = root: one coalesced store of the vertex's 256 B parent record and
+// one of its 64 B level record.  A vertex's first event stores the whole
+// records (its other lanes read unreached: no partial sectors); later events
+// store the new lanes only.  Lanes found at this level through spilled hits
+// already hold their parents and are left alone.
+__device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *__restrict__ parent,
+                                          uint8_t *__restrict__ level, uint8_t lv)
+{
+    uint64_t todo = __ballot(k.nh != 0);
+    while (todo) {
+        const int src = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const int nhj = (int)lane_read((uint32_t)k.nh, src);
+        const bool whole = lane_read((uint32_t)k.first, src) != 0;
+        const bool found_here = (lane_read64(k.found, src) >> lane & 1) != 0;
+        uint32_t p = 0;
+        bool mine = false;
+#pragma unroll
+        for (int h = 0; h < kHits; ++h) {
+            if (h >= nhj) break;
+            const uint64_t fk = lane_read64(k.hf[h], src);
+            const uint32_t uk = lane_read(k.hu[h], src);
+            if (fk >> lane & 1) {
+                p = uk;
+                mine = true;
+            }
+        }
+        if (mine || (whole && !found_here)) {
+            const int64_t slot = (int64_t)lane_read(k.o, src) * 64 + lane;
+            __builtin_nontemporal_store(mine ? (int32_t)p : -1, &parent[slot]);
+            __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
+        }
+    }
+}
+
+// The wave's steps are software-pipelined: a step's translation gathers are
+// issued right after its scan, and its write-out waits until the end of the
+// next step's scan, so the gathers complete under that scan instead of
+// stalling the wave.
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
     uint64_t *__restrict__ act_next)
@@ -377,9 +456,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const uint64_t used = (uint64_t)cnt_all[kUsed];
     const PullOut out{parent, level, new2old, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
+    KeptHits pend;  // the previous step's hits, translated, not yet written
     // wave-uniform loop: the wave takes 64 consecutive vertices per step, one
     // word of the next frontier's active bitmap (the bin starts at a multiple
     // of 64)
@@ -387,27 +468,24 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
          base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + lane;
         const uint32_t v = (uint32_t)(first_v + i);
-        uint64_t found = 0;
-        int nh = 0;  // hits kept in hu / hf
-        uint32_t hu[kHits];
-        uint64_t hf[kHits];
-#pragma unroll
-        for (int k = 0; k < kHits; ++k) {
-            hu[k] = 0;
-            hf[k] = 0;
-        }
+        KeptHits cur;
+        uint32_t from_head = 0;
         if (i < nlist) {
             const uint64_t vis = visited[v];
             const uint64_t unv = ~vis & lanes;
+            cur.first = (vis & used) == 0;
             if (unv) {
+                uint64_t found = 0;
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
                 int64_t j = b;
                 if (j < e)
-                    j = thread_scan_step<kThreadFirst, kActive, true>(j, e, v, unv, found, nh, hu, hf, ngath, cols,
-                                                                      head, frontier, active, out);
+                    j = thread_scan_step<kThreadFirst, kActive, true>(j, e, v, unv, found, cur.nh, cur.hu, cur.hf,
+                                                                      from_head, ngath, cols, head, frontier, active,
+                                                                      out);
                 while (j < e && found != unv)
-                    j = thread_scan_step<kThreadUnroll, kActive, false>(j, e, v, unv, found, nh, hu, hf, ngath,
-                                                                        cols, head, frontier, active, out);
+                    j = thread_scan_step<kThreadUnroll, kActive, false>(j, e, v, unv, found, cur.nh, cur.hu, cur.hf,
+                                                                        from_head, ngath, cols, head, frontier,
+                                                                        active, out);
                 j = j < e ? j : e;
                 if (found) {
                     visited[v] = vis | found;
@@ -417,47 +495,18 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                 nedges += j - b;
                 ++nverts;
                 nbits += __popcll(found);
+                cur.found = found;
             }
-            next[v] = found;
+            next[v] = cur.found;
         }
-        live |= found;
-        const uint64_t word = __ballot(found != 0);
+        live |= cur.found;
+        const uint64_t word = __ballot(cur.found != 0);
         if (lane == 0) act_next[(first_v + base) >> 6] = word;
-        // the kept parents' original ids and the vertex's own, all gathers
-        // issued before the write-out
-        uint32_t o = 0;
-        if (nh) {
-            o = new2old[v];
-#pragma unroll
-            for (int k = 0; k < kHits; ++k)
-                if (k < nh) hu[k] = new2old[hu[k]];
-        }
-        // write-out: one vertex per step, lane = root; the vertex's kept
-        // hits are disjoint and cover its found lanes but the spilled ones
-        uint64_t todo = __ballot(nh != 0);
-        while (todo) {
-            const int src = __ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            const int nhj = (int)lane_read((uint32_t)nh, src);
-            uint32_t p = 0;
-            bool mine = false;
-#pragma unroll
-            for (int k = 0; k < kHits; ++k) {
-                if (k >= nhj) break;
-                const uint64_t fk = lane_read64(hf[k], src);
-                const uint32_t uk = lane_read(hu[k], src);
-                if (fk >> lane & 1) {
-                    p = uk;
-                    mine = true;
-                }
-            }
-            if (mine) {
-                const int64_t slot = (int64_t)lane_read(o, src) * 64 + lane;
-                __builtin_nontemporal_store((int32_t)p, &parent[slot]);
-                __builtin_nontemporal_store(out.lv, &level[slot]);
-            }
-        }
+        translate(cur, v, from_head, new2old, head_orig);
+        write_out(pend, lane, parent, level, out.lv);
+        pend = cur;
     }
+    write_out(pend, lane, parent, level, out.lv);
     live = g5_wave_or(live);
     if (lane == 0) s_live[threadIdx.x >> 6] = live;
     block_add(cnt_all + kUnitSlots * bin, {nbits, nedges, nverts, ngath, nnext, enext});  // its barrier orders s_live
@@ -488,7 +537,8 @@ static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
     uint64_t *__restrict__ act_next)
@@ -498,6 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     const uint8_t lv = level_byte(cnt_all);
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const uint64_t used = (uint64_t)cnt_all[kUsed];
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += stride) {
@@ -549,11 +600,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (found >> lane & 1) {
-                // original ids: lanes with the same parent gather the same word
+            // original ids: lanes with the same parent gather the same word;
+            // a vertex's first event stores the whole records
+            const bool mine = (found >> lane & 1) != 0;
+            if (mine || (vis & used) == 0) {
                 const int64_t slot = (int64_t)new2old[v] * 64 + lane;
-                __builtin_nontemporal_store((int32_t)new2old[sp[lane]], &parent[slot]);
-                __builtin_nontemporal_store(lv, &level[slot]);
+                __builtin_nontemporal_store(mine ? (int32_t)new2old[sp[lane]] : -1, &parent[slot]);
+                __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -579,7 +632,8 @@ constexpr int kBlockUnroll = 1;
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
     uint64_t *__restrict__ act_next)
@@ -588,6 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     __shared__ uint64_t s_vis;
     __shared__ uint32_t s_par[64];
     const uint8_t lv = level_byte(cnt_all);
+    const uint64_t used = (uint64_t)cnt_all[kUsed];
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
@@ -655,10 +710,14 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             __syncthreads();
         }
         // s_par is complete: the round's second barrier follows its writes
-        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) {
-            const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
-            __builtin_nontemporal_store((int32_t)new2old[s_par[threadIdx.x]], &parent[slot]);
-            __builtin_nontemporal_store(lv, &level[slot]);
+        if (threadIdx.x < 64 && found) {
+            // a vertex's first event stores the whole records
+            const bool mine = (found >> threadIdx.x & 1) != 0;
+            if (mine || (vis & used) == 0) {
+                const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
+                __builtin_nontemporal_store(mine ? (int32_t)new2old[s_par[threadIdx.x]] : -1, &parent[slot]);
+                __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
+            }
         }
         if (threadIdx.x == 0) {
             next[v] = found;
@@ -940,10 +999,11 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
     }
 }
 
-// One thread: the lanes the next pull kernels work on.
-__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes)
+// One thread: the lanes the next pull kernels work on, and the lanes in use.
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t used)
 {
     cnt[kPullLanes] = (int64_t)lanes;
+    cnt[kUsed] = (int64_t)used;
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -1099,14 +1159,17 @@ __global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ r
 }
 
 // head[v] = the first two neighbours of relabelled row v (0 past its end)
-__global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols, int64_t n,
-                            uint64_t *__restrict__ head)
+__global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+                            const uint32_t *__restrict__ new2old, int64_t n, uint64_t *__restrict__ head,
+                            uint64_t *__restrict__ head_orig)
 {
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
          v += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = rowptr[v], d = rowptr[v + 1] - b;
         const uint64_t lo = d > 0 ? cols[b] : 0u, hi = d > 1 ? cols[b + 1] : 0u;
         head[v] = lo | hi << 32;
+        const uint64_t olo = d > 0 ? new2old[lo] : 0u, ohi = d > 1 ? new2old[hi] : 0u;
+        head_orig[v] = olo | ohi << 32;
     }
 }
 
@@ -1378,7 +1441,8 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
 
 // The three pull kernels of parity p (act: they consult the active bitmap;
 // prof: events between them).  They work on the lanes in cnt[kPullLanes].
-// They share one argument list; only the thread kernel reads `head`.
+// They share one argument list; only the thread kernel reads `head` and
+// `head_orig`.
 void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
 {
     const Relabeled &R = m->rl;
@@ -1392,6 +1456,7 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
+                     (const uint64_t *)R.head_orig, \
                      m->d_visited, front, act, next, m->d_parent, m->d_level, (const uint32_t *)R.new2old, m->d_cnt, \
                      k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
@@ -1567,6 +1632,7 @@ int build_relabel(g5_msbfs *m)
     if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
     if (e == hipSuccess) e = hipMalloc(&R.head, (size_t)std::max<int64_t>(n, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&R.head_orig, (size_t)std::max<int64_t>(n, 1) * 8);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_degree_keys, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)g->d_rowptr, n,
                            keys);
@@ -1588,7 +1654,7 @@ int build_relabel(g5_msbfs *m)
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_row_heads, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)R.rowptr,
-                           (const uint32_t *)R.cols, n, R.head);
+                           (const uint32_t *)R.cols, (const uint32_t *)R.new2old, n, R.head, R.head_orig);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1630,6 +1696,13 @@ void destroy_graphs(g5_msbfs *m)
                 (void)hipGraphExecDestroy(ge);
                 ge = nullptr;
             }
+}
+
+// the lanes in use: a batch run's roots' lanes, continuous mode's busy lanes
+uint64_t used_lanes(const g5_msbfs *m)
+{
+    if (m->cont) return m->busy;
+    return m->nroots == 64 ? ~0ull : (1ull << m->nroots) - 1;
 }
 
 // The lanes a pull level pushes instead: those of the frontier with fewer
@@ -1675,7 +1748,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         int rc = push_level(m, s, p, act_verts, listed_in, m->prof);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm);
+        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, used_lanes(m));
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
             if (rc) return rc;
@@ -1816,6 +1889,28 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     return G5_OK;
 }
 
+extern "C" int g5_msbfs_footprint(const g5_graph *g, int64_t *graph_bytes, int64_t *fixed_bytes,
+                                  int64_t *lane_bytes)
+{
+    G5_REQUIRE(g, G5_E_ARG, "g5_msbfs_footprint: null graph");
+    const int64_t n = std::max<int64_t>(g->n_rows, 64), nnz = std::max<int64_t>(g->nnz, 1);
+    // the records: parent int32 and level byte per (vertex, lane); one bit
+    // per vertex and lane in visited and the two frontier buffers
+    const int64_t per_lane = 5 * n + (3 * n + 7) / 8;
+    const int64_t all = 3 * 8 * n                      // visited, frontier buffers
+                        + 2 * (n / 64) * 8             // active bitmaps
+                        + 3 * 4 * n + 8 * (n + 1)      // push queues, degree prefix
+                        + 64 * 5 * n                   // parent and level records
+                        + kDirtyCap * 8 + kStripes * kCntStride * 8 + kLaneStripes * 64 * 8 +
+                        ((int64_t)g5::scan_scratch_words(n + 1) + 16) * 8 + 64 * 12 + 194 * 8 + 128 * 8 +
+                        3 * 4 * n + 8 * (n + 1) + 4 * nnz  // relabelled copy: ids, degrees, rows
+                        + 2 * 8 * n;                       // head, head_orig
+    if (graph_bytes) *graph_bytes = 8 * (g->n_rows + 1) + 4 * g->nnz + 4 * g->n_rows;
+    if (fixed_bytes) *fixed_bytes = all - 64 * per_lane;
+    if (lane_bytes) *lane_bytes = per_lane;
+    return G5_OK;
+}
+
 extern "C" int g5_msbfs_free(g5_msbfs *m)
 {
     if (!m) return G5_OK;
@@ -1826,7 +1921,7 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
-                   m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty};
+                   m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty,  m->rl.head_orig};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);

@@ -2,6 +2,7 @@
 // device memory helpers, exclusive scan, bitmap <-> sorted queue.
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 
 #include "g5_internal.hpp"
 
@@ -230,10 +231,90 @@ hipError_t queue_to_bitmap(const uint32_t *q, int64_t n, uint64_t *bits, hipStre
 
 extern "C" const char *g5_last_error(void) { return g5::g_err; }
 
+// ---------------------------------------------------------------- exchange accounting
+
+namespace {
+std::mutex g_ex_mu;
+bool g_ex_on = false;
+double g_ex_ms[G5_UNIT_COUNT] = {}, g_ex_bytes[G5_UNIT_COUNT] = {};
+int64_t g_ex_calls[G5_UNIT_COUNT] = {};
+}  // namespace
+
+namespace g5 {
+
+ExchangeTimer::ExchangeTimer(hipStream_t s) : s_(s)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_ex_mu);
+        if (!g_ex_on) return;
+    }
+    if (hipEventCreate(&e0_) != hipSuccess || hipEventCreate(&e1_) != hipSuccess) {
+        if (e0_) (void)hipEventDestroy(e0_);
+        if (e1_) (void)hipEventDestroy(e1_);
+        e0_ = e1_ = nullptr;
+    }
+}
+
+ExchangeTimer::~ExchangeTimer()
+{
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+}
+
+hipError_t ExchangeTimer::start() { return e0_ ? hipEventRecord(e0_, s_) : hipSuccess; }
+
+hipError_t ExchangeTimer::stop() { return e1_ ? hipEventRecord(e1_, s_) : hipSuccess; }
+
+hipError_t ExchangeTimer::commit(int unit, double bytes, int64_t calls)
+{
+    if (!e0_) return hipSuccess;
+    hipError_t e = hipEventSynchronize(e1_);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0_, e1_);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_ex_mu);
+    g_ex_ms[unit] += ms;
+    g_ex_bytes[unit] += bytes;
+    g_ex_calls[unit] += calls;
+    return hipSuccess;
+}
+
+}  // namespace g5
+
+extern "C" int g5_exchange_profile(int enable)
+{
+    std::lock_guard<std::mutex> lk(g_ex_mu);
+    g_ex_on = enable != 0;
+    for (int u = 0; u < G5_UNIT_COUNT; ++u) {
+        g_ex_ms[u] = g_ex_bytes[u] = 0;
+        g_ex_calls[u] = 0;
+    }
+    return G5_OK;
+}
+
+extern "C" int g5_exchange_stats(int unit, double *ms, double *bytes, int64_t *calls)
+{
+    G5_REQUIRE(unit >= 0 && unit < G5_UNIT_COUNT, G5_E_ARG, "g5_exchange_stats: unit %d out of range", unit);
+    std::lock_guard<std::mutex> lk(g_ex_mu);
+    if (ms) *ms = g_ex_ms[unit];
+    if (bytes) *bytes = g_ex_bytes[unit];
+    if (calls) *calls = g_ex_calls[unit];
+    return G5_OK;
+}
+
 extern "C" int g5_init(int device)
 {
     G5_HIP(hipSetDevice(device));
     G5_HIP(hipFree(nullptr));
+    return G5_OK;
+}
+
+extern "C" int g5_mem_info(int64_t *free_bytes, int64_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    G5_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
     return G5_OK;
 }
 

@@ -10,6 +10,8 @@
 // a list of up to kMaxList pointers (the _list forms, passed by value as a
 // kernel argument), and combined in one pass: k reads + 1 write per element,
 // 16-byte loads, grid-stride.
+#include <hip/hip_ext.h>
+
 #include "g5_internal.hpp"
 
 namespace {
@@ -119,15 +121,19 @@ __global__ __launch_bounds__(256) void k_min_pred_list(PtrList<int64_t> m, int k
 
 namespace g5 {
 
-hipError_t vreduce_or(const uint64_t *d_maps, int k, int64_t n, uint64_t *d_out, hipStream_t s)
+// e0 / e1 (both or neither): events the runtime records at the kernel's
+// start and end (hipExtLaunchKernelGGL), for the exchange accounting
+hipError_t vreduce_or(const uint64_t *d_maps, int k, int64_t n, uint64_t *d_out, hipStream_t s, hipEvent_t e0,
+                      hipEvent_t e1)
 {
-    hipLaunchKernelGGL(k_or, grid_for(n / 2 + 1, 256, 8192), 256, 0, s, d_maps, k, n, d_out);
+    hipExtLaunchKernelGGL(k_or, grid_for(n / 2 + 1, 256, 8192), 256, 0, s, e0, e1, 0, d_maps, k, n, d_out);
     return hipGetLastError();
 }
 
-hipError_t vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out, hipStream_t s)
+hipError_t vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1)
 {
-    hipLaunchKernelGGL(k_min_pred, grid_for(n / 2 + 1, 256, 8192), 256, 0, s, d_preds, k, n, d_out);
+    hipExtLaunchKernelGGL(k_min_pred, grid_for(n / 2 + 1, 256, 8192), 256, 0, s, e0, e1, 0, d_preds, k, n, d_out);
     return hipGetLastError();
 }
 
@@ -139,8 +145,10 @@ extern "C" int g5_vreduce_or(const uint64_t *d_maps, int k, int64_t n_words, uin
     G5_REQUIRE(((uintptr_t)d_maps & 15) == 0 && ((uintptr_t)d_out & 15) == 0, G5_E_ARG,
                "g5_vreduce_or: buffers must be 16-byte aligned");
     if (n_words == 0) return G5_OK;
-    G5_HIP(g5::vreduce_or(d_maps, k, n_words, d_out, nullptr));
+    g5::ExchangeTimer t(nullptr);
+    G5_HIP(g5::vreduce_or(d_maps, k, n_words, d_out, nullptr, t.begin(), t.end()));
     G5_HIP(hipStreamSynchronize(nullptr));
+    G5_HIP(t.commit(G5_UNIT_OR, 8.0 * (k + 1) * n_words, 1));
     return G5_OK;
 }
 
@@ -150,8 +158,10 @@ extern "C" int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int
     G5_REQUIRE(((uintptr_t)d_preds & 15) == 0 && ((uintptr_t)d_out & 15) == 0, G5_E_ARG,
                "g5_vreduce_min_pred: buffers must be 16-byte aligned");
     if (n == 0) return G5_OK;
-    G5_HIP(g5::vreduce_min_pred(d_preds, k, n, d_out, nullptr));
+    g5::ExchangeTimer t(nullptr);
+    G5_HIP(g5::vreduce_min_pred(d_preds, k, n, d_out, nullptr, t.begin(), t.end()));
     G5_HIP(hipStreamSynchronize(nullptr));
+    G5_HIP(t.commit(G5_UNIT_MIN_PRED, 8.0 * (k + 1) * n, 1));
     return G5_OK;
 }
 
@@ -167,9 +177,12 @@ extern "C" int g5_vreduce_or_list(const uint64_t *const *d_maps, int k, int64_t 
     }
     G5_REQUIRE(((uintptr_t)d_out & 15) == 0, G5_E_ARG, "g5_vreduce_or_list: output must be 16-byte aligned");
     if (n_words == 0) return G5_OK;
-    hipLaunchKernelGGL(k_or_list, g5::grid_for(n_words / 2 + 1, 256, 8192), 256, 0, nullptr, m, k, n_words, d_out);
+    g5::ExchangeTimer t(nullptr);
+    hipExtLaunchKernelGGL(k_or_list, g5::grid_for(n_words / 2 + 1, 256, 8192), 256, 0, nullptr, t.begin(), t.end(), 0,
+                          m, k, n_words, d_out);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(nullptr));
+    G5_HIP(t.commit(G5_UNIT_OR, 8.0 * (k + 1) * n_words, 1));
     return G5_OK;
 }
 
@@ -183,8 +196,11 @@ extern "C" int g5_vreduce_min_pred_list(const int64_t *const *d_preds, int k, in
         m.p[j] = d_preds[j];
     }
     if (n == 0) return G5_OK;
-    hipLaunchKernelGGL(k_min_pred_list, g5::grid_for(n, 256, 8192), 256, 0, nullptr, m, k, n, d_out);
+    g5::ExchangeTimer t(nullptr);
+    hipExtLaunchKernelGGL(k_min_pred_list, g5::grid_for(n, 256, 8192), 256, 0, nullptr, t.begin(), t.end(), 0, m, k,
+                          n, d_out);
     G5_HIP(hipGetLastError());
     G5_HIP(hipStreamSynchronize(nullptr));
+    G5_HIP(t.commit(G5_UNIT_MIN_PRED, 8.0 * (k + 1) * n, 1));
     return G5_OK;
 }

@@ -3,8 +3,8 @@
 // queries in real time.
 //
 //   g500_serve [-g 18,20,22,24,25,26] [-e 16] [-rate q/s] [-nq queries] [-seed s]
-//              [-maxroots k] [-lanes l] [-window w] [-small k] [-alone] [-compare]
-//              [-dump file]
+//              [-maxroots k] [-lanes l] [-window w] [-small k] [-budget MiB] [-alone]
+//              [-compare] [-dump file]
 //
 //   -rate      mean arrival rate; inter-arrival times are exponential
 //   -maxroots  roots per query drawn uniformly from 1..k (default 8)
@@ -15,6 +15,12 @@
 //   -small     a query of at most k roots that finds its engine idle and
 //              nothing queued behind it runs as single-root searches
 //              (default 4; 0: always the multi-lane search)
+//   -budget    HBM (MiB, fractions allowed) that the running roots of all
+//              graphs may hold at once: a root holds its lane's bytes
+//              (g5_msbfs_footprint) from joining to leaving, and a query
+//              waits until its roots' bytes are free (default: no budget,
+//              lanes only).  Graphs are admitted at start only if they fit
+//              the device's free memory.
 //   -alone     every query by itself on an idle engine, one after another
 //   -compare   after the served run, run every query alone and compare each
 //              root's (nedge, checksum): results must be identical
@@ -46,6 +52,7 @@ struct Opts {
     uint64_t seed = 1;
     bool alone = false, compare = false;
     std::string dump;
+    double budget_mib = -1;
 };
 
 Opts parse(int argc, char **argv)
@@ -70,6 +77,7 @@ Opts parse(int argc, char **argv)
         else if (a == "-lanes") o.lanes = std::stoi(next());
         else if (a == "-window") o.window = std::stoi(next());
         else if (a == "-small") o.small = std::stoi(next());
+        else if (a == "-budget") o.budget_mib = std::stod(next());
         else if (a == "-alone") o.alone = true;
         else if (a == "-compare") o.compare = true;
         else if (a == "-dump") o.dump = next();
@@ -98,8 +106,10 @@ void check(int rc, const char *what)
 int run(const Opts &o)
 {
     check(g5_init(0), "g5_init");
+    std::unique_ptr<MemoryBudget> budget;
+    if (o.budget_mib >= 0) budget.reset(new MemoryBudget((int64_t)(o.budget_mib * 1048576.0)));
     std::vector<std::unique_ptr<Engine>> eng;
-    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window, o.small));
+    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window, o.small, budget.get()));
 
     // the query stream: seeded, independent of the serving run
     std::mt19937_64 rng(o.seed);
@@ -171,7 +181,7 @@ int run(const Opts &o)
         }
         fclose(f);
     }
-    int64_t levels = 0, split = 0, single = 0;
+    int64_t levels = 0, split = 0, single = 0, waits = 0;
     double dev_ms = 0;
     std::string per;
     for (auto &e : eng) {
@@ -179,12 +189,14 @@ int run(const Opts &o)
         levels += st.levels;
         split += st.split_levels;
         single += st.single_roots;
+        waits += st.budget_waits;
         dev_ms += st.device_ms;
         per += (per.empty() ? "" : ", ") + std::string("\"") + std::to_string(e->scale()) + "\": {\"levels\": " +
                std::to_string(st.levels) + ", \"split_levels\": " + std::to_string(st.split_levels) +
                ", \"roots\": " + std::to_string(st.joins) + ", \"single_roots\": " +
-               std::to_string(st.single_roots) + ", \"device_ms\": " +
-               std::to_string(st.device_ms) + "}";
+               std::to_string(st.single_roots) + ", \"budget_waits\": " + std::to_string(st.budget_waits) +
+               ", \"lane_bytes\": " + std::to_string(e->lane_bytes()) + ", \"device_bytes\": " +
+               std::to_string(e->device_bytes()) + ", \"device_ms\": " + std::to_string(st.device_ms) + "}";
     }
     std::string scales;
     for (size_t i = 0; i < o.scales.size(); ++i) scales += (i ? "," : "") + std::to_string(o.scales[i]);
@@ -192,11 +204,13 @@ int run(const Opts &o)
            "\"small\": %d, "
            "\"wall_s\": %.4f, \"achieved_qps\": %.1f, \"gteps\": %.4f, \"latency_ms\": {\"p50\": %.3f, "
            "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"split_levels\": %lld, "
-           "\"single_roots\": %lld, \"device_ms\": %.3f, "
+           "\"single_roots\": %lld, \"device_ms\": %.3f, \"budget_bytes\": %lld, \"budget_peak_bytes\": %lld, "
+           "\"budget_waits\": %lld, "
            "\"mismatches\": %d, \"engines\": {%s}}\n",
            o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, o.small, wall,
            o.nq / wall, nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100),
            (long long)levels, (long long)split, (long long)single, dev_ms,
+           (long long)(budget ? budget->limit() : -1), (long long)(budget ? budget->peak() : -1), (long long)waits,
            mismatches, per.c_str());
     return mismatches > 0 ? 3 : 0;
 }

@@ -25,13 +25,68 @@ double now_seconds()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-Engine::Engine(int scale, int edgefactor, int lanes, int window, int small)
-    : scale_(scale), max_lanes_(lanes), window_(window), small_(small)
+bool MemoryBudget::try_take(int64_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    if (used_ + bytes > limit_) return false;
+    used_ += bytes;
+    peak_ = std::max(peak_, used_);
+    return true;
+}
+
+void MemoryBudget::give(int64_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        used_ -= bytes;
+        ++gen_;
+    }
+    cv_.notify_all();
+}
+
+uint64_t MemoryBudget::gen() const
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return gen_;
+}
+
+int64_t MemoryBudget::peak() const
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return peak_;
+}
+
+void MemoryBudget::wait_for_release(uint64_t seen, double timeout_s)
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return gen_ != seen; });
+}
+
+Engine::Engine(int scale, int edgefactor, int lanes, int window, int small, MemoryBudget *budget)
+    : scale_(scale), max_lanes_(lanes), window_(window), small_(small), budget_(budget)
 {
     if (lanes < 1 || lanes > 64) throw std::invalid_argument("Engine: lanes must be in [1, 64]");
+    // graph admission: the graph (its generation's peak, the edge list and
+    // its sorted copy included, is bounded by the same estimate) and its
+    // search state must fit the device's free memory
+    {
+        const int64_t nv = (int64_t)1 << scale, m = (int64_t)edgefactor << scale;
+        const int64_t need = 8 * (nv + 1) + 4 * 2 * m + 4 * nv     // the graph
+                             + 8 * (nv + 1) + 4 * 2 * m            // its degree-ordered copy
+                             + 64 * 5 * nv + 3 * 8 * nv + 40 * nv;  // records, masks, queues, ids
+        int64_t free_b = 0, total_b = 0;
+        check(g5_mem_info(&free_b, &total_b), "g5_mem_info");
+        if (need > free_b)
+            throw std::runtime_error("graph of scale " + std::to_string(scale) + " not admitted: it needs about " +
+                                     std::to_string(need >> 20) + " MiB of HBM, " + std::to_string(free_b >> 20) +
+                                     " MiB are free");
+    }
     check(g5_graph_kronecker(scale, edgefactor, 1, 2, &g_), "g5_graph_kronecker");
     int64_t nc = 0, nnz = 0, ng = 0, gen = 0;
     check(g5_graph_info(g_, &n_, &nc, &nnz, &ng, &gen), "g5_graph_info");
+    int64_t graph_b = 0, fixed_b = 0;
+    check(g5_msbfs_footprint(g_, &graph_b, &fixed_b, &lane_bytes_), "g5_msbfs_footprint");
+    device_bytes_ = graph_b + fixed_b + 64 * lane_bytes_;
     keys_.resize(64);
     int found = 0;
     check(g5_select_roots(g_, (int64_t)1 << scale, 64, keys_.data(), &found), "g5_select_roots");
@@ -114,6 +169,13 @@ int Engine::admit(std::deque<Query *> &from)
         const int need = (int)q->roots.size();
         if (need > max_lanes_) throw std::invalid_argument("query with more roots than lanes");
         if (free_lanes < need) break;  // FIFO: later queries wait behind it
+        if (budget_ && need * lane_bytes_ > budget_->limit())
+            throw std::invalid_argument("query whose roots need more memory than the whole budget");
+        if (budget_ && !budget_->try_take(need * lane_bytes_)) {
+            if (blocked_ != q) ++stats_.budget_waits;  // its roots' memory is not free yet
+            blocked_ = q;
+            break;
+        }
         from.pop_front();
         free_lanes -= need;
         q->admitted = clock();
@@ -193,6 +255,7 @@ bool Engine::advance()
         }
         busy_ &= ~finished;
         stats_.leaves += popc(finished);
+        if (budget_) budget_->give(popc(finished) * lane_bytes_);
     }
     return busy_ != 0;
 }
@@ -222,8 +285,13 @@ void Engine::loop_body()
             if (stop_ && busy_ == 0 && local.empty()) return;
         }
         if (run_small(local)) continue;
-        admit(local);  // between two levels
-        advance();
+        const uint64_t seen = budget_ ? budget_->gen() : 0;
+        const int joined = admit(local);  // between two levels
+        if (!advance() && !joined && !local.empty() && budget_) {
+            // idle, with queries held back by memory that other engines hold:
+            // wait until some of it is given back (or new queries arrive)
+            budget_->wait_for_release(seen, 0.002);
+        }
     }
 }
 
@@ -231,7 +299,10 @@ void Engine::run_alone(Query *q)
 {
     if (busy_) throw std::logic_error("run_alone on a busy engine");
     std::deque<Query *> one{q};
+    MemoryBudget *b = budget_;
+    budget_ = nullptr;  // a reference run by itself: no other query holds memory
     admit(one);
+    budget_ = b;
     if (!one.empty()) throw std::invalid_argument("query does not fit the engine's lanes");
     while (advance()) {
     }

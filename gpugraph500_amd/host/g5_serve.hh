@@ -21,6 +21,15 @@
 // direction-optimised single-root search, root after root (a few ms at
 // scale 26), instead of a multi-lane search whose pull levels sweep every
 // vertex for a handful of lanes; the results are the same.
+// Admission by memory: with a MemoryBudget shared by the engines, a root in
+// a lane holds that lane's bytes of HBM (g5_msbfs_footprint: its parent and
+// level columns and its bits in the lane masks) against the budget from the
+// moment it joins until its lane leaves, and a query is admitted only when
+// the budget holds all of its roots' bytes (FIFO: later queries wait behind
+// it); a graph is admitted at start-up only if its graph and search state fit
+// the device's free memory (g5_mem_info).  The lanes' memory is allocated
+// when the engine is created, so nothing is allocated while serving; the
+// budget bounds how much of it is in use at once, across all graphs.
 #pragma once
 
 #include <condition_variable>
@@ -48,18 +57,43 @@ struct Query {
     int remaining = 0;
 };
 
+// Bytes of HBM the running roots of all engines may hold at once.
+class MemoryBudget
+{
+public:
+    explicit MemoryBudget(int64_t limit) : limit_(limit) {}
+    int64_t limit() const { return limit_; }
+    // takes `bytes` if they fit, else nothing (returns false)
+    bool try_take(int64_t bytes);
+    void give(int64_t bytes);
+    // waits until bytes were given back since `seen` (a count from gen()) or
+    // the timeout passes
+    void wait_for_release(uint64_t seen, double timeout_s);
+    uint64_t gen() const;
+    int64_t peak() const;
+
+private:
+    int64_t limit_;
+    int64_t used_ = 0, peak_ = 0;
+    uint64_t gen_ = 0;
+    mutable std::mutex mu_;
+    std::condition_variable cv_;
+};
+
 struct EngineStats {
     int64_t levels = 0, joins = 0, leaves = 0;
     int64_t split_levels = 0;  // levels that pulled some lanes and pushed the others
     int64_t single_roots = 0;  // roots answered by the single-root search
+    int64_t budget_waits = 0;  // queries that had to wait because the memory budget was held
     double device_ms = 0;  // sum of the levels' device time
 };
 
 class Engine
 {
 public:
-    // owns the graph (built on the device from the Kronecker generator)
-    Engine(int scale, int edgefactor, int lanes, int window = 1, int small = 0);
+    // owns the graph (built on the device from the Kronecker generator);
+    // budget (may be null: lanes only) is shared by the engines
+    Engine(int scale, int edgefactor, int lanes, int window = 1, int small = 0, MemoryBudget *budget = nullptr);
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -72,6 +106,8 @@ public:
 
     int scale() const { return scale_; }
     int64_t vertices() const { return n_; }
+    int64_t lane_bytes() const { return lane_bytes_; }
+    int64_t device_bytes() const { return device_bytes_; }  // graph + search state held by the engine
     const std::vector<int64_t> &keys() const { return keys_; }  // search keys (degree > 0)
     EngineStats stats() const { return stats_; }
     double clock() const;         // seconds since the shared epoch
@@ -84,6 +120,9 @@ private:
     int small_;
     int64_t level_ = 0;  // levels run by this engine
     int64_t n_ = 0;
+    MemoryBudget *budget_ = nullptr;
+    int64_t lane_bytes_ = 0, device_bytes_ = 0;
+    const Query *blocked_ = nullptr;  // the query the budget last held back
     g5_graph *g_ = nullptr;
     g5_msbfs *m_ = nullptr;
     g5_bfs *b_ = nullptr;  // single-root search for small queries (small > 0)

@@ -46,6 +46,9 @@ typedef struct g5_validator g5_validator;
 const char *g5_last_error(void);
 int g5_init(int device);
 int g5_device_sync(void);
+/* Free and total device memory of the current device (hipMemGetInfo), for
+ * admission by memory. */
+int g5_mem_info(int64_t *free_bytes, int64_t *total_bytes);
 int g5_dmalloc(void **d_ptr, size_t bytes);
 int g5_dfree(void *d_ptr);
 int g5_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
@@ -211,6 +214,13 @@ int g5_bfs_summary(g5_bfs *b, int64_t *nedge, uint64_t *checksum);
  * per vertex. */
 int g5_msbfs_create(g5_graph *g, g5_msbfs **out);
 int g5_msbfs_free(g5_msbfs *m);
+/* The device memory of a batched search over g, for admission by memory
+ * (host arithmetic, nothing is allocated): *graph_bytes held by the graph
+ * itself (offsets, columns, degrees); *fixed_bytes that g5_msbfs_create
+ * allocates whatever the lanes in use; *lane_bytes per lane (its parent and
+ * level columns, 5 bytes per vertex, and its bit in the visited and frontier
+ * masks), allocated for all 64 lanes at creation.  Any pointer may be NULL. */
+int g5_msbfs_footprint(const g5_graph *g, int64_t *graph_bytes, int64_t *fixed_bytes, int64_t *lane_bytes);
 /* Searches all roots; *nlevels = levels of the deepest search, *ms = device
  * time from the start to the final maps (g5_msbfs_maps), as the reference's
  * runBFS window ends with the predecessors final (cpu_2d/main.cpp:419-462,
@@ -343,6 +353,20 @@ int g5_vreduce_or_list(const uint64_t *const *d_maps, int k, int64_t n_words, ui
 int g5_vreduce_min_pred(const int64_t *d_preds, int k, int64_t n, int64_t *d_out);
 /* The same over k <= 64 arrays named by a host array of device pointers. */
 int g5_vreduce_min_pred_list(const int64_t *const *d_preds, int k, int64_t n, int64_t *d_out);
+
+/* Device-time accounting of the exchange entry points called through this
+ * ABI: the vreduce combines (g5_vreduce_*) and the codec's g5_s4_encode and
+ * g5_s4_decode.  g5_exchange_profile(1) starts it (and resets the totals), 0
+ * stops it; g5_exchange_stats reads per unit the device ms of the unit's
+ * kernels, their algorithmic bytes and the calls: OR (k+1)*8*n_words,
+ * min-pred (k+1)*8*n, encode 4n + 4*csize, decode 4*csize + 4n.  A
+ * combine's time is its kernel's, from start to end; the codec's runs from
+ * its first kernel's launch to its last kernel's end (launch gaps included),
+ * and a decode's excludes the host round trip between its header search and
+ * its unpacking. */
+enum { G5_UNIT_OR = 0, G5_UNIT_MIN_PRED = 1, G5_UNIT_ENCODE = 2, G5_UNIT_DECODE = 3, G5_UNIT_COUNT = 4 };
+int g5_exchange_profile(int enable);
+int g5_exchange_stats(int unit, double *ms, double *bytes, int64_t *calls);
 
 /* ---- result validation (validate_bfs_result, validate/validate.h:249-491) ----
  * For a whole (1x1) graph.  *bad is 0 for a valid predecessor map, else the

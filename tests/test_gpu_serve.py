@@ -91,3 +91,40 @@ def test_join_window_does_not_change_results(tmp_path, window):
     rec, rows = _serve(tmp_path, "-g", "11,13", "-rate", "50000", "-nq", "150", "-seed", "9", "-window", window,
                        "-compare")
     assert rec["mismatches"] == 0 and rec["window"] == int(window)
+
+
+def test_memory_budget_holds_queries_back(tmp_path):
+    """Admission by memory: the running roots may hold 3.5 lanes' bytes of the
+    scale-12 search at once.  Queries of up to 3 roots arrive in a burst;
+    those whose roots' bytes are not free wait, and every query completes
+    with the result it has alone."""
+    import gpugraph500_amd as g5
+    g = g5.Graph.kronecker(12)
+    _, _, lane = g5.MSBFS.footprint(g)
+    g.free()
+    budget = 3.5 * lane
+    rec, rows = _serve(tmp_path, "-g", "12", "-rate", "50000", "-nq", "120", "-seed", "9", "-maxroots", "3",
+                       "-small", "0", "-budget", repr(budget / 1048576.0), "-compare")
+    assert rec["mismatches"] == 0 and rec["queries"] == 120
+    assert rec["engines"]["12"]["lane_bytes"] == lane
+    assert rec["budget_peak_bytes"] <= rec["budget_bytes"] < 4 * lane
+    assert rec["budget_peak_bytes"] >= lane
+    assert rec["budget_waits"] > 0
+    assert all(done >= arr for _, _, arr, done, _ in rows)
+
+
+def test_memory_budget_rejects_what_can_never_fit(tmp_path):
+    """A query whose roots need more than the whole budget is an error, and a
+    graph that does not fit the device's free memory is not admitted."""
+    import gpugraph500_amd as g5
+    g = g5.Graph.kronecker(10)
+    _, _, lane = g5.MSBFS.footprint(g)
+    g.free()
+    cmd = [gbuild.SERVE, "-g", "10", "-nq", "20", "-maxroots", "4", "-small", "0", "-seed", "2",
+           "-budget", repr(1.5 * lane / 1048576.0)]
+    print("case", cmd, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "whole budget" in r.stderr
+    # scale 33: about 2.3 TB of search state, checked before anything is built
+    r = subprocess.run([gbuild.SERVE, "-g", "33", "-nq", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "not admitted" in r.stderr

@@ -87,3 +87,17 @@ def test_driver_rejects_bad_options():
     for args in (["-rate", "0"], ["-maxroots", "9", "-lanes", "8"], ["-lanes", "65"], ["-g", ""], ["-x"]):
         r = subprocess.run([gbuild.SERVE] + args, capture_output=True, text=True)
         assert r.returncode == 1 and "g500_serve:" in r.stderr
+
+
+def test_persisted_shapes_round_trip(tmp_path, monkeypatch):
+    """Tuned shapes are kept per graph key in a JSON file and read back."""
+    from gpugraph500_amd import shapes
+    monkeypatch.setattr(shapes, "PATH", str(tmp_path / "tuned.json"))
+    assert shapes.load("s20-h1.2.3") is None
+    rec = {"params": {"thread_max": 64, "wave_max": 2048}, "best_ms": 1.5}
+    shapes.save("s20-h1.2.3", rec)
+    shapes.save("s21-h4.5", {"params": {"thread_max": 32}})
+    assert shapes.load("s20-h1.2.3") == rec
+    assert shapes.load("s21-h4.5")["params"] == {"thread_max": 32}
+    (tmp_path / "tuned.json").write_text("not json")
+    assert shapes.load("s20-h1.2.3") is None

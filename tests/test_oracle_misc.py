@@ -44,3 +44,13 @@ def test_vreduce_closed_forms():
     masked = np.where(preds < 0, np.iinfo(np.int64).max, preds).min(axis=0)
     want = np.where(masked == np.iinfo(np.int64).max, -1, masked)
     assert np.array_equal(O.vreduce_min_pred(preds), want)
+
+
+def test_tuning_roots_follow_the_timed_roots():
+    # the benchmark tunes on the next 64 draws: asking for 128 keys gives the
+    # 64 timed keys first, unchanged, then 64 disjoint ones
+    g = O.build_csr(O.kronecker_edges(14))
+    timed = O.select_roots(g, 1 << 14, 64)
+    both = O.select_roots(g, 1 << 14, 128)
+    assert np.array_equal(both[:64], timed)
+    assert len(both) == 128 and not set(both[64:].tolist()) & set(timed.tolist())
