@@ -125,6 +125,7 @@ def lib():
     _sig(L, "g5_s4_encode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_decode", I, [P, c_sz, P, c_sz, ctypes.POINTER(c_sz)])
     _sig(L, "g5_s4_roundtrip", I, [P, c_sz, P, c_sz, P, ctypes.POINTER(c_sz)])
+    _sig(L, "g5_s4_roundtrip_noalloc", I, [P, c_sz, P, c_sz, P, ctypes.POINTER(c_sz)])
     _sig(L, "g5_vreduce_or", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_min_pred", I, [P, I, c_i64, P])
     _sig(L, "g5_vreduce_or_list", I, [P, I, c_i64, P])

@@ -74,6 +74,9 @@ hipError_t fill_u32(uint32_t *d, int64_t n, uint32_t v, hipStream_t s);
 hipError_t fill_i32(int32_t *d, int64_t n, int32_t v, hipStream_t s);
 hipError_t fill_i64(int64_t *d, int64_t n, int64_t v, hipStream_t s);
 int grid_for(int64_t n, int block, int max_grid = 1 << 20);
+// Once per device, with that device current: the device's address of the
+// static scratch of g5_s4_roundtrip_noalloc (g5_noalloc.hip).
+int noalloc_setup(int device);
 // Device-time accounting of the exchange entry points (g5_exchange_profile).
 // An ExchangeTimer on stream s is live only while accounting is on (else
 // every call does nothing and begin()/end() are null): start() opens an

@@ -306,7 +306,8 @@ extern "C" int g5_init(int device)
 {
     G5_HIP(hipSetDevice(device));
     G5_HIP(hipFree(nullptr));
-    return G5_OK;
+    // the device's address of the no-allocation round trip's scratch
+    return g5::noalloc_setup(device);
 }
 
 extern "C" int g5_mem_info(int64_t *free_bytes, int64_t *total_bytes)

@@ -339,6 +339,17 @@ int g5_s4_decode(const uint32_t *d_in, size_t nwords, uint32_t *d_out, size_t ou
  * G5_E_STREAM unless exactly the n values came back. */
 int g5_s4_roundtrip(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t cap_words, uint32_t *d_out,
                     size_t *nwords);
+/* The same round trip, allocating nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is a static device array of 16 MiB that every GPU the library runs
+ * on holds until the process ends; n is limited to G5_S4_NOALLOC_MAX_VALUES
+ * (G5_E_ARG beyond).  Results, streams and errors are those of
+ * g5_s4_roundtrip.  All calls on one GPU share the array: calls that overlap
+ * on one GPU race. */
+enum { G5_S4_NOALLOC_MAX_VALUES = 1 << 22 };
+int g5_s4_roundtrip_noalloc(const uint32_t *d_in, size_t n, uint32_t *d_words, size_t cap_words, uint32_t *d_out,
+                            size_t *nwords);
 
 /* ---- vreduce combines (vreduce.hpp:41-544, globalbfs.hh:235-524) ----
  * d_maps: k contiguous bitmaps of n_words uint64 each -> d_out = OR. */
