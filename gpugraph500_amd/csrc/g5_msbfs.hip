@@ -32,13 +32,16 @@
 // before.
 //
 // Relabelling: the search runs on a copy of the block whose vertices are
-// renumbered by descending degree (ties by id).  Frontier words of the hubs,
+// renumbered by descending degree class (powers of two; ties by id).
+// Frontier words of the hubs,
 // which most neighbour lists point at, then sit in a small id range that the
 // L2 and the Infinity Cache hold, instead of being scattered over the whole
 // n * 8 bytes.  Every row keeps its neighbours in ascending ORIGINAL id
 // order and parents are recorded as original ids, so the first hit of a
 // scan is still the minimum-id parent.  The degree bins are contiguous id
-// ranges: one thread per vertex up to thread_max neighbours, one wavefront
+// ranges (exactly so for bounds that are powers of two, as the autotuner's
+// are; other bounds move some vertices into a neighbouring bin, which gives
+// the same results): one thread per vertex up to thread_max neighbours, one wavefront
 // (64 neighbours per step, first hits from an exclusive prefix-OR across
 // lanes) up to wave_max, one 256-thread workgroup beyond (Tune; 32 and 2048
 // by default).  Each pull level's launch
@@ -1101,13 +1104,18 @@ __global__ void k_ms_roots(const int64_t *__restrict__ roots, int nroots, const 
     if (r == 0) cnt[kDepth] = 1;
 }
 
-// Degree-order keys: (2^32-1 - degree) << 32 | id, so an ascending sort
-// gives descending degree, ties by id.
+// Degree-class keys: (2^32-1 - class) << 32 | id, so an ascending sort gives
+// descending degree class, ties by id.  Class 0 holds the vertices without
+// neighbours, class c >= 1 the degrees d with 2^(c-2) < d <= 2^(c-1) (1: d = 1,
+// 2: d = 2, 3: d in 3..4, 4: d in 5..8, ...).  Within a class the vertices keep
+// their original order, so the 64 vertices of a wave have ascending original
+// ids and the parent and level records they write lie close together.
 __global__ void k_degree_keys(const int64_t *__restrict__ rowptr, int64_t n, uint64_t *__restrict__ keys)
 {
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = rowptr[v + 1] - rowptr[v];
-        keys[v] = ((uint64_t)(0xFFFFFFFFu - (uint32_t)d) << 32) | (uint64_t)v;
+        const uint32_t cls = d == 0 ? 0u : 1u + (uint32_t)(64 - __clzll((long long)(d - 1)));
+        keys[v] = ((uint64_t)(0xFFFFFFFFu - cls) << 32) | (uint64_t)v;
     }
 }
 
@@ -1789,8 +1797,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             // 16 B row offsets per frontier vertex; per frontier edge and
             // pass (two passes) 4 B column + 8 B frontier word + 8 B visited
             // word + 8 B next word; 16 B visited read-modify-write per newly
-            // reached vertex; 4 B first parent candidate + 4 B parent per new
-            // (vertex, root) pair
+            // reached vertex; 4 B first parent candidate + 4 B parent + 1 B
+            // level per new (vertex, root) pair
             float kms = 0;
             if (pm) G5_HIP(hipEventElapsedTime(&kms, m->ek_push[0], m->ek_push[1]));
             else G5_HIP(hipEventElapsedTime(&kms, m->ek[0], m->ek[1]));
@@ -1798,7 +1806,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             m->k_ms[kPushUnit] += kms;
             m->k_launches[kPushUnit] += 1;
             m->k_bytes[kPushUnit] += 16.0 * h[kFqLen] + 56.0 * E + 16.0 * h[kUnitSlots * kPushUnit + kVisitedVerts] +
-                                     8.0 * h[kUnitSlots * kPushUnit + kNewBits];
+                                     9.0 * h[kUnitSlots * kPushUnit + kNewBits];
             rec[kPushUnit] = kms;
             rec[4 + kPushUnit] = E;
         }
@@ -1806,7 +1814,9 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             // per bin: 16 B row offsets + 8 B visited read + 8 B next write
             // per vertex with an unvisited lane; 4 B column (+ 1 bit of the
             // active bitmap when checked) per edge read; 8 B per frontier
-            // word gathered; 4 B parent per new (vertex, root) pair
+            // word gathered; 4 B parent + 1 B level per new (vertex, root)
+            // pair; per newly reached vertex its original id and at least
+            // one parent's (4 + 4 B)
             for (int k = 0; k < kBins; ++k) {
                 if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
                 float kms = 0;
@@ -1815,7 +1825,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
                 m->k_launches[k] += 1;
                 m->k_bytes[k] += 32.0 * h[kUnitSlots * k + kVisitedVerts] +
                                  (act_check ? 4.125 : 4.0) * h[kUnitSlots * k + kEdgesRead] +
-                                 8.0 * h[kUnitSlots * k + kGathers] + 4.0 * h[kUnitSlots * k + kNewBits];
+                                 8.0 * h[kUnitSlots * k + kGathers] + 5.0 * h[kUnitSlots * k + kNewBits] +
+                                 8.0 * h[kUnitSlots * k + kNextVerts];
                 rec[k] = kms;
                 rec[4 + k] = (double)h[kUnitSlots * k + kEdgesRead];
             }

@@ -163,6 +163,7 @@ def _check_all_lanes(m, ref, roots):
     dict(thread_max=1 << 20, wave_max=1 << 20),  # every vertex in the thread kernel
     dict(thread_max=0, wave_max=1 << 20),        # every vertex in the wavefront kernel
     dict(thread_max=4, wave_max=64, push_ratio=1, active_ratio=1),
+    dict(thread_max=3, wave_max=100),            # bounds inside degree classes: mixed bins
     dict(push_ratio=1000, active_ratio=1000),
     dict(lane_ratio=0),                          # no split levels
     dict(lane_ratio=4, push_ratio=2),            # split levels push most lanes
