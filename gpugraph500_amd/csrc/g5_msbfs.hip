@@ -53,9 +53,10 @@
 // level uint8[n][64], both indexed by the ORIGINAL vertex id, slot = lane.
 // A parent is an original id (root -> root, -1 unreached); a level byte is
 // the distance from the root (255 unreached).  The kernels that find a
-// (vertex, root) pair write both at once: the pull kernels translate a
-// parent's relabelled id through new2old once per distinct parent (a hit),
-// not once per slot, and store each vertex's new slots with one masked,
+// (vertex, root) pair write both at once: the pull kernels take a parent's
+// original id from cols_orig at the hit's row position (the rows they scan),
+// once per distinct parent (a hit), not once per slot, and store each
+// vertex's new slots with one masked,
 // non-temporal store of its 256 B record and one of its 64 B level record
 // (the records are not read again during the search, so they should not
 // displace frontier words from the caches).  Slots of pairs the search did
@@ -131,13 +132,17 @@ struct Relabeled {
     uint32_t *new2old = nullptr, *old2new = nullptr;
     int64_t *rowptr = nullptr;
     uint32_t *cols = nullptr;  // new ids, rows in ascending original-id order
+    // the same entries' original ids: a hit at row position j names its
+    // parent as cols_orig[j], read from the rows the wave is scanning anyway
+    // instead of by a random new2old gather
+    uint32_t *cols_orig = nullptr;
     // per vertex, its row's first two neighbours (low word first; 0 past the
     // row's end): the thread kernel's first step reads them from here, 64
     // vertices' in 512 contiguous bytes, instead of from 64 row starts
     uint64_t *head = nullptr;
     // the same two neighbours' original ids: the parent a hit among them
-    // names, read with the vertex's other coalesced words instead of by a
-    // random new2old gather per hit
+    // names, read with the vertex's other coalesced words instead of from
+    // cols_orig at 64 row starts
     uint64_t *head_orig = nullptr;
     uint32_t *gdeg = nullptr;
     int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
@@ -319,18 +324,22 @@ constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
 
 // the record outputs of a pull level: parent and level records by original
-// id, the level byte of the pairs it reaches, and new2old
+// id, the level byte of the pairs it reaches, new2old and the columns'
+// original ids
 struct PullOut {
     int32_t *__restrict__ parent;
     uint8_t *__restrict__ level;
     const uint32_t *__restrict__ new2old;
+    const uint32_t *__restrict__ cols_orig;
     uint8_t lv;
 };
 
 // One step of a thread's scan: neighbours [j, j + K) of the row ending at e
 // (kHead: the row's first two, j at the row's start, from the head array).
-// A kept hit from the head step keeps its head slot (0, 1) in hu and sets its
-// bit in from_head: its original id comes from head_orig, not from new2old.
+// A kept hit keeps its parent's original id in hu, loaded from cols_orig at
+// the hit's row position, except a hit of the head step: that one keeps its
+// head slot (0, 1) and sets its bit in from_head, and its original id comes
+// from head_orig, which the wave reads coalesced.
 template <int K, bool kActive, bool kHead>
 __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32_t v, uint64_t unv, uint64_t &found,
                                                     int &nh, uint32_t (&hu)[kHits], uint64_t (&hf)[kHits],
@@ -367,20 +376,20 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 #pragma unroll
             for (int h = 0; h < kHits; ++h)
                 if (h == nh) {
-                    hu[h] = kHead ? (uint32_t)k : u[k];
+                    hu[h] = kHead ? (uint32_t)k : out.cols_orig[j + k];
                     hf[h] = f;
                 }
             if (kHead) from_head |= 1u << nh;
             ++nh;
         } else {
-            write_slots(out.parent, out.level, out.new2old[v], f, out.new2old[u[k]], out.lv);
+            write_slots(out.parent, out.level, out.new2old[v], f, out.cols_orig[j + k], out.lv);
         }
     }
     return j + K;
 }
 
 // The hits a wave step keeps for its write-out: per thread (vertex) the
-// number kept, their parents (after translate(): original ids), their new
+// number kept, their parents (after translate(): all original ids), their new
 // lanes, the vertex's original id, all lanes found at this level (spilled
 // hits included), and whether the vertex had no lane in use visited before
 // (its first event: the write-out then stores the whole record).
@@ -393,18 +402,19 @@ struct KeptHits {
     bool first = false;
 };
 
-// Issues the gathers of the kept parents' original ids (head-step hits from
-// head_orig, coalesced; the others from new2old) and of the vertex's own.
+// Issues the loads of the head-step hits' original ids (from head_orig,
+// coalesced; the other hits hold theirs already) and of the vertex's own.
 __device__ __forceinline__ void translate(KeptHits &k, uint32_t v, uint32_t from_head,
                                           const uint32_t *__restrict__ new2old,
                                           const uint64_t *__restrict__ head_orig)
 {
     if (!k.nh) return;
     k.o = new2old[v];
-    const uint64_t ho = from_head ? head_orig[v] : 0ull;
+    if (!from_head) return;
+    const uint64_t ho = head_orig[v];
 #pragma unroll
     for (int h = 0; h < kHits; ++h)
-        if (h < k.nh) k.hu[h] = (from_head >> h & 1) ? (uint32_t)(ho >> (32 * k.hu[h])) : new2old[k.hu[h]];
+        if (h < k.nh && (from_head >> h & 1)) k.hu[h] = (uint32_t)(ho >> (32 * k.hu[h]));
 }
 
 // The wave writes the kept hits of its 64 vertices out, one vertex per step,
@@ -450,7 +460,8 @@ __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig,
+    const uint32_t *__restrict__ cols_orig, uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
@@ -460,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
     const uint64_t used = (uint64_t)cnt_all[kUsed];
-    const PullOut out{parent, level, new2old, level_byte(cnt_all)};
+    const PullOut out{parent, level, new2old, cols_orig, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
     KeptHits pend;  // the previous step's hits, translated, not yet written
@@ -525,7 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
 
 // One wavefront per vertex (thread_max < degree <= wave_max): a step loads
 // 64 neighbours at once; the wave's exclusive prefix-OR gives each root's first hit, the
-// inclusive OR the new bits.  First-hit lanes drop the parent id into the
+// inclusive OR the new bits.  First-hit lanes drop the parent's original id
+// (cols_orig at their row position) into the
 // wave's 64-slot LDS row (slot r = root r); when the vertex is done lane r
 // stores slot r, so the vertex's parents leave in one contiguous store.
 // Each iteration loads kWaveUnroll steps (64 neighbours each) and their
@@ -540,7 +552,8 @@ static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_wave(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig,
+    const uint32_t *__restrict__ cols_orig, uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
@@ -570,6 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
             const int64_t lim = min(e, base + step * kWaveUnroll);
             uint32_t u[kWaveUnroll];
             uint64_t fr[kWaveUnroll];
+            const int64_t j0 = base + lane, jstep = step;
 #pragma unroll
             for (int k = 0; k < kWaveUnroll; ++k) {
                 const int64_t j = base + step * k + lane;
@@ -590,9 +604,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
                 if (__ballot(fresh != 0) == 0) continue;
                 const uint64_t prev = lanes_excl_or(fresh);
                 uint64_t first = fresh & ~prev;
-                while (first) {
-                    sp[__ffsll((long long)first) - 1] = u[k];
-                    first &= first - 1;
+                if (first) {
+                    const uint32_t uo = cols_orig[j0 + jstep * k];
+                    while (first) {
+                        sp[__ffsll((long long)first) - 1] = uo;
+                        first &= first - 1;
+                    }
                 }
                 found |= __shfl(prev | fresh, 63, 64);
             }
@@ -603,12 +620,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // original ids: lanes with the same parent gather the same word;
-            // a vertex's first event stores the whole records
+            // the slots hold original ids; a vertex's first event stores the
+            // whole records
             const bool mine = (found >> lane & 1) != 0;
             if (mine || (vis & used) == 0) {
                 const int64_t slot = (int64_t)new2old[v] * 64 + lane;
-                __builtin_nontemporal_store(mine ? (int32_t)new2old[sp[lane]] : -1, &parent[slot]);
+                __builtin_nontemporal_store(mine ? (int32_t)sp[lane] : -1, &parent[slot]);
                 __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
             }
             __builtin_amdgcn_wave_barrier();
@@ -635,7 +652,8 @@ constexpr int kBlockUnroll = 1;
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_block(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig, uint64_t *__restrict__ visited,
+    const uint64_t *__restrict__ head, const uint64_t *__restrict__ head_orig,
+    const uint32_t *__restrict__ cols_orig, uint64_t *__restrict__ visited,
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
@@ -703,9 +721,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
                     all |= t;
                 }
                 uint64_t first = f[k] & ~(in_wave[k] | before);
-                while (first) {
-                    s_par[__ffsll((long long)first) - 1] = u[k];
-                    first &= first - 1;
+                if (first) {
+                    const uint32_t uo = cols_orig[base - kBlock * kBlockUnroll + kBlock * k + threadIdx.x];
+                    while (first) {
+                        s_par[__ffsll((long long)first) - 1] = uo;
+                        first &= first - 1;
+                    }
                 }
                 earlier |= all;
             }
@@ -718,7 +739,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             const bool mine = (found >> threadIdx.x & 1) != 0;
             if (mine || (vis & used) == 0) {
                 const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
-                __builtin_nontemporal_store(mine ? (int32_t)new2old[s_par[threadIdx.x]] : -1, &parent[slot]);
+                __builtin_nontemporal_store(mine ? (int32_t)s_par[threadIdx.x] : -1, &parent[slot]);
                 __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
             }
         }
@@ -781,13 +802,13 @@ __global__ void k_fq_degrees(const uint32_t *__restrict__ fq, const int64_t *__r
     }
 }
 
-// one pushed edge: u (relabelled id, uo its original id) -> v.  mark (first
+// one pushed edge: u (relabelled id, uo its original id) -> v (vo).  mark (first
 // pass only, when given): the edge that makes next[v] non-zero sets v's bit
 // in mark, the next frontier's active bitmap.  The first pass writes the
 // level bytes of the pairs it sets (every candidate parent of a pair is of
 // the same level).
 template <bool kMin>
-__device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, const uint32_t *__restrict__ new2old,
+__device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, uint32_t vo,
                                           const uint64_t *__restrict__ visited,
                                           const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
                                           int32_t *__restrict__ parent, uint8_t *__restrict__ level, uint8_t lv,
@@ -802,11 +823,11 @@ __device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, c
         const uint64_t fresh = f & ~old;
         // the edge that sets a (v, root) bit first stores its source as the
         // pair's first parent candidate; the kMin pass lowers it
-        if (fresh) write_slots(parent, level, new2old[v], fresh, uo, lv);
+        if (fresh) write_slots(parent, level, vo, fresh, uo, lv);
         if (mark && old == 0) atomicOr((unsigned long long *)&mark[v >> 6], 1ull << (v & 63));
     } else if (f) {
         uint64_t bits = f;  // every bit of f is new at this level (visited excludes older ones)
-        int32_t *rec = &parent[(int64_t)new2old[v] * 64];
+        int32_t *rec = &parent[(int64_t)vo * 64];
         while (bits) {
             const int r = __ffsll((long long)bits) - 1;
             int32_t *slot = &rec[r];
@@ -824,9 +845,10 @@ __device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, c
 template <bool kMin>
 __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
-    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ new2old, const uint64_t *__restrict__ visited,
-    const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
-    uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm, uint64_t *__restrict__ mark)
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ cols_orig, const uint32_t *__restrict__ new2old,
+    const uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
+    int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm,
+    uint64_t *__restrict__ mark)
 {
     __shared__ int64_t s_lo, s_hi;
     __shared__ int32_t s_off[kPushTile];
@@ -863,15 +885,17 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                     else b = mid - 1;
                 }
                 j = a;
-                push_edge<kMin>(s_u[j], s_uo[j], cols[s_rb[j] + e], new2old, visited, frontier, next, parent, level,
+                const int64_t pos = s_rb[j] + e;
+                push_edge<kMin>(s_u[j], s_uo[j], cols[pos], cols_orig[pos], visited, frontier, next, parent, level,
                                 lv, lm, mark);
             }
         } else {
             for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
                 const int64_t si = fq_src(qoff, lo, hi, e);
                 const uint32_t u = fq[si];
-                push_edge<kMin>(u, new2old[u], cols[rowptr[u] + (e - qoff[si])], new2old, visited, frontier, next,
-                                parent, level, lv, lm, mark);
+                const int64_t pos = rowptr[u] + (e - qoff[si]);
+                push_edge<kMin>(u, new2old[u], cols[pos], cols_orig[pos], visited, frontier, next, parent, level, lv,
+                                lm, mark);
             }
         }
         __syncthreads();
@@ -1151,24 +1175,30 @@ __global__ void k_perm(const uint64_t *__restrict__ keys, int64_t n, uint32_t *_
 }
 
 // one wavefront per relabelled row: copy the original row (ascending
-// original ids) mapped to new ids
+// original ids) mapped to new ids, and as it is into ncols_orig
 __global__ __launch_bounds__(256) void k_perm_rows(const int64_t *__restrict__ rowptr,
                                                    const uint32_t *__restrict__ cols,
                                                    const uint32_t *__restrict__ new2old,
                                                    const uint32_t *__restrict__ old2new, int64_t n,
-                                                   const int64_t *__restrict__ nrowptr, uint32_t *__restrict__ ncols)
+                                                   const int64_t *__restrict__ nrowptr, uint32_t *__restrict__ ncols,
+                                                   uint32_t *__restrict__ ncols_orig)
 {
     const int64_t waves = (int64_t)gridDim.x * 4;
     for (int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves) {
         const uint32_t o = new2old[i];
         const int64_t b = rowptr[o], d = rowptr[o + 1] - b, dst = nrowptr[i];
-        for (int64_t j = __lane_id(); j < d; j += 64) ncols[dst + j] = old2new[cols[b + j]];
+        for (int64_t j = __lane_id(); j < d; j += 64) {
+            const uint32_t c = cols[b + j];
+            ncols[dst + j] = old2new[c];
+            ncols_orig[dst + j] = c;
+        }
     }
 }
 
-// head[v] = the first two neighbours of relabelled row v (0 past its end)
+// head[v] = the first two neighbours of relabelled row v (0 past its end),
+// head_orig[v] their original ids
 __global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
-                            const uint32_t *__restrict__ new2old, int64_t n, uint64_t *__restrict__ head,
+                            const uint32_t *__restrict__ cols_orig, int64_t n, uint64_t *__restrict__ head,
                             uint64_t *__restrict__ head_orig)
 {
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
@@ -1176,7 +1206,7 @@ __global__ void k_row_heads(const int64_t *__restrict__ rowptr, const uint32_t *
         const int64_t b = rowptr[v], d = rowptr[v + 1] - b;
         const uint64_t lo = d > 0 ? cols[b] : 0u, hi = d > 1 ? cols[b + 1] : 0u;
         head[v] = lo | hi << 32;
-        const uint64_t olo = d > 0 ? new2old[lo] : 0u, ohi = d > 1 ? new2old[hi] : 0u;
+        const uint64_t olo = d > 0 ? cols_orig[b] : 0u, ohi = d > 1 ? cols_orig[b + 1] : 0u;
         head_orig[v] = olo | ohi << 32;
     }
 }
@@ -1464,7 +1494,7 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
         const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
-                     (const uint64_t *)R.head_orig, \
+                     (const uint64_t *)R.head_orig, (const uint32_t *)R.cols_orig, \
                      m->d_visited, front, act, next, m->d_parent, m->d_level, (const uint32_t *)R.new2old, m->d_cnt, \
                      k, act_next
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
@@ -1524,7 +1554,8 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     if (whole) G5_HIP(g5::fill_u64(act_next, nw, 0, s));
     hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
+                       front, next, m->d_parent,
                        m->d_level, m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr);
     // the newly reached vertices of the lanes lm
     if (!whole)
@@ -1536,7 +1567,8 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                            (const uint64_t *)next, (const int64_t *)R.rowptr, m->d_cnt, m->d_lane);
     hipLaunchKernelGGL(k_push_edges<true>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
-                       (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited, front, next, m->d_parent,
+                       (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
+                       front, next, m->d_parent,
                        m->d_level, m->d_cnt, lm, (uint64_t *)nullptr);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
@@ -1639,6 +1671,7 @@ int build_relabel(g5_msbfs *m)
     if (e == hipSuccess) e = hipMalloc(&R.gdeg, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&R.rowptr, (size_t)(n + 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.cols, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&R.cols_orig, (size_t)std::max<int64_t>(g->nnz, 1) * 4);
     if (e == hipSuccess) e = hipMalloc(&R.head, (size_t)std::max<int64_t>(n, 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&R.head_orig, (size_t)std::max<int64_t>(n, 1) * 8);
     if (e == hipSuccess) {
@@ -1657,12 +1690,12 @@ int build_relabel(g5_msbfs *m)
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_perm_rows, g5::grid_for((n + 3) / 4, 1, 65536), 256, 0, s,
                            (const int64_t *)g->d_rowptr, (const uint32_t *)g->d_cols, (const uint32_t *)R.new2old,
-                           (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols);
+                           (const uint32_t *)R.old2new, n, (const int64_t *)R.rowptr, R.cols, R.cols_orig);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_row_heads, g5::grid_for(n, 256, 16384), 256, 0, s, (const int64_t *)R.rowptr,
-                           (const uint32_t *)R.cols, (const uint32_t *)R.new2old, n, R.head, R.head_orig);
+                           (const uint32_t *)R.cols, (const uint32_t *)R.cols_orig, n, R.head, R.head_orig);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1914,7 +1947,7 @@ extern "C" int g5_msbfs_footprint(const g5_graph *g, int64_t *graph_bytes, int64
                         + 64 * 5 * n                   // parent and level records
                         + kDirtyCap * 8 + kStripes * kCntStride * 8 + kLaneStripes * 64 * 8 +
                         ((int64_t)g5::scan_scratch_words(n + 1) + 16) * 8 + 64 * 12 + 194 * 8 + 128 * 8 +
-                        3 * 4 * n + 8 * (n + 1) + 4 * nnz  // relabelled copy: ids, degrees, rows
+                        3 * 4 * n + 8 * (n + 1) + 2 * 4 * nnz  // relabelled copy: ids, degrees, rows
                         + 2 * 8 * n;                       // head, head_orig
     if (graph_bytes) *graph_bytes = 8 * (g->n_rows + 1) + 4 * g->nnz + 4 * g->n_rows;
     if (fixed_bytes) *fixed_bytes = all - 64 * per_lane;
@@ -1932,7 +1965,8 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
     void *dev[] = {m->d_fq,      m->d_nq,      m->d_qoff,     m->d_act,   m->d_visited, m->d_fa,
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
-                   m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty,  m->rl.head_orig};
+                   m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty,  m->rl.head_orig,
+                   m->rl.cols_orig};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);

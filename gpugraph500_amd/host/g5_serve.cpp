@@ -72,7 +72,7 @@ Engine::Engine(int scale, int edgefactor, int lanes, int window, int small, Memo
     {
         const int64_t nv = (int64_t)1 << scale, m = (int64_t)edgefactor << scale;
         const int64_t need = 8 * (nv + 1) + 4 * 2 * m + 4 * nv     // the graph
-                             + 8 * (nv + 1) + 4 * 2 * m            // its degree-ordered copy
+                             + 8 * (nv + 1) + 4 * 4 * m            // its degree-ordered copy (two id arrays)
                              + 64 * 5 * nv + 3 * 8 * nv + 40 * nv;  // records, masks, queues, ids
         int64_t free_b = 0, total_b = 0;
         check(g5_mem_info(&free_b, &total_b), "g5_mem_info");
