@@ -235,11 +235,12 @@ def probe(args):
     single    the 64 single-root searches (profiled) and the copy calibration kernel
     exchange  the c3 and c2 sub-records by themselves
     batched   one profiled and six timed batched searches with the given shapes
+              (after the parent records' placement, --place trials)
     Each prints one JSON line."""
     import gpugraph500_amd as g5
     g5.init(0)
     g = g5.Graph.kronecker(args.scale)
-    roots = g.select_roots(1 << args.scale, args.roots)
+    roots, tune_roots = root_sets(g, args.scale, args.roots)
     if args.probe == "single":
         b = g5.BFS(g)
         for r in roots[:2].tolist():
@@ -263,6 +264,8 @@ def probe(args):
         if args.tune_json:
             m.tune(**json.loads(args.tune_json))
         m.run(roots)
+        if args.place > 0:
+            m.place(tune_roots, args.place)
         m.profile(True)
         m.run(roots)
         st, lv = m.stats(), m.levels()
@@ -409,6 +412,17 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         tuned["timed_on"] = "roots 1-64 of the benchmark's draw"
         tuned["disjoint"] = bool(len(set(roots.tolist()) & set(tune_roots.tolist())) == 0)
     log(f"msbfs degree bins (workgroup, wavefront, thread per vertex): {m.bins()}")
+    # the parent records' physical placement sets one of two speeds: keep the
+    # faster of a few fresh allocations (setup, on the tuning roots)
+    placement = None
+    if args.place > 0:
+        t0 = time.perf_counter()
+        first, best, moves = m.place(tune_roots, args.place)
+        placement = {"trials": args.place, "ms_before": first, "ms_after": best, "moves": moves,
+                     "seconds": time.perf_counter() - t0, "on": "roots 65-128 of the benchmark's draw",
+                     "what": "g5_msbfs_place: fresh allocations of the parent records (copies), the one with "
+                             "the fastest search kept"}
+        log(f"record placement: {first:.2f} -> {best:.2f} ms ({moves} moves in {args.place} trials)")
     m.profile(True)            # one run as direct launches with per-kernel events
     m.run(roots)
     kst = m.stats()
@@ -525,6 +539,7 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         "single_root_kernel": single_kernel,
         "roofline": roofline,
         "autotune": tuned,
+        "placement": placement,
         "validation": validation,
     }
     m.free()
@@ -772,6 +787,8 @@ def main():
     ap.add_argument("--levels", action="store_true", help="print per-level kernel accounting to stderr")
     ap.add_argument("--validate", choices=["on", "off"], default="on",
                     help="validate every measured search on the device (c4)")
+    ap.add_argument("--place", type=int, default=4,
+                    help="c4: fresh allocations of the parent records tried before timing (0: none)")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="c4: pick the batched search's shape parameters before timing")
     ap.add_argument("--subconfigs", choices=["on", "off"], default="on",

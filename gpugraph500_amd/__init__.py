@@ -118,6 +118,8 @@ def lib():
     _sig(L, "g5_msbfs_tune", I, [P, I, c_i64])
     _sig(L, "g5_msbfs_get_tune", I, [P, I, ctypes.POINTER(c_i64)])
     _sig(L, "g5_msbfs_autotune", I, [P, P, I, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)])
+    _sig(L, "g5_msbfs_place", I, [P, P, I, I, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(I)])
     _sig(L, "g5_msbfs_join", I, [P, P, P, I])
     _sig(L, "g5_msbfs_step", I, [P, ctypes.POINTER(c_u64), ctypes.POINTER(ctypes.c_float)])
     _sig(L, "g5_msbfs_leave", I, [P, c_u64, P, P])
@@ -500,6 +502,17 @@ class MSBFS:
         check(lib().g5_msbfs_autotune(self.h, _p(r), r.size, passes, ctypes.byref(best), ctypes.byref(trials)))
         self.nroots = r.size
         return best.value, trials.value, self.get_tune()
+
+    def place(self, roots, trials=4):
+        """Re-allocates the parent records up to `trials` times and keeps the
+        placement whose search of `roots` is fastest: (ms before, ms after,
+        allocations kept).  The maps then hold the results of `roots`."""
+        r = np.ascontiguousarray(roots, dtype=np.int64)
+        first, best, moves = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        check(lib().g5_msbfs_place(self.h, _p(r), r.size, trials, ctypes.byref(first), ctypes.byref(best),
+                                   ctypes.byref(moves)))
+        self.nroots = r.size
+        return first.value, best.value, moves.value
 
     # ---- continuous mode ----
     def open(self):

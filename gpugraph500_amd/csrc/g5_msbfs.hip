@@ -2526,6 +2526,83 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     return G5_OK;
 }
 
+// Placement of the parent records.  The batched search runs at one of two
+// speeds (about 12 % apart at scale 26) depending on the physical memory
+// behind the 256 B parent records, which take scattered whole and partial
+// record stores: moving only the records to a fresh allocation switches
+// between the two, at the same virtual address as well, while moving any
+// other buffer changes nothing.  g5_msbfs_place tries `trials` fresh
+// allocations (each a copy of the current records, so the records stay
+// valid) and keeps the one whose search of `roots` is fastest (best of two
+// runs; a move must gain 2 %).  It needs the records' size again in free
+// memory while it runs and stops trying when an allocation fails.
+extern "C" int g5_msbfs_place(g5_msbfs *m, const int64_t *roots, int nroots, int trials, float *first_ms,
+                              float *best_ms, int *moves)
+{
+    G5_REQUIRE(m && roots && nroots >= 1 && nroots <= 64 && trials >= 0 && trials <= 16, G5_E_ARG,
+               "g5_msbfs_place: bad arguments");
+    G5_REQUIRE(!(m->cont && m->busy), G5_E_STATE, "g5_msbfs_place: lanes in use");
+    const size_t bytes = (size_t)m->n * 64 * 4;
+    hipStream_t s = m->s;
+    const bool prof = m->prof;
+    m->prof = false;
+    auto time_runs = [&](float *out) -> int {
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            int32_t nl = 0;
+            float ms = 0;
+            int rc = g5_msbfs_run(m, roots, nroots, &nl, &ms);
+            if (rc) return rc;
+            best = std::min(best, ms);
+        }
+        *out = best;
+        return G5_OK;
+    };
+    // the captured pull levels hold the records' address
+    auto swap_records = [&](int32_t *to) -> int {
+        m->d_parent = to;
+        destroy_graphs(m);
+        return capture_graphs(m);
+    };
+    float cur = 0;
+    int kept = 0;
+    int rc = time_runs(&cur);
+    if (first_ms) *first_ms = cur;
+    for (int t = 0; t < trials && !rc; ++t) {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            break;
+        }
+        int32_t *old = m->d_parent;
+        hipError_t e = hipMemcpyAsync(q, old, bytes, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            (void)hipFree(q);
+            rc = g5::fail_hip(e, "g5_msbfs_place: copy", __FILE__, __LINE__);
+            break;
+        }
+        rc = swap_records((int32_t *)q);
+        float ms = 0;
+        if (!rc) rc = time_runs(&ms);
+        if (!rc && ms < cur * 0.98f) {
+            // the copy ran the same roots as the original: both hold their maps
+            (void)hipFree(old);
+            cur = ms;
+            ++kept;
+        } else {
+            const int rc2 = swap_records(old);
+            (void)hipFree(q);
+            if (!rc) rc = rc2;
+        }
+    }
+    m->prof = prof;
+    if (rc) return rc;
+    if (best_ms) *best_ms = cur;
+    if (moves) *moves = kept;
+    return G5_OK;
+}
+
 extern "C" int g5_msbfs_reach(g5_msbfs *m, const int64_t *roots, int n, int64_t *reach)
 {
     G5_REQUIRE(m && n >= 0 && n <= 64 && (n == 0 || (roots && reach)), G5_E_ARG, "g5_msbfs_reach: bad arguments");

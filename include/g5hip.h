@@ -314,6 +314,18 @@ enum {
 int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value);
 int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value);
 int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, int passes, double *best_ms, int *trials);
+/* Placement of the parent records (no counterpart in the reference, whose
+ * host predecessor buffers have no such effect): the batched search runs at
+ * one of two speeds, about 12 % apart at scale 26, depending on the physical
+ * memory behind the parent records.  Tries `trials` (0..16) fresh
+ * allocations of the records (each a copy, so results stay valid) and keeps
+ * the one whose search of roots is fastest (best of two runs; a move must
+ * gain 2 %).  first_ms: the time before, best_ms: after, moves: allocations
+ * kept.  Needs the records' size (256 B per vertex) in free memory while it
+ * runs; stops trying when an allocation fails.  After it the maps hold the
+ * results of roots.  G5_E_STATE while continuous-mode lanes are in use. */
+int g5_msbfs_place(g5_msbfs *m, const int64_t *roots, int nroots, int trials, float *first_ms, float *best_ms,
+                   int *moves);
 int g5_msbfs_open(g5_msbfs *m);
 int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes, int n);
 /* reach[i] = the edges of roots[i]'s neighbours (the frontier edges of its

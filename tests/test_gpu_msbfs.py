@@ -273,6 +273,36 @@ def test_final_maps_as_they_lie_in_hbm(mode):
             assert np.array_equal(lev[:, lane_of[i]], want), (mode, i)
 
 
+def test_record_placement_keeps_results():
+    # fresh allocations of the parent records (copies) keep every map right:
+    # the kept placement holds the placement roots' maps, later runs (an
+    # edgeless root among the earlier ones) the oracle's
+    e, ref, g = kron(12)
+    deg = np.diff(ref.rowptr)
+    iso = np.flatnonzero(deg == 0)
+    sel = O.select_roots(ref, 1 << 12)
+    m = g5.MSBFS(g)
+    m.run([int(iso[0])] + sel[:20].tolist())
+    first, best, moves = m.place(sel[20:60], trials=4)
+    assert 0 < best <= first and 0 <= moves <= 4
+    par, lev, lane_of, final = _download_maps(m, g.n_rows)
+    assert final
+    for i, r in enumerate(sel[20:60].tolist()):
+        opred, olev, _ = O.bfs_bin(ref, r)
+        assert np.array_equal(par[:, lane_of[i]].astype(np.int64), opred), i
+        assert np.array_equal(lev[:, lane_of[i]], np.where(olev < 0, 255, olev).astype(np.uint8)), i
+    m.run(sel)
+    _check_all_lanes(m, ref, sel)
+    par, lev, lane_of, _ = _download_maps(m, g.n_rows)
+    assert (par[iso[0], :] == -1).all() and (lev[iso[0], :] == 255).all()
+    with pytest.raises(g5.G5Error):
+        m.place(sel, trials=17)
+    m.open()
+    m.join([int(sel[0])], [0])
+    with pytest.raises(g5.G5Error):
+        m.place(sel, trials=1)
+
+
 def test_edgeless_root_leaves_no_trace():
     # an edgeless root's own slot holds the root; the next runs, with other
     # roots in that lane, read the vertex as unreached again (also after
