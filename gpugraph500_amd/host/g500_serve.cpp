@@ -3,7 +3,7 @@
 // queries in real time.
 //
 //   g500_serve [-g 18,20,22,24,25,26] [-e 16] [-rate q/s] [-nq queries] [-seed s]
-//              [-maxroots k] [-lanes l] [-window w] [-small k] [-budget MiB] [-alone]
+//              [-maxroots k] [-lanes l] [-window w] [-small k] [-budget MiB] [-place t] [-alone]
 //              [-compare] [-dump file]
 //
 //   -rate      mean arrival rate; inter-arrival times are exponential
@@ -21,6 +21,9 @@
 //              waits until its roots' bytes are free (default: no budget,
 //              lanes only).  Graphs are admitted at start only if they fit
 //              the device's free memory.
+//   -place     at start, each engine tries t fresh placements of its parent
+//              records and keeps the one whose search of its 64 keys is
+//              fastest (g5_msbfs_place; default 0: none)
 //   -alone     every query by itself on an idle engine, one after another
 //   -compare   after the served run, run every query alone and compare each
 //              root's (nedge, checksum): results must be identical
@@ -47,7 +50,7 @@ namespace {
 
 struct Opts {
     std::vector<int> scales{18, 20, 22, 24, 25, 26};
-    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64, window = 1, small = 4;
+    int edgefactor = 16, nq = 400, maxroots = 8, lanes = 64, window = 1, small = 4, place = 0;
     double rate = 1000;
     uint64_t seed = 1;
     bool alone = false, compare = false;
@@ -77,6 +80,7 @@ Opts parse(int argc, char **argv)
         else if (a == "-lanes") o.lanes = std::stoi(next());
         else if (a == "-window") o.window = std::stoi(next());
         else if (a == "-small") o.small = std::stoi(next());
+        else if (a == "-place") o.place = std::stoi(next());
         else if (a == "-budget") o.budget_mib = std::stod(next());
         else if (a == "-alone") o.alone = true;
         else if (a == "-compare") o.compare = true;
@@ -109,7 +113,7 @@ int run(const Opts &o)
     std::unique_ptr<MemoryBudget> budget;
     if (o.budget_mib >= 0) budget.reset(new MemoryBudget((int64_t)(o.budget_mib * 1048576.0)));
     std::vector<std::unique_ptr<Engine>> eng;
-    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window, o.small, budget.get()));
+    for (int s : o.scales) eng.emplace_back(new Engine(s, o.edgefactor, o.lanes, o.window, o.small, budget.get(), o.place));
 
     // the query stream: seeded, independent of the serving run
     std::mt19937_64 rng(o.seed);
@@ -201,13 +205,13 @@ int run(const Opts &o)
     std::string scales;
     for (size_t i = 0; i < o.scales.size(); ++i) scales += (i ? "," : "") + std::to_string(o.scales[i]);
     printf("{\"mode\": \"%s\", \"graphs\": [%s], \"queries\": %d, \"rate_qps\": %.1f, \"lanes\": %d, \"window\": %d, "
-           "\"small\": %d, "
+           "\"small\": %d, \"place\": %d, "
            "\"wall_s\": %.4f, \"achieved_qps\": %.1f, \"gteps\": %.4f, \"latency_ms\": {\"p50\": %.3f, "
            "\"p90\": %.3f, \"p99\": %.3f, \"max\": %.3f}, \"levels\": %lld, \"split_levels\": %lld, "
            "\"single_roots\": %lld, \"device_ms\": %.3f, \"budget_bytes\": %lld, \"budget_peak_bytes\": %lld, "
            "\"budget_waits\": %lld, "
            "\"mismatches\": %d, \"engines\": {%s}}\n",
-           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, o.small, wall,
+           o.alone ? "alone" : "served", scales.c_str(), o.nq, o.rate, o.lanes, o.window, o.small, o.place, wall,
            o.nq / wall, nedge / wall / 1e9, pct(lat, 50), pct(lat, 90), pct(lat, 99), pct(lat, 100),
            (long long)levels, (long long)split, (long long)single, dev_ms,
            (long long)(budget ? budget->limit() : -1), (long long)(budget ? budget->peak() : -1), (long long)waits,

@@ -62,7 +62,7 @@ void MemoryBudget::wait_for_release(uint64_t seen, double timeout_s)
     cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return gen_ != seen; });
 }
 
-Engine::Engine(int scale, int edgefactor, int lanes, int window, int small, MemoryBudget *budget)
+Engine::Engine(int scale, int edgefactor, int lanes, int window, int small, MemoryBudget *budget, int place)
     : scale_(scale), max_lanes_(lanes), window_(window), small_(small), budget_(budget)
 {
     if (lanes < 1 || lanes > 64) throw std::invalid_argument("Engine: lanes must be in [1, 64]");
@@ -94,6 +94,12 @@ Engine::Engine(int scale, int edgefactor, int lanes, int window, int small, Memo
     check(g5_msbfs_create(g_, &m_), "g5_msbfs_create");
     key_reach_.assign(keys_.size(), 0);
     check(g5_msbfs_reach(m_, keys_.data(), (int)keys_.size(), key_reach_.data()), "g5_msbfs_reach");
+    if (place > 0 && !keys_.empty()) {
+        float first = 0, best = 0;
+        int moves = 0;
+        check(g5_msbfs_place(m_, keys_.data(), (int)std::min<size_t>(keys_.size(), 64), place, &first, &best, &moves),
+              "g5_msbfs_place");
+    }
     check(g5_msbfs_open(m_), "g5_msbfs_open");
     // allocated up front: a device allocation in the middle of serving can
     // stall the other engines' streams

@@ -92,8 +92,11 @@ class Engine
 {
 public:
     // owns the graph (built on the device from the Kronecker generator);
-    // budget (may be null: lanes only) is shared by the engines
-    Engine(int scale, int edgefactor, int lanes, int window = 1, int small = 0, MemoryBudget *budget = nullptr);
+    // budget (may be null: lanes only) is shared by the engines; place > 0:
+    // try that many placements of the parent records at start
+    // (g5_msbfs_place on the graph's search keys) and keep the fastest
+    Engine(int scale, int edgefactor, int lanes, int window = 1, int small = 0, MemoryBudget *budget = nullptr,
+           int place = 0);
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
