@@ -17,6 +17,11 @@ roots (cpu_2d/main.cpp:944-950), in GTEPS.
 --config c3, scale 26
     One rank's share of a 4 x 4 grid: the g500_hip driver's emulated grid
     (device exchange, codec on the fold), rank (0, 0)'s share timed.
+--config c1, scale 16
+    The CPU-runnable case: a step is the 64 single-root searches on the
+    GPU; the CPU reference algorithm (the oracle's CPUBFS_bin restatement,
+    one thread) runs every one of the 64 roots beside it, and the batched
+    search of the same roots is reported too.
 
 The batched time runs from the start to the 64 final predecessor and level
 maps in HBM (parent int32[n][64], level uint8[n][64], original ids), as the
@@ -40,7 +45,7 @@ plain copy kernel of known size; null when rocprofv3 is unavailable.  A
 first child (no profiler) finds the shapes; the counter passes and the timed
 run all use those shapes, so the kernels measured are the kernels timed.
 
-    python bench.py [--config c4|c2|c3] [--steps K] [--warmup W] [--scale S]
+    python bench.py [--config c4|c2|c3|c1] [--steps K] [--warmup W] [--scale S]
 """
 import argparse
 import csv
@@ -63,7 +68,7 @@ METRIC = "GTEPS hmean, batched-64 vs single-root, scale 26, 1 MI355X; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 COPY_WORDS = 1 << 26          # copy-kernel calibration: 64 Mi uint64 words read and written
 PULL_KERNELS = ("k_pull_block", "k_pull_wave", "k_pull_thread")
-DEFAULT_SCALE = {"c4": 26, "c2": 22, "c3": 26}
+DEFAULT_SCALE = {"c4": 26, "c2": 22, "c3": 26, "c1": 16}
 
 
 def hmean(x):
@@ -725,6 +730,66 @@ def bench_c2(args, g5):
     return out
 
 
+def bench_c1(args, g5):
+    g = g5.Graph.kronecker(args.scale)
+    roots = g.select_roots(1 << args.scale, args.roots)
+    log(f"graph: scale {args.scale}, {g.nnz} nonzeros; roots {len(roots)}")
+    b = g5.BFS(g)
+    for r in roots[:max(args.warmup, 1)].tolist():
+        b.run(r)
+    b.profile(True)
+    step_ms, per_root_ms, nedge = [], None, None
+    for i in range(args.warmup + args.steps):
+        g5.lib().g5_device_sync()
+        t = time.perf_counter()
+        ms_r, ne_r = [], []
+        for r in roots.tolist():
+            nl, ms = b.run(r)
+            ms_r.append(ms)
+            ne_r.append(b.nedge())
+        g5.lib().g5_device_sync()
+        if i >= args.warmup:
+            step_ms.append((time.perf_counter() - t) * 1e3)
+            per_root_ms = ms_r if per_root_ms is None else [a + c for a, c in zip(per_root_ms, ms_r)]
+        nedge = np.array(ne_r, np.float64)
+    per_root_ms = np.array(per_root_ms) / args.steps
+    value = hmean(nedge / (per_root_ms / 1e3)) / 1e9
+    kst1 = b.kernel_stats()
+    kname = max(kst1, key=lambda k: kst1[k]["ms"])
+    st = kst1[kname]
+    gbs = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
+    b.free()
+    m = g5.MSBFS(g)
+    for _ in range(max(args.warmup, 1)):
+        m.run(roots)
+    bms = [m.run(roots)[1] for _ in range(max(args.steps, 1))]
+    ne_b = m.nedge().astype(np.float64)
+    assert np.array_equal(ne_b, nedge), "batched and single-root TEPS numerators differ"
+    t_batch = float(np.mean(bms))
+    batched = hmean(ne_b / (t_batch / 1e3 / len(roots))) / 1e9
+    log(f"c1: single-root hmean {value:.3f} GTEPS ({per_root_ms.mean():.3f} ms/root), batched {t_batch:.3f} ms "
+        f"per {len(roots)}-root search, {batched:.2f} GTEPS")
+    out = {
+        "metric": METRIC, "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 ids / u64 bitmaps (integer)",
+        "data": "synthetic: Graph500 Kronecker (A=.57 B=C=.19), edgefactor 16, seeds (1,2), generated on the device",
+        "config": {"workload": f"single-root BFS of 64 roots, Kronecker scale {args.scale}, 1x1 block "
+                               f"(the CPU-runnable case)",
+                   "scale": args.scale, "edgefactor": 16, "roots": int(len(roots)), "nnz": int(g.nnz)},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                     "avg_launch_ms": st["ms"] / max(st["launches"], 1),
+                     "algorithmic_bytes_per_launch": st["bytes"] / max(st["launches"], 1)},
+        "batched_ms": t_batch, "batched_hmean_gteps": batched,
+    }
+    if args.cpu_baseline == "on":
+        # every root, whatever the budget: the case is small enough
+        out["cpu_baseline"] = cpu_baseline(g, args.scale, roots, nedge, budget_s=float("inf"))
+        log(f"c1: CPU reference algorithm {out['cpu_baseline']['value']:.4f} GTEPS over all roots")
+    return out
+
+
 def bench_c3(args):
     """The driver runs as a child process; this process touches the GPU only
     afterwards (for the CPU baseline's graph)."""
@@ -776,7 +841,7 @@ def bench_c3(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", choices=["c4", "c2", "c3"], default="c4")
+    ap.add_argument("--config", choices=["c4", "c2", "c3", "c1"], default="c4")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=None)
@@ -823,7 +888,12 @@ def main():
     import gpugraph500_amd as g5
 
     g5.init(0)
-    out = bench_c4(args, g5, traffic, traffic_info, probe_tuned) if args.config == "c4" else bench_c2(args, g5)
+    if args.config == "c4":
+        out = bench_c4(args, g5, traffic, traffic_info, probe_tuned)
+    elif args.config == "c1":
+        out = bench_c1(args, g5)
+    else:
+        out = bench_c2(args, g5)
     print(json.dumps(out))
     return 0
 

@@ -93,6 +93,14 @@ def test_join_window_does_not_change_results(tmp_path, window):
     assert rec["mismatches"] == 0 and rec["window"] == int(window)
 
 
+def test_record_placement_at_start_does_not_change_results(tmp_path):
+    """Engines that pick their parent records' placement at start (-place)
+    serve the same results as alone."""
+    rec, rows = _serve(tmp_path, "-g", "11,13", "-rate", "20000", "-nq", "120", "-seed", "4", "-place", "2",
+                       "-compare")
+    assert rec["mismatches"] == 0 and rec["place"] == 2
+
+
 def test_memory_budget_holds_queries_back(tmp_path):
     """Admission by memory: the running roots may hold 3.5 lanes' bytes of the
     scale-12 search at once.  Queries of up to 3 roots arrive in a burst;
