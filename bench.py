@@ -447,6 +447,10 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
                 "algorithmic_bytes_per_launch": d["bytes"] / max(d["launches"], 1),
                 "traffic_source": traffic_info,
                 "traffic_all_kernels": traffic,
+                # the same kernel's measured HBM bytes per launch over its
+                # launch time: how close its line fills come to the peak
+                "traffic_gbs": (traffic[dom] / (d["ms"] / max(d["launches"], 1) / 1e3) / 1e9
+                                if traffic and dom in traffic and d["ms"] > 0 else None),
                 "all_kernels": {k: {"ms": v["ms"], "launches": v["launches"],
                                     "gbs": (v["bytes"] / (v["ms"] / 1e3) / 1e9) if v["ms"] > 0 else 0.0}
                                 for k, v in kst.items()}}
