@@ -101,15 +101,13 @@ struct Tune {
 // words (the lanes still searching).  Then set-once slots: the level number,
 // the newly reached queue length (push), the frontier queue length and its
 // edge total (push), and the lanes the pull kernels work on (written by the
-// host before each pull level) and the lanes in use (batch run: its roots'
-// lanes; continuous mode: the busy lanes), also set before each pull level.
+// host before each pull level).
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
-    kUsed = kAccum + 5,
     kCntStride = 32
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -117,7 +115,7 @@ enum MsCnt {
 // kStripes (stripes 256 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kUsed < kCntStride, "counter stripe too small");
+static_assert(kPullLanes < kCntStride, "counter stripe too small");
 // Per-lane frontier edges (counted by push levels, from their new-vertex
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
@@ -390,16 +388,14 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 
 // The hits a wave step keeps for its write-out: per thread (vertex) the
 // number kept, their parents (after translate(): all original ids), their new
-// lanes, the vertex's original id, all lanes found at this level (spilled
-// hits included), and whether the vertex had no lane in use visited before
-// (its first event: the write-out then stores the whole record).
+// lanes, the vertex's original id and all lanes found at this level (spilled
+// hits included).
 struct KeptHits {
     int nh = 0;
     uint32_t hu[kHits] = {};
     uint64_t hf[kHits] = {};
     uint32_t o = 0;
     uint64_t found = 0;
-    bool first = false;
 };
 
 // Issues the loads of the head-step hits' original ids (from head_orig,
@@ -418,11 +414,13 @@ __device__ __forceinline__ void translate(KeptHits &k, uint32_t v, uint32_t from
 }
 
 // The wave writes the kept hits of its 64 vertices out, one vertex per step,
-// lane = root: one coalesced store of the vertex's 256 B parent record and
-// one of its 64 B level record.  A vertex's first event stores the whole
-// records (its other lanes read unreached: no partial sectors); later events
-// store the new lanes only.  Lanes found at this level through spilled hits
-// already hold their parents and are left alone.
+// lane = root: one coalesced store of the vertex's new slots in its 256 B
+// parent record and one in its 64 B level record.  Lanes found at this level
+// through spilled hits already hold their parents and are left alone; slots
+// no level reaches are set by k_ms_finalize.  (Storing a vertex's whole
+// records at its first event, the other lanes as unreached, wrote more bytes
+// than it saved in partial sectors once lanes are ranked by reach: the lanes
+// a vertex gains at one level mostly share their sectors.)
 __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *__restrict__ parent,
                                           uint8_t *__restrict__ level, uint8_t lv)
 {
@@ -431,8 +429,6 @@ __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *
         const int src = __ffsll((long long)todo) - 1;
         todo &= todo - 1;
         const int nhj = (int)lane_read((uint32_t)k.nh, src);
-        const bool whole = lane_read((uint32_t)k.first, src) != 0;
-        const bool found_here = (lane_read64(k.found, src) >> lane & 1) != 0;
         uint32_t p = 0;
         bool mine = false;
 #pragma unroll
@@ -445,10 +441,10 @@ __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *
                 mine = true;
             }
         }
-        if (mine || (whole && !found_here)) {
+        if (mine) {
             const int64_t slot = (int64_t)lane_read(k.o, src) * 64 + lane;
-            __builtin_nontemporal_store(mine ? (int32_t)p : -1, &parent[slot]);
-            __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
+            __builtin_nontemporal_store((int32_t)p, &parent[slot]);
+            __builtin_nontemporal_store(lv, &level[slot]);
         }
     }
 }
@@ -470,7 +466,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
-    const uint64_t used = (uint64_t)cnt_all[kUsed];
     const PullOut out{parent, level, new2old, cols_orig, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
@@ -487,7 +482,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
         if (i < nlist) {
             const uint64_t vis = visited[v];
             const uint64_t unv = ~vis & lanes;
-            cur.first = (vis & used) == 0;
             if (unv) {
                 uint64_t found = 0;
                 const int64_t b = rowptr[v], e = rowptr[v + 1];
@@ -564,7 +558,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     const uint8_t lv = level_byte(cnt_all);
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
-    const uint64_t used = (uint64_t)cnt_all[kUsed];
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += stride) {
@@ -620,13 +613,11 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // the slots hold original ids; a vertex's first event stores the
-            // whole records
-            const bool mine = (found >> lane & 1) != 0;
-            if (mine || (vis & used) == 0) {
+            // the slots hold original ids; lane r stores root r's new slot
+            if (found >> lane & 1) {
                 const int64_t slot = (int64_t)new2old[v] * 64 + lane;
-                __builtin_nontemporal_store(mine ? (int32_t)sp[lane] : -1, &parent[slot]);
-                __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
+                __builtin_nontemporal_store((int32_t)sp[lane], &parent[slot]);
+                __builtin_nontemporal_store(lv, &level[slot]);
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -663,7 +654,6 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     __shared__ uint64_t s_vis;
     __shared__ uint32_t s_par[64];
     const uint8_t lv = level_byte(cnt_all);
-    const uint64_t used = (uint64_t)cnt_all[kUsed];
     int64_t ngath = 0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
@@ -734,14 +724,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
             __syncthreads();
         }
         // s_par is complete: the round's second barrier follows its writes
-        if (threadIdx.x < 64 && found) {
-            // a vertex's first event stores the whole records
-            const bool mine = (found >> threadIdx.x & 1) != 0;
-            if (mine || (vis & used) == 0) {
-                const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
-                __builtin_nontemporal_store(mine ? (int32_t)s_par[threadIdx.x] : -1, &parent[slot]);
-                __builtin_nontemporal_store(mine ? lv : (uint8_t)0xFF, &level[slot]);
-            }
+        if (threadIdx.x < 64 && (found >> threadIdx.x & 1)) {
+            const int64_t slot = (int64_t)new2old[v] * 64 + threadIdx.x;
+            __builtin_nontemporal_store((int32_t)s_par[threadIdx.x], &parent[slot]);
+            __builtin_nontemporal_store(lv, &level[slot]);
         }
         if (threadIdx.x == 0) {
             next[v] = found;
@@ -1026,11 +1012,10 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
     }
 }
 
-// One thread: the lanes the next pull kernels work on, and the lanes in use.
-__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t used)
+// One thread: the lanes the next pull kernels work on.
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes)
 {
     cnt[kPullLanes] = (int64_t)lanes;
-    cnt[kUsed] = (int64_t)used;
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -1789,7 +1774,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         int rc = push_level(m, s, p, act_verts, listed_in, m->prof);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, used_lanes(m));
+        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm);
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
             if (rc) return rc;
