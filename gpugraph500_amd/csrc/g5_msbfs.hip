@@ -298,11 +298,11 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 }
 
 // One thread per vertex (degree <= thread_max): a sequential ascending scan.  Each
-// thread keeps its first kHits hits (parent's relabelled id, new lanes) in
-// registers; once the wave's 64 vertices are scanned, each thread translates
-// its kept parents to original ids (one gather per hit, not per slot), and
-// the wave writes them out vertex by vertex, lane r storing root r's parent
-// and level, so every store instruction covers one vertex's contiguous 256 B
+// thread keeps its first kHits hits (the parent's original id, read from
+// cols_orig at the hit's row position or from head_orig, and the new lanes)
+// in registers; once the wave's 64 vertices are scanned, the wave writes them
+// out vertex by vertex, lane r storing root r's parent and level, so every
+// store instruction covers one vertex's new slots in its contiguous 256 B
 // parent record (or 64 B level record) instead of 64 scattered slots.
 // Hits beyond kHits (rare: they need more than kHits distinct first-hit
 // neighbours) are stored directly.  The scan loads kThreadUnroll neighbours
