@@ -449,10 +449,10 @@ __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *
     }
 }
 
-// The wave's steps are software-pipelined: a step's translation gathers are
-// issued right after its scan, and its write-out waits until the end of the
-// next step's scan, so the gathers complete under that scan instead of
-// stalling the wave.
+// The wave's steps are software-pipelined: a step's write-out waits until the
+// end of the next step's scan, so the loads it needs (the vertices' original
+// ids, the head-step parents' from head_orig and the cols_orig loads issued
+// at the hits) complete under that scan instead of stalling the wave.
 template <bool kActive>
 __global__ __launch_bounds__(kBlock) void k_pull_thread(
     int64_t first_v, int64_t nlist, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
