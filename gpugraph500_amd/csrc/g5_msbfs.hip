@@ -108,6 +108,7 @@ enum MsCnt {
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
+    kUsed = kAccum + 5, kPrevPull = kAccum + 6,
     kCntStride = 32
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -115,7 +116,7 @@ enum MsCnt {
 // kStripes (stripes 256 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kPullLanes < kCntStride, "counter stripe too small");
+static_assert(kPrevPull < kCntStride, "counter stripe too small");
 // Per-lane frontier edges (counted by push levels, from their new-vertex
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
@@ -202,6 +203,8 @@ struct g5_msbfs {
     uint32_t *listed = nullptr;
     int64_t listed_len = 0;
     bool listed_ok = false;
+    bool prev_pull = false;            // the last level ran the pull kernels (its level flags hold)
+    uint64_t *d_lvflag = nullptr;      // bit v: v's level record holds the next level's prediction
     bool last_empty = false;  // the level just run found no new (vertex, root) pair
     // continuous mode (g5_msbfs_open): lanes in use
     bool cont = false;
@@ -390,12 +393,16 @@ __device__ __forceinline__ int64_t thread_scan_step(int64_t j, int64_t e, uint32
 // number kept, their parents (after translate(): all original ids), their new
 // lanes, the vertex's original id and all lanes found at this level (spilled
 // hits included).
+constexpr uint32_t kLvFlagged = 1;  // the level bytes of this level's lanes were predicted
+constexpr uint32_t kLvWhole = 2;    // store the whole level record with the next level predicted
 struct KeptHits {
     int nh = 0;
     uint32_t hu[kHits] = {};
     uint64_t hf[kHits] = {};
     uint32_t o = 0;
+    uint32_t lvmode = 0;
     uint64_t found = 0;
+    uint64_t own = 0;  // kLvWhole: the lanes that reached the vertex at the level before
 };
 
 // Issues the loads of the head-step hits' original ids (from head_orig,
@@ -421,6 +428,15 @@ __device__ __forceinline__ void translate(KeptHits &k, uint32_t v, uint32_t from
 // records at its first event, the other lanes as unreached, wrote more bytes
 // than it saved in partial sectors once lanes are ranked by reach: the lanes
 // a vertex gains at one level mostly share their sectors.)
+// The level record is one 64 B sector that a vertex's lanes would write in
+// pieces at two or three levels.  Where all of a vertex's earlier lanes came
+// at the level before (or none did), the wave stores the whole record instead
+// -- this level for the lanes found now, the level before for those, and the
+// next level, predicted, for the lanes still open -- and flags the vertex in
+// lvflag; at the next level (when it runs the pull kernels too) the lanes a
+// flagged vertex gains need no level store, and k_ms_finalize overwrites the
+// prediction of lanes that never come.  A lane found later than predicted is
+// stored as usual.
 __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *__restrict__ parent,
                                           uint8_t *__restrict__ level, uint8_t lv)
 {
@@ -441,9 +457,14 @@ __device__ __forceinline__ void write_out(const KeptHits &k, int lane, int32_t *
                 mine = true;
             }
         }
-        if (mine) {
-            const int64_t slot = (int64_t)lane_read(k.o, src) * 64 + lane;
-            __builtin_nontemporal_store((int32_t)p, &parent[slot]);
+        const uint32_t mode = lane_read(k.lvmode, src);
+        const int64_t slot = (int64_t)lane_read(k.o, src) * 64 + lane;
+        if (mine) __builtin_nontemporal_store((int32_t)p, &parent[slot]);
+        if (mode & kLvWhole) {
+            const uint64_t fs = lane_read64(k.found, src), os = lane_read64(k.own, src);
+            const uint8_t b = (fs >> lane & 1) ? lv : (os >> lane & 1) ? (uint8_t)(lv - 1) : (uint8_t)(lv + 1);
+            __builtin_nontemporal_store(b, &level[slot]);
+        } else if (mine && !(mode & kLvFlagged)) {
             __builtin_nontemporal_store(lv, &level[slot]);
         }
     }
@@ -461,11 +482,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
-    uint64_t *__restrict__ act_next)
+    uint64_t *__restrict__ act_next, uint64_t *__restrict__ lvflag)
 {
     __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const uint64_t used = (uint64_t)cnt_all[kUsed];
+    const bool flags_in = cnt_all[kPrevPull] != 0;
     const PullOut out{parent, level, new2old, cols_orig, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
@@ -477,7 +500,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
          base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + lane;
         const uint32_t v = (uint32_t)(first_v + i);
+        const int64_t fword = (first_v + base) >> 6;
+        const uint64_t fw = flags_in ? lvflag[fword] : 0ull;
         KeptHits cur;
+        cur.lvmode = (fw >> lane & 1) ? kLvFlagged : 0u;
         uint32_t from_head = 0;
         if (i < nlist) {
             const uint64_t vis = visited[v];
@@ -504,10 +530,24 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
                 ++nverts;
                 nbits += __popcll(found);
                 cur.found = found;
+                // the vertex's lanes so far all came at the level before (or
+                // this is its first): its whole level record can be stored,
+                // with the next level predicted for the lanes still open
+                if (found && !cur.lvmode) {
+                    const uint64_t own = frontier[v];
+                    if ((vis & used & ~own) == 0) {
+                        cur.lvmode = kLvWhole;
+                        cur.own = own;
+                    }
+                }
             }
             next[v] = cur.found;
         }
         live |= cur.found;
+        {
+            const uint64_t nf = __ballot(cur.lvmode == kLvWhole);
+            if (lane == 0) lvflag[fword] = nf;
+        }
         const uint64_t word = __ballot(cur.found != 0);
         if (lane == 0) act_next[(first_v + base) >> 6] = word;
         translate(cur, v, from_head, new2old, head_orig);
@@ -551,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
-    uint64_t *__restrict__ act_next)
+    uint64_t *__restrict__ act_next, uint64_t *__restrict__ lvflag)
 {
     __shared__ uint32_t s_par[kBlock / 64][64];
     uint32_t *__restrict__ sp = s_par[threadIdx.x >> 6];
@@ -648,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
     const uint64_t *__restrict__ frontier,
     const uint64_t *__restrict__ active, uint64_t *__restrict__ next, int32_t *__restrict__ parent,
     uint8_t *__restrict__ level, const uint32_t *__restrict__ new2old, int64_t *__restrict__ cnt_all, int bin,
-    uint64_t *__restrict__ act_next)
+    uint64_t *__restrict__ act_next, uint64_t *__restrict__ lvflag)
 {
     __shared__ uint64_t s_wave[kBlockUnroll][kBlock / 64];
     __shared__ uint64_t s_vis;
@@ -1012,10 +1052,14 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
     }
 }
 
-// One thread: the lanes the next pull kernels work on.
-__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes)
+// One thread: the lanes the next pull kernels work on, the lanes in use, and
+// whether the level flags of the thread kernel's vertices are from the
+// level just before (it ran the pull kernels).
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t used, int prev_pull)
 {
     cnt[kPullLanes] = (int64_t)lanes;
+    cnt[kUsed] = (int64_t)used;
+    cnt[kPrevPull] = prev_pull;
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -1481,7 +1525,7 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
                      (const uint64_t *)R.head_orig, (const uint32_t *)R.cols_orig, \
                      m->d_visited, front, act, next, m->d_parent, m->d_level, (const uint32_t *)R.new2old, m->d_cnt, \
-                     k, act_next
+                     k, act_next, m->d_lvflag
         const unsigned grid = k == 0 ? (unsigned)std::min<int64_t>(cnt, 4096)
                               : k == 1 ? g5::grid_for(cnt, kBlock / 64, 16384)
                                        : g5::grid_for(cnt, kBlock, 16384);
@@ -1774,7 +1818,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         int rc = push_level(m, s, p, act_verts, listed_in, m->prof);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm);
+        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, used_lanes(m),
+                           m->prev_pull ? 1 : 0);
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
             if (rc) return rc;
@@ -1857,6 +1902,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     m->act_verts = next_verts;
     m->act_edges = next_edges;
     m->p ^= 1;
+    m->prev_pull = !push;
     return G5_OK;
 }
 
@@ -1884,6 +1930,7 @@ extern "C" int g5_msbfs_create(g5_graph *g, g5_msbfs **out)
     if (e == hipSuccess) e = hipMalloc(&m->d_fa, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_fb, (size_t)n * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_act, (size_t)(n / 64) * 2 * 8);
+    if (e == hipSuccess) e = hipMalloc(&m->d_lvflag, (size_t)std::max<int64_t>(n / 64, 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&m->d_fq, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_fq2, (size_t)n * 4);
     if (e == hipSuccess) e = hipMalloc(&m->d_nq, (size_t)n * 4);
@@ -1951,7 +1998,7 @@ extern "C" int g5_msbfs_free(g5_msbfs *m)
                    m->d_fb,      m->d_parent,  m->d_cnt,      m->d_scratch, m->d_roots, m->d_join,
                    m->d_sums,    m->rl.new2old, m->rl.old2new, m->rl.rowptr, m->rl.cols, m->rl.gdeg,
                    m->d_lane,    m->d_fq2,     m->rl.head,    m->d_level,   m->d_dirty,  m->rl.head_orig,
-                   m->rl.cols_orig};
+                   m->rl.cols_orig, m->d_lvflag};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (m->h_cnt) (void)hipHostFree(m->h_cnt);
@@ -2021,6 +2068,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     // the first frontier is the set of distinct roots; its edge count is not
     // known on the host, and a handful of roots always takes the push path
     m->p = 0;
+    m->prev_pull = false;
     m->act_edges = -1;
     m->front_lanes = nroots == 64 ? ~0ull : (1ull << nroots) - 1;
     m->lane_known = 0;
@@ -2076,6 +2124,7 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->busy = 0;
     m->nroots = 0;
     m->p = 0;
+    m->prev_pull = false;
     m->act_verts = 0;
     m->act_edges = 0;
     m->front_lanes = 0;
@@ -2133,6 +2182,7 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
         if (added[2 + i]) m->front_lanes |= 1ull << lanes[i];
     }
     m->split_ok = true;
+    m->prev_pull = false;
     return G5_OK;
 }
 
