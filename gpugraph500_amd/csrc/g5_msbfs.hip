@@ -25,10 +25,13 @@
 // frontier is non-empty, and right after a push level (which counts each
 // lane's frontier edges) or after lanes joined (whose frontier is their
 // root), the lanes with fewer than nnz / lane_ratio frontier edges are
-// pushed instead in the same level: the pull kernels run on the other
-// lanes, then the push kernels, restricted to the pushed lanes, OR their
-// bits into next and set their parents with atomicMin.  Lanes are disjoint,
-// so each (vertex, root) pair gets the same level and minimum-id parent as
+// pushed instead in the same level: the block and wave kernels run on the
+// other lanes, then the push kernels, restricted to the pushed lanes and to
+// targets in the block and wave bins, OR their bits into next and set their
+// parents with atomicMin.  The thread kernel's rows are short and scanned to
+// the end at such a level anyway, so it pulls every lane, the pushed ones
+// included, and the push skips its vertices.  Each (vertex, root) pair is
+// found by one kernel, so it gets the same level and minimum-id parent as
 // before.
 //
 // Relabelling: the search runs on a copy of the block whose vertices are
@@ -100,23 +103,24 @@ struct Tune {
 // the next frontier's vertices and edges; then the OR of the next frontier's
 // words (the lanes still searching).  Then set-once slots: the level number,
 // the newly reached queue length (push), the frontier queue length and its
-// edge total (push), and the lanes the pull kernels work on (written by the
-// host before each pull level).
+// edge total (push), the lanes the block and wave kernels work on and those
+// the thread kernel works on (all written by the host before each pull
+// level), the lanes in use and whether the level flags hold.
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
-    kUsed = kAccum + 5, kPrevPull = kAccum + 6,
-    kCntStride = 32
+    kUsed = kAccum + 5, kPrevPull = kAccum + 6, kThreadLanes = kAccum + 7,
+    kCntStride = 40
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
 // the accumulated slots are striped: workgroup b adds into stripe b %
-// kStripes (stripes 256 B apart) and k_level_end folds the stripes into
+// kStripes (stripes 320 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kPrevPull < kCntStride, "counter stripe too small");
+static_assert(kThreadLanes < kCntStride, "counter stripe too small");
 // Per-lane frontier edges (counted by push levels, from their new-vertex
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
@@ -318,8 +322,11 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // each in a line of its own once rows exceed 32 neighbours.
 // kActive (all pull kernels): skip neighbours whose bit in the active bitmap
 // is clear before gathering their frontier word (pays only when few sources
-// are active).  All pull kernels work on the lanes in cnt[kPullLanes] only:
-// a vertex is done when every unvisited lane among them is found.
+// are active).  The block and wave kernels work on the lanes in
+// cnt[kPullLanes] only, the thread kernel on those in cnt[kThreadLanes] (the
+// same, except on a split level: there it takes every lane, the pushed ones
+// too, since its rows are scanned to the end anyway): a vertex is done when
+// every unvisited lane among them is found.
 constexpr int kHits = 4;
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4; 12 after the short first step: the same
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
 {
     __shared__ uint64_t s_live[kBlock / 64];
     const int lane = __lane_id();
-    const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const uint64_t lanes = (uint64_t)cnt_all[kThreadLanes];
     const uint64_t used = (uint64_t)cnt_all[kUsed];
     const bool flags_in = cnt_all[kPrevPull] != 0;
     const PullOut out{parent, level, new2old, cols_orig, level_byte(cnt_all)};
@@ -793,7 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_block(
 // kPushTile consecutive edges of the frontier and finds each edge's source by
 // binary search in the frontier's degree prefix.  The push kernels work on
 // the lanes in lm (all lanes on push levels, the pushed lanes of a split
-// level).
+// level) and the targets below vend (all on push levels; on a split level
+// the block and wave bins, whose kernels leave the pushed lanes out).
 // a push level clears next entry by entry when the list is below n / kListedFrac
 constexpr int64_t kListedFrac = 32;
 constexpr int kPushTile = 1024;  // 20 KB of LDS per block (2048: 40 KB, 4 waves per SIMD)
@@ -874,7 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ cols_orig, const uint32_t *__restrict__ new2old,
     const uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
     int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm,
-    uint64_t *__restrict__ mark)
+    uint64_t *__restrict__ mark, int64_t vend)
 {
     __shared__ int64_t s_lo, s_hi;
     __shared__ int32_t s_off[kPushTile];
@@ -912,16 +920,20 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                 }
                 j = a;
                 const int64_t pos = s_rb[j] + e;
-                push_edge<kMin>(s_u[j], s_uo[j], cols[pos], cols_orig[pos], visited, frontier, next, parent, level,
-                                lv, lm, mark);
+                const uint32_t v = cols[pos];
+                if (v < vend)
+                    push_edge<kMin>(s_u[j], s_uo[j], v, cols_orig[pos], visited, frontier, next, parent, level, lv,
+                                    lm, mark);
             }
         } else {
             for (int64_t e = t0 + threadIdx.x; e < t1; e += kBlock) {
                 const int64_t si = fq_src(qoff, lo, hi, e);
                 const uint32_t u = fq[si];
                 const int64_t pos = rowptr[u] + (e - qoff[si]);
-                push_edge<kMin>(u, new2old[u], cols[pos], cols_orig[pos], visited, frontier, next, parent, level, lv,
-                                lm, mark);
+                const uint32_t v = cols[pos];
+                if (v < vend)
+                    push_edge<kMin>(u, new2old[u], v, cols_orig[pos], visited, frontier, next, parent, level, lv,
+                                    lm, mark);
             }
         }
         __syncthreads();
@@ -1052,12 +1064,14 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
     }
 }
 
-// One thread: the lanes the next pull kernels work on, the lanes in use, and
-// whether the level flags of the thread kernel's vertices are from the
-// level just before (it ran the pull kernels).
-__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t used, int prev_pull)
+// One thread: the lanes the next block and wave kernels work on, those the
+// next thread kernel works on, the lanes in use, and whether the level flags
+// of the thread kernel's vertices are from the level just before (it ran the
+// pull kernels).
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t thread_lanes, uint64_t used, int prev_pull)
 {
     cnt[kPullLanes] = (int64_t)lanes;
+    cnt[kThreadLanes] = (int64_t)thread_lanes;
     cnt[kUsed] = (int64_t)used;
     cnt[kPrevPull] = prev_pull;
 }
@@ -1507,7 +1521,8 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
 }
 
 // The three pull kernels of parity p (act: they consult the active bitmap;
-// prof: events between them).  They work on the lanes in cnt[kPullLanes].
+// prof: events between them).  They work on the lanes in cnt[kPullLanes]
+// (the thread kernel: cnt[kThreadLanes]).
 // They share one argument list; only the thread kernel reads `head` and
 // `head_orig`.
 void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
@@ -1565,7 +1580,8 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
 // vertices as it goes -- the edge that makes next[v] non-zero sets v's
 // active bit, in act_next cleared beforehand -- and counts them from their
 // queue, so it makes no pass over all n words; a split level builds the
-// bitmaps from next.
+// bitmaps from next, and its newly reached vertices are those of the block
+// and wave bins.
 int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int64_t fq_bound, uint64_t lm,
                uint32_t *fq)
 {
@@ -1576,6 +1592,10 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     const unsigned agrid = g5::grid_for(nw, 16, 4096);
     const bool whole = lm == ~0ull;
+    // a split level pushes only into the block and wave bins: the thread
+    // kernel pulls every lane of its vertices
+    const int64_t vend = whole ? m->n : R.bin_first[2];
+    const int64_t nw_push = whole ? nw : (vend + 63) / 64;
     G5_HIP(g5::bitmap_to_queue(act_front, nw, fq, m->d_cnt + kFqLen, m->d_scratch, s));
     hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fq_bound, 256, 8192), 256, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff, fq_bound);
@@ -1585,12 +1605,12 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
                        front, next, m->d_parent,
-                       m->d_level, m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr);
+                       m->d_level, m->d_cnt, lm, whole ? act_next : (uint64_t *)nullptr, vend);
     // the newly reached vertices of the lanes lm
-    if (!whole)
-        hipLaunchKernelGGL((k_build_active<false, false>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, lm);
-    G5_HIP(g5::bitmap_to_queue(act_next, nw, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
+    if (!whole && nw_push)
+        hipLaunchKernelGGL((k_build_active<false, false>), g5::grid_for(nw_push, 16, 4096), kBlock, 0, s,
+                           (const uint64_t *)next, nw_push, (const int64_t *)R.rowptr, act_next, m->d_cnt, lm);
+    G5_HIP(g5::bitmap_to_queue(act_next, nw_push, m->d_nq, m->d_cnt + kNq, m->d_scratch, s));
     if (whole)
         hipLaunchKernelGGL(k_nq_counts, g5::grid_for(m->n, kBlock, 2048), kBlock, 0, s, (const uint32_t *)m->d_nq,
                            (const uint64_t *)next, (const int64_t *)R.rowptr, m->d_cnt, m->d_lane);
@@ -1598,7 +1618,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
                        front, next, m->d_parent,
-                       m->d_level, m->d_cnt, lm, (uint64_t *)nullptr);
+                       m->d_level, m->d_cnt, lm, (uint64_t *)nullptr, vend);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
     // the whole next frontier: active bitmap, counts, live lanes, lane edges
@@ -1638,9 +1658,10 @@ int push_level(g5_msbfs *m, hipStream_t s, int p, int64_t fqn, bool listed_in, b
     return G5_OK;
 }
 
-// One split level on parity p: the pull kernels on the lanes in
-// cnt[kPullLanes], then the push kernels on the lanes pm (disjoint from
-// them).  fqn = active vertices of the whole frontier.
+// One split level on parity p: the block and wave kernels on the lanes in
+// cnt[kPullLanes] and the thread kernel on every frontier lane, then the
+// push kernels on the lanes pm (disjoint from cnt[kPullLanes]) into the
+// block and wave bins only.  fqn = active vertices of the whole frontier.
 int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, int64_t fqn, bool prof)
 {
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
@@ -1818,7 +1839,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         int rc = push_level(m, s, p, act_verts, listed_in, m->prof);
         if (rc) return rc;
     } else {
-        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, used_lanes(m),
+        hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, m->front_lanes, used_lanes(m),
                            m->prev_pull ? 1 : 0);
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);

@@ -180,14 +180,24 @@ def test_shape_parameters_do_not_change_results(shape):
     _check_all_lanes(m, ref, roots)
 
 
-@pytest.mark.parametrize("scale", [12, 14])
-def test_split_levels_match_oracle(scale):
+@pytest.mark.parametrize("scale,shape", [
+    (12, {}), (14, {}),
+    # every vertex in the thread kernel: a split level pushes into no bin,
+    # the thread kernel pulls every lane
+    (14, dict(thread_max=1 << 20, wave_max=1 << 20)),
+    # no thread bin: the split level's push reaches every target
+    (14, dict(thread_max=0, wave_max=64)),
+])
+def test_split_levels_match_oracle(scale, shape):
     # a pull level after a push level pushes the lanes with few frontier
-    # edges; every lane ratio gives the oracle's levels and parents, and some
-    # ratio splits at least one level
+    # edges (into the block and wave bins; the thread kernel pulls them);
+    # every lane ratio gives the oracle's levels and parents, and some ratio
+    # splits at least one level
     e, ref, g = kron(scale)
     roots = O.select_roots(ref, 1 << scale)
     m = g5.MSBFS(g)
+    if shape:
+        m.tune(**shape)
     m.profile(True)
     kinds = set()
     for ratio in (16, 64, 256, 1024):
