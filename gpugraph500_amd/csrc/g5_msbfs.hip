@@ -871,18 +871,72 @@ __device__ __forceinline__ void push_edge(uint32_t u, uint32_t uo, uint32_t v, u
     }
 }
 
+// The candidate pass of a split level (its targets, the block and wave
+// bins, are few): every pushed edge lowers the candidate of each of its new
+// (v, root) pairs, cand[v * npm + the lane's rank in lm], to its source's
+// original id with atomicMin.  The candidates start at ~0u, so one pass
+// gives the minimum-id parent; k_cand_commit then writes the pairs out.
+__device__ __forceinline__ void push_cand(uint32_t u, uint32_t uo, uint32_t v, const uint64_t *__restrict__ visited,
+                                          const uint64_t *__restrict__ frontier, uint32_t *__restrict__ cand,
+                                          int npm, uint64_t lm)
+{
+    uint64_t bits = frontier[u] & lm & ~visited[v];
+    while (bits) {
+        const int r = __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        uint32_t *slot = &cand[(int64_t)v * npm + __popcll(lm & ((1ull << r) - 1))];
+        if (*slot > uo) atomicMin(slot, uo);
+    }
+}
+
+// One thread per vertex below vend: the pushed lanes that found a candidate
+// parent become the vertex's new pairs -- parent and level written at its
+// original index, the bits ORed into next and visited (the pull kernels of
+// the level, which own these words for the other lanes, have finished).
+__global__ __launch_bounds__(kBlock) void k_cand_commit(const uint32_t *__restrict__ cand, int npm, uint64_t lm,
+                                                        int64_t vend, const uint32_t *__restrict__ new2old,
+                                                        uint64_t *__restrict__ next, uint64_t *__restrict__ visited,
+                                                        int32_t *__restrict__ parent, uint8_t *__restrict__ level,
+                                                        int64_t *__restrict__ cnt)
+{
+    const uint8_t lv = level_byte(cnt);
+    int64_t nbits = 0, nv = 0;
+    for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < vend; v += (int64_t)gridDim.x * kBlock) {
+        const uint32_t *c = cand + v * npm;
+        uint64_t bits = 0, b = lm;
+        int64_t o = 0;
+        for (int i = 0; b; ++i) {
+            const int r = __ffsll((long long)b) - 1;
+            b &= b - 1;
+            const uint32_t x = c[i];
+            if (x == ~0u) continue;
+            if (!bits) o = (int64_t)new2old[v] * 64;
+            bits |= 1ull << r;
+            parent[o + r] = (int32_t)x;
+            level[o + r] = lv;
+        }
+        if (bits) {
+            next[v] |= bits;
+            visited[v] |= bits;
+            nbits += __popcll(bits);
+            ++nv;
+        }
+    }
+    block_add(cnt + kUnitSlots * kPushUnit, {nbits, (int64_t)0, nv});
+}
+
 // A tile's sources (at most kPushTile: every queue vertex has an edge) are
 // staged in LDS first -- their first edge's offset within the tile, their
 // relabelled and original ids and their row start less their queue offset --
 // so that each edge finds its source by a binary search in LDS rather than
 // in the global prefix.
-template <bool kMin>
+template <bool kMin, bool kCand = false>
 __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ fq, const int64_t *__restrict__ qoff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ cols_orig, const uint32_t *__restrict__ new2old,
     const uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
     int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm,
-    uint64_t *__restrict__ mark, int64_t vend)
+    uint64_t *__restrict__ mark, int64_t vend, uint32_t *__restrict__ cand = nullptr, int npm = 0)
 {
     __shared__ int64_t s_lo, s_hi;
     __shared__ int32_t s_off[kPushTile];
@@ -921,7 +975,9 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                 j = a;
                 const int64_t pos = s_rb[j] + e;
                 const uint32_t v = cols[pos];
-                if (v < vend)
+                if (v >= vend) continue;
+                if (kCand) push_cand(s_u[j], s_uo[j], v, visited, frontier, cand, npm, lm);
+                else
                     push_edge<kMin>(s_u[j], s_uo[j], v, cols_orig[pos], visited, frontier, next, parent, level, lv,
                                     lm, mark);
             }
@@ -931,7 +987,9 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
                 const uint32_t u = fq[si];
                 const int64_t pos = rowptr[u] + (e - qoff[si]);
                 const uint32_t v = cols[pos];
-                if (v < vend)
+                if (v >= vend) continue;
+                if (kCand) push_cand(u, new2old[u], v, visited, frontier, cand, npm, lm);
+                else
                     push_edge<kMin>(u, new2old[u], v, cols_orig[pos], visited, frontier, next, parent, level, lv,
                                     lm, mark);
             }
@@ -1581,7 +1639,10 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
 // active bit, in act_next cleared beforehand -- and counts them from their
 // queue, so it makes no pass over all n words; a split level builds the
 // bitmaps from next, and its newly reached vertices are those of the block
-// and wave bins.
+// and wave bins.  When those bins' vertices times the pushed lanes fit in n
+// words, a split level takes the candidate path instead of the two passes:
+// one pass lowers per-pair candidates (push_cand) and k_cand_commit writes
+// the pairs out.
 int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int64_t fq_bound, uint64_t lm,
                uint32_t *fq)
 {
@@ -1601,6 +1662,26 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                        (const int64_t *)m->d_cnt, (const int64_t *)R.rowptr, m->d_qoff, fq_bound);
     G5_HIP(g5::exclusive_scan_i64(m->d_qoff, fq_bound, m->d_cnt + kFqEdges, m->d_scratch, s));
     if (whole) G5_HIP(g5::fill_u64(act_next, nw, 0, s));
+    // a split level with few pushed lanes: one candidate pass into the newly
+    // reached queue's buffer (n words; idle on split levels) and a commit
+    const int npm = __builtin_popcountll(lm);
+    if (!whole && vend * npm <= m->n) {
+        uint32_t *cand = m->d_nq;
+        if (vend) {
+            G5_HIP(g5::fill_u32(cand, vend * npm, ~0u, s));
+            hipLaunchKernelGGL((k_push_edges<false, true>), kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
+                               (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
+                               (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old,
+                               (const uint64_t *)m->d_visited, front, next, m->d_parent, m->d_level, m->d_cnt, lm,
+                               (uint64_t *)nullptr, vend, cand, npm);
+            hipLaunchKernelGGL(k_cand_commit, g5::grid_for(vend, kBlock, 4096), kBlock, 0, s, (const uint32_t *)cand,
+                               npm, lm, vend, (const uint32_t *)R.new2old, next, m->d_visited, m->d_parent,
+                               m->d_level, m->d_cnt);
+        }
+        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
+                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
+        return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
+    }
     hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
@@ -1995,7 +2076,7 @@ extern "C" int g5_msbfs_footprint(const g5_graph *g, int64_t *graph_bytes, int64
     // per vertex and lane in visited and the two frontier buffers
     const int64_t per_lane = 5 * n + (3 * n + 7) / 8;
     const int64_t all = 3 * 8 * n                      // visited, frontier buffers
-                        + 2 * (n / 64) * 8             // active bitmaps
+                        + 3 * (n / 64) * 8             // active bitmaps, level flags
                         + 3 * 4 * n + 8 * (n + 1)      // push queues, degree prefix
                         + 64 * 5 * n                   // parent and level records
                         + kDirtyCap * 8 + kStripes * kCntStride * 8 + kLaneStripes * 64 * 8 +

@@ -192,7 +192,9 @@ def test_split_levels_match_oracle(scale, shape):
     # a pull level after a push level pushes the lanes with few frontier
     # edges (into the block and wave bins; the thread kernel pulls them);
     # every lane ratio gives the oracle's levels and parents, and some ratio
-    # splits at least one level
+    # splits at least one level.  Few pushed lanes on few block and wave
+    # vertices take the candidate pass, many (or no thread bin) the two
+    # passes of a push level.
     e, ref, g = kron(scale)
     roots = O.select_roots(ref, 1 << scale)
     m = g5.MSBFS(g)
