@@ -1651,7 +1651,6 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     uint64_t *next = p == 0 ? m->d_fb : m->d_fa;
     const int64_t nw = m->n / 64;
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
-    const unsigned agrid = g5::grid_for(nw, 16, 4096);
     const bool whole = lm == ~0ull;
     // a split level pushes only into the block and wave bins: the thread
     // kernel pulls every lane of its vertices
@@ -1678,8 +1677,9 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                                npm, lm, vend, (const uint32_t *)R.new2old, next, m->d_visited, m->d_parent,
                                m->d_level, m->d_cnt);
         }
-        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
+        if (nw_push)
+            hipLaunchKernelGGL((k_build_active<true, true>), g5::grid_for(nw_push, 16, 4096), kBlock, 0, s,
+                               (const uint64_t *)next, nw_push, (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
         return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
     }
     hipLaunchKernelGGL(k_push_edges<false>, kPushGrid, kBlock, 0, s, (const uint32_t *)fq,
@@ -1702,10 +1702,11 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                        m->d_level, m->d_cnt, lm, (uint64_t *)nullptr, vend);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
-    // the whole next frontier: active bitmap, counts, live lanes, lane edges
-    if (!whole)
-        hipLaunchKernelGGL((k_build_active<true, true>), agrid, kBlock, 0, s, (const uint64_t *)next, nw,
-                           (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
+    // the block and wave bins' part of the next frontier: active bitmap,
+    // counts, live lanes (the thread kernel has done its own bin's)
+    if (!whole && nw_push)
+        hipLaunchKernelGGL((k_build_active<true, true>), g5::grid_for(nw_push, 16, 4096), kBlock, 0, s,
+                           (const uint64_t *)next, nw_push, (const int64_t *)R.rowptr, act_next, m->d_cnt, ~0ull);
     return hipGetLastError() == hipSuccess ? G5_OK : G5_E_HIP;
 }
 
@@ -1748,14 +1749,16 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
     const int64_t nw = m->n / 64;
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
+    uint64_t *act_cur = m->d_act + (p == 0 ? 0 : nw);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
     pull_bins(m, s, p, act_check, prof);
     if (prof) G5_HIP(hipEventRecord(m->ek_push[0], s));
-    // the frontier vertices with a pushed lane, in act_next until push_lanes
-    // rebuilds it
+    // the frontier vertices with a pushed lane, in the frontier's own active
+    // bitmap (the pull kernels are done with it; the thread kernel has
+    // written its words of act_next)
     hipLaunchKernelGGL((k_build_active<false, false>), g5::grid_for(nw, 16, 4096), kBlock, 0, s, front, nw,
-                       (const int64_t *)m->rl.rowptr, act_next, m->d_cnt, pm);
-    int rc = push_lanes(m, s, p, act_next, fqn, pm, m->listed == m->d_fq ? m->d_fq2 : m->d_fq);
+                       (const int64_t *)m->rl.rowptr, act_cur, m->d_cnt, pm);
+    int rc = push_lanes(m, s, p, act_cur, fqn, pm, m->listed == m->d_fq ? m->d_fq2 : m->d_fq);
     if (rc) return rc;
     if (prof) G5_HIP(hipEventRecord(m->ek_push[1], s));
     hipLaunchKernelGGL(k_level_end, 1, kBlock, 0, s, m->d_cnt, (const int64_t *)m->d_lane);
@@ -1939,8 +1942,9 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     int64_t next_verts = 0, next_edges = 0;
     for (int c = 0; c < kUnits; ++c) {
         new_bits += h[kUnitSlots * c + kNewBits];
-        // a split level counts the whole next frontier in the push unit
-        if (pm && c != kPushUnit) continue;
+        // a split level counts the block and wave bins' next frontier in the
+        // push unit (after the push), the thread bin's in its own
+        if (pm && c != kPushUnit && c != 2) continue;
         next_verts += h[kUnitSlots * c + kNextVerts];
         next_edges += h[kUnitSlots * c + kNextEdges];
     }
