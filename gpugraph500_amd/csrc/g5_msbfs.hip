@@ -95,6 +95,8 @@ struct Tune {
                                // frontier's edges < nnz / active_ratio
     int64_t lane_ratio = 64;   // a pull level pushes the lanes whose frontier
                                // edges < nnz / lane_ratio (0: never)
+    int64_t split_shift = 1;   // split levels bound the thread bin at
+                               // thread_max >> split_shift
 };
 
 // Level counters.  Per accounting unit u (the kBins pull bins, then
@@ -112,7 +114,7 @@ enum MsCnt {
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
-    kUsed = kAccum + 5, kPrevPull = kAccum + 6, kThreadLanes = kAccum + 7,
+    kUsed = kAccum + 5, kFlagWord = kAccum + 6, kThreadLanes = kAccum + 7,
     kCntStride = 40
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -149,6 +151,7 @@ struct Relabeled {
     uint64_t *head_orig = nullptr;
     uint32_t *gdeg = nullptr;
     int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
+    int64_t split_first[kBins + 1] = {0};  // the same on split levels (thread_max >> split_shift)
 };
 
 }  // namespace
@@ -207,7 +210,7 @@ struct g5_msbfs {
     uint32_t *listed = nullptr;
     int64_t listed_len = 0;
     bool listed_ok = false;
-    bool prev_pull = false;            // the last level ran the pull kernels (its level flags hold)
+    int64_t flag_word = INT64_MAX;     // the first word of lvflag the last level wrote (INT64_MAX: none)
     uint64_t *d_lvflag = nullptr;      // bit v: v's level record holds the next level's prediction
     bool last_empty = false;  // the level just run found no new (vertex, root) pair
     // continuous mode (g5_msbfs_open): lanes in use
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kThreadLanes];
     const uint64_t used = (uint64_t)cnt_all[kUsed];
-    const bool flags_in = cnt_all[kPrevPull] != 0;
+    const int64_t flags_from = cnt_all[kFlagWord];
     const PullOut out{parent, level, new2old, cols_orig, level_byte(cnt_all)};
     int64_t ngath = 0, nbits = 0, nedges = 0, nverts = 0, nnext = 0, enext = 0;
     uint64_t live = 0;
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
         const int64_t i = base + lane;
         const uint32_t v = (uint32_t)(first_v + i);
         const int64_t fword = (first_v + base) >> 6;
-        const uint64_t fw = flags_in ? lvflag[fword] : 0ull;
+        const uint64_t fw = fword >= flags_from ? lvflag[fword] : 0ull;
         KeptHits cur;
         cur.lvmode = (fw >> lane & 1) ? kLvFlagged : 0u;
         uint32_t from_head = 0;
@@ -1123,15 +1126,16 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
 }
 
 // One thread: the lanes the next block and wave kernels work on, those the
-// next thread kernel works on, the lanes in use, and whether the level flags
-// of the thread kernel's vertices are from the level just before (it ran the
-// pull kernels).
-__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t thread_lanes, uint64_t used, int prev_pull)
+// next thread kernel works on, the lanes in use, and the first word of level
+// flags that the level just before wrote (its thread kernel's bin; INT64_MAX
+// after a push level): the thread kernel trusts the flags from there on.
+__global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t thread_lanes, uint64_t used,
+                                 int64_t flag_word)
 {
     cnt[kPullLanes] = (int64_t)lanes;
     cnt[kThreadLanes] = (int64_t)thread_lanes;
     cnt[kUsed] = (int64_t)used;
-    cnt[kPrevPull] = prev_pull;
+    cnt[kFlagWord] = flag_word;
 }
 
 // one workgroup: fold the stripes into stripe 0, then advance the level
@@ -1583,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_leave(uint64_t *__restrict__ visited, c
 // (the thread kernel: cnt[kThreadLanes]).
 // They share one argument list; only the thread kernel reads `head` and
 // `head_orig`.
-void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
+void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof, const int64_t *bf)
 {
     const Relabeled &R = m->rl;
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
@@ -1593,7 +1597,7 @@ void pull_bins(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof)
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     for (int k = 0; k < kBins; ++k) {
         if (prof) (void)hipEventRecord(m->ek[k], s);
-        const int64_t first = R.bin_first[k], cnt = R.bin_first[k + 1] - first;
+        const int64_t first = bf[k], cnt = bf[k + 1] - first;
         if (!cnt) continue;
 #define G5_PULL_ARGS first, cnt, (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint64_t *)R.head, \
                      (const uint64_t *)R.head_orig, (const uint32_t *)R.cols_orig, \
@@ -1618,7 +1622,7 @@ int level_launches(g5_msbfs *m, hipStream_t s, int p, bool act_check, bool prof 
     const int64_t nw = m->n / 64;
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
-    pull_bins(m, s, p, act_check, prof);
+    pull_bins(m, s, p, act_check, prof, m->rl.bin_first);
     // the thread kernel wrote its words of the active bitmap; the block and
     // wave bins' words (and the live lanes among them) come from next
     const int64_t nw01 = (m->rl.bin_first[2] + 63) / 64;
@@ -1654,7 +1658,7 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
     const bool whole = lm == ~0ull;
     // a split level pushes only into the block and wave bins: the thread
     // kernel pulls every lane of its vertices
-    const int64_t vend = whole ? m->n : R.bin_first[2];
+    const int64_t vend = whole ? m->n : R.split_first[2];
     const int64_t nw_push = whole ? nw : (vend + 63) / 64;
     G5_HIP(g5::bitmap_to_queue(act_front, nw, fq, m->d_cnt + kFqLen, m->d_scratch, s));
     hipLaunchKernelGGL(k_fq_degrees, g5::grid_for(fq_bound, 256, 8192), 256, 0, s, (const uint32_t *)fq,
@@ -1751,7 +1755,7 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
     uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     uint64_t *act_cur = m->d_act + (p == 0 ? 0 : nw);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
-    pull_bins(m, s, p, act_check, prof);
+    pull_bins(m, s, p, act_check, prof, m->rl.split_first);
     if (prof) G5_HIP(hipEventRecord(m->ek_push[0], s));
     // the frontier vertices with a pushed lane, in the frontier's own active
     // bitmap (the pull kernels are done with it; the thread kernel has
@@ -1840,27 +1844,36 @@ int build_relabel(g5_msbfs *m)
 
 // The degree bins of the relabelled order for the current Tune bounds
 // (contiguous id ranges, degrees descending).
-int compute_bins(g5_msbfs *m)
+// bf: the bins' first vertices for the thread-bin bound thread_max
+int bins_for(g5_msbfs *m, int64_t thread_max, int64_t *bf)
 {
     hipStream_t s = m->s;
-    Relabeled &R = m->rl;
+    const Relabeled &R = m->rl;
     unsigned long long *d = (unsigned long long *)m->d_sums;  // transient scratch, free between calls
     unsigned long long hb[kBins] = {0, 0, 0};
     G5_HIP(g5::fill_u64((uint64_t *)d, kBins, 0, s));
     hipLaunchKernelGGL(k_bin_counts, g5::grid_for(m->n, 256, 16384), 256, 0, s, (const int64_t *)R.rowptr, m->n,
-                       m->tune.thread_max, m->tune.wave_max, d);
+                       thread_max, m->tune.wave_max, d);
     G5_HIP(hipGetLastError());
     G5_HIP(hipMemcpyAsync(hb, d, kBins * 8, hipMemcpyDeviceToHost, s));
     G5_HIP(hipStreamSynchronize(s));
-    R.bin_first[0] = 0;
-    for (int k = 0; k < kBins; ++k) R.bin_first[k + 1] = R.bin_first[k] + (int64_t)hb[k];
+    bf[0] = 0;
+    for (int k = 0; k < kBins; ++k) bf[k + 1] = bf[k] + (int64_t)hb[k];
     // the thread bin starts at a multiple of 64 (its waves write whole words
-    // of the active bitmap): the first bins' ends round up, moving at most 63
-    // vertices each into the bin before (any bin gives the same results)
-    const int64_t end = R.bin_first[kBins];
-    for (int k = 1; k < kBins; ++k)
-        R.bin_first[k] = std::min(end, std::max(R.bin_first[k - 1], (R.bin_first[k] + 63) / 64 * 64));
+    // of the active bitmap and of the level flags): the first bins' ends
+    // round up, moving at most 63 vertices each into the bin before (any bin
+    // gives the same results)
+    const int64_t end = bf[kBins];
+    for (int k = 1; k < kBins; ++k) bf[k] = std::min(end, std::max(bf[k - 1], (bf[k] + 63) / 64 * 64));
     return G5_OK;
+}
+
+// the bins of pull levels and those of split levels
+int compute_bins(g5_msbfs *m)
+{
+    int rc = bins_for(m, m->tune.thread_max, m->rl.bin_first);
+    if (!rc) rc = bins_for(m, m->tune.thread_max >> m->tune.split_shift, m->rl.split_first);
+    return rc;
 }
 
 void destroy_graphs(g5_msbfs *m)
@@ -1924,7 +1937,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         if (rc) return rc;
     } else {
         hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, m->front_lanes, used_lanes(m),
-                           m->prev_pull ? 1 : 0);
+                           m->flag_word);
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
             if (rc) return rc;
@@ -1987,7 +2000,8 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
             // pair; per newly reached vertex its original id and at least
             // one parent's (4 + 4 B)
             for (int k = 0; k < kBins; ++k) {
-                if (m->rl.bin_first[k + 1] == m->rl.bin_first[k]) continue;
+                const int64_t *bf = pm ? m->rl.split_first : m->rl.bin_first;
+                if (bf[k + 1] == bf[k]) continue;
                 float kms = 0;
                 G5_HIP(hipEventElapsedTime(&kms, m->ek[k], m->ek[k + 1]));
                 m->k_ms[k] += kms;
@@ -2008,7 +2022,10 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
     m->act_verts = next_verts;
     m->act_edges = next_edges;
     m->p ^= 1;
-    m->prev_pull = !push;
+    // the level flags this level's thread kernel wrote (none: a push level,
+    // or an empty thread bin)
+    const int64_t *tf = pm ? m->rl.split_first : m->rl.bin_first;
+    m->flag_word = push || tf[2] == tf[3] ? INT64_MAX : tf[2] / 64;
     return G5_OK;
 }
 
@@ -2174,7 +2191,7 @@ extern "C" int g5_msbfs_run(g5_msbfs *m, const int64_t *roots, int nroots, int32
     // the first frontier is the set of distinct roots; its edge count is not
     // known on the host, and a handful of roots always takes the push path
     m->p = 0;
-    m->prev_pull = false;
+    m->flag_word = INT64_MAX;
     m->act_edges = -1;
     m->front_lanes = nroots == 64 ? ~0ull : (1ull << nroots) - 1;
     m->lane_known = 0;
@@ -2230,7 +2247,7 @@ extern "C" int g5_msbfs_open(g5_msbfs *m)
     m->busy = 0;
     m->nroots = 0;
     m->p = 0;
-    m->prev_pull = false;
+    m->flag_word = INT64_MAX;
     m->act_verts = 0;
     m->act_edges = 0;
     m->front_lanes = 0;
@@ -2288,7 +2305,7 @@ extern "C" int g5_msbfs_join(g5_msbfs *m, const int64_t *roots, const int *lanes
         if (added[2 + i]) m->front_lanes |= 1ull << lanes[i];
     }
     m->split_ok = true;
-    m->prev_pull = false;
+    m->flag_word = INT64_MAX;
     return G5_OK;
 }
 
@@ -2555,6 +2572,7 @@ int64_t *tune_slot(g5_msbfs *m, int key)
     case G5_TUNE_PUSH_RATIO: return &m->tune.push_ratio;
     case G5_TUNE_ACTIVE_RATIO: return &m->tune.active_ratio;
     case G5_TUNE_LANE_RATIO: return &m->tune.lane_ratio;
+    case G5_TUNE_SPLIT_SHIFT: return &m->tune.split_shift;
     default: return nullptr;
     }
 }
@@ -2562,7 +2580,8 @@ int64_t *tune_slot(g5_msbfs *m, int key)
 // Installs t: new degree bins and re-captured level graphs when the bounds move.
 int apply_tune(g5_msbfs *m, const Tune &t)
 {
-    const bool rebin = t.thread_max != m->tune.thread_max || t.wave_max != m->tune.wave_max;
+    const bool rebin = t.thread_max != m->tune.thread_max || t.wave_max != m->tune.wave_max ||
+                       t.split_shift != m->tune.split_shift;
     m->tune = t;
     if (!rebin) return G5_OK;
     G5_HIP(hipStreamSynchronize(m->s));
@@ -2585,13 +2604,15 @@ extern "C" int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value)
     case G5_TUNE_WAVE_MAX: t.wave_max = value; break;
     case G5_TUNE_PUSH_RATIO: t.push_ratio = value; break;
     case G5_TUNE_ACTIVE_RATIO: t.active_ratio = value; break;
-    default: t.lane_ratio = value; break;
+    case G5_TUNE_LANE_RATIO: t.lane_ratio = value; break;
+    default: t.split_shift = value; break;
     }
     G5_REQUIRE(t.thread_max >= 0 && t.wave_max >= t.thread_max && t.wave_max <= (int64_t)1 << 40, G5_E_ARG,
                "g5_msbfs_tune: need 0 <= thread_max <= wave_max");
     G5_REQUIRE(t.push_ratio >= 1 && t.active_ratio >= 1, G5_E_ARG, "g5_msbfs_tune: ratios must be >= 1");
     G5_REQUIRE(t.lane_ratio >= 0 && t.lane_ratio <= (int64_t)1 << 40, G5_E_ARG,
                "g5_msbfs_tune: lane_ratio must be >= 0");
+    G5_REQUIRE(t.split_shift >= 0 && t.split_shift <= 40, G5_E_ARG, "g5_msbfs_tune: split_shift must be in [0, 40]");
     return apply_tune(m, t);
 }
 
@@ -2610,12 +2631,14 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     G5_REQUIRE(m && roots && nroots >= 1 && nroots <= 64 && passes >= 1 && passes <= 4, G5_E_ARG,
                "g5_msbfs_autotune: bad arguments");
     G5_REQUIRE(!(m->cont && m->busy), G5_E_STATE, "g5_msbfs_autotune: lanes in use");
-    // candidates per key; lane_ratio -1 stands for 0 (never split)
-    static const int64_t cand[5][6] = {{8, 16, 32, 64, 128, 0},
+    // candidates per key; -1 stands for 0 (lane_ratio: never split;
+    // split_shift: the same thread bin on split levels)
+    static const int64_t cand[6][6] = {{8, 16, 32, 64, 128, 0},
                                        {512, 1024, 2048, 4096, 8192, 16384},
                                        {4, 8, 16, 32, 64, 0},
                                        {1, 2, 4, 8, 0, 0},
-                                       {-1, 16, 32, 64, 128, 256}};
+                                       {-1, 16, 32, 64, 128, 256},
+                                       {-1, 1, 2, 3, 0, 0}};
     const bool prof = m->prof;
     m->prof = false;
     int n_trials = 0;
@@ -2639,7 +2662,7 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     double best_t = 0;
     int rc = measure(best, &best_t);
     for (int pass = 0; pass < passes && !rc; ++pass)
-        for (int key = 0; key < 5 && !rc; ++key)
+        for (int key = 0; key < 6 && !rc; ++key)
             for (int i = 0; i < 6 && !rc; ++i) {
                 if (cand[key][i] == 0) continue;
                 const int64_t v = cand[key][i] < 0 ? 0 : cand[key][i];
@@ -2648,7 +2671,8 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
                              : key == 1 ? &t.wave_max
                              : key == 2 ? &t.push_ratio
                              : key == 3 ? &t.active_ratio
-                                        : &t.lane_ratio;
+                             : key == 4 ? &t.lane_ratio
+                                        : &t.split_shift;
                 if (*f == v) continue;
                 *f = v;
                 if (t.wave_max < t.thread_max) continue;

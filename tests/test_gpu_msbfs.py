@@ -187,6 +187,11 @@ def test_shape_parameters_do_not_change_results(shape):
     (14, dict(thread_max=1 << 20, wave_max=1 << 20)),
     # no thread bin: the split level's push reaches every target
     (14, dict(thread_max=0, wave_max=64)),
+    # split levels with the pull levels' thread bin, and with a quarter of it
+    # (the level after a split then trusts the level flags only from the
+    # split level's thread bin on)
+    (14, dict(split_shift=0)),
+    (14, dict(thread_max=64, split_shift=2)),
 ])
 def test_split_levels_match_oracle(scale, shape):
     # a pull level after a push level pushes the lanes with few frontier
