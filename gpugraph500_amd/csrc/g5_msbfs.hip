@@ -114,7 +114,7 @@ enum MsCnt {
     kLive = kUnits * kUnitSlots,
     kAccum = kLive + 1,
     kDepth = kAccum, kNq = kAccum + 1, kFqLen = kAccum + 2, kFqEdges = kAccum + 3, kPullLanes = kAccum + 4,
-    kUsed = kAccum + 5, kFlagWord = kAccum + 6, kThreadLanes = kAccum + 7,
+    kUsed = kAccum + 5, kFlagWord = kAccum + 6, kThreadLanes = kAccum + 7, kWaveStep = kAccum + 8,
     kCntStride = 40
 };
 // Many workgroups adding into one address serialise in one L2 channel, so
@@ -122,7 +122,7 @@ enum MsCnt {
 // kStripes (stripes 320 B apart) and k_level_end folds the stripes into
 // stripe 0, which also holds the set-once slots.
 constexpr int kStripes = 32;
-static_assert(kThreadLanes < kCntStride, "counter stripe too small");
+static_assert(kWaveStep < kCntStride, "counter stripe too small");
 // Per-lane frontier edges (counted by push levels, from their new-vertex
 // queue): kLaneStripes stripes of 64 slots, summed on the host.
 constexpr int kLaneStripes = 8;
@@ -588,7 +588,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_thread(
 // frontier words before processing the steps in order.  The first step takes
 // only kWaveFirst neighbours (the other lanes idle): on dense levels most
 // vertices are done after a few, and every gather past the early exit is a
-// wasted line fill.
+// wasted line fill.  On split levels every row is scanned to its end, so
+// there the first step is a whole one (cnt[kWaveStep]).
 constexpr int kWaveUnroll = 1;
 constexpr int kWaveFirst = 16;  // 8: same time
 static_assert(kWaveFirst <= 64, "a step spans at most one neighbour per lane");
@@ -608,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
     const uint8_t lv = level_byte(cnt_all);
     const int lane = __lane_id();
     const uint64_t lanes = (uint64_t)cnt_all[kPullLanes];
+    const int64_t first_step = cnt_all[kWaveStep];
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
     int64_t nbits = 0, nedges = 0, nverts = 0, ngath = 0, nnext = 0, enext = 0;
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nlist; i += stride) {
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_wave(
         uint64_t found = 0;
         const int64_t b = rowptr[v], e = rowptr[v + 1];
         int64_t base = b;
-        int64_t step = kWaveFirst;
+        int64_t step = first_step;
         while (base < e && found != unv) {
             const int64_t lim = min(e, base + step * kWaveUnroll);
             uint32_t u[kWaveUnroll];
@@ -1126,12 +1128,15 @@ __global__ __launch_bounds__(kBlock) void k_build_active(const uint64_t *__restr
 }
 
 // One thread: the lanes the next block and wave kernels work on, those the
-// next thread kernel works on, the lanes in use, and the first word of level
+// next thread kernel works on, the lanes in use, the first word of level
 // flags that the level just before wrote (its thread kernel's bin; INT64_MAX
-// after a push level): the thread kernel trusts the flags from there on.
+// after a push level): the thread kernel trusts the flags from there on, and
+// the wave kernel's first step (kWaveFirst neighbours, or 64 on split
+// levels, whose rows are scanned to the end).
 __global__ void k_set_pull_lanes(int64_t *cnt, uint64_t lanes, uint64_t thread_lanes, uint64_t used,
-                                 int64_t flag_word)
+                                 int64_t flag_word, int64_t wave_step)
 {
+    cnt[kWaveStep] = wave_step;
     cnt[kPullLanes] = (int64_t)lanes;
     cnt[kThreadLanes] = (int64_t)thread_lanes;
     cnt[kUsed] = (int64_t)used;
@@ -1937,7 +1942,7 @@ int run_level(g5_msbfs *m, int64_t *new_bits_out, uint64_t *live_out)
         if (rc) return rc;
     } else {
         hipLaunchKernelGGL(k_set_pull_lanes, 1, 1, 0, s, m->d_cnt, m->front_lanes & ~pm, m->front_lanes, used_lanes(m),
-                           m->flag_word);
+                           m->flag_word, pm ? (int64_t)64 : (int64_t)kWaveFirst);
         if (pm) {
             int rc = split_level(m, s, p, pm, act_check, act_verts, m->prof);
             if (rc) return rc;
