@@ -15,7 +15,8 @@
 // (edge-balanced tiles, atomicOr; the edge that sets a pair's bit first
 // leaves its source in the pair's parent slot), then the new (vertex, root)
 // parent slots are lowered with atomicMin over the same edges -- the same
-// minimum-id parent.  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
+// minimum-id parent -- or, when the new vertices have fewer edges than the
+// frontier, by a scan of the new vertices' rows (k_min_rows).  A 1-bit-per-vertex active bitmap lets gathers skip vertices with
 // an empty frontier word.
 //
 // Split levels: a pull scan stops only when all of the vertex's unvisited
@@ -930,6 +931,14 @@ __global__ __launch_bounds__(kBlock) void k_cand_commit(const uint32_t *__restri
     block_add(cnt + kUnitSlots * kPushUnit, {nbits, (int64_t)0, nv});
 }
 
+// the push unit's next-frontier edges so far, summed over the stripes
+__device__ __forceinline__ int64_t push_next_edges(const int64_t *__restrict__ cnt)
+{
+    int64_t t = 0;
+    for (int st = 0; st < kStripes; ++st) t += cnt[st * kCntStride + kUnitSlots * kPushUnit + kNextEdges];
+    return t;
+}
+
 // A tile's sources (at most kPushTile: every queue vertex has an edge) are
 // staged in LDS first -- their first edge's offset within the tile, their
 // relabelled and original ids and their row start less their queue offset --
@@ -941,9 +950,13 @@ __global__ __launch_bounds__(kBlock) void k_push_edges(
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ cols_orig, const uint32_t *__restrict__ new2old,
     const uint64_t *__restrict__ visited, const uint64_t *__restrict__ frontier, uint64_t *__restrict__ next,
     int32_t *__restrict__ parent, uint8_t *__restrict__ level, int64_t *__restrict__ cnt, uint64_t lm,
-    uint64_t *__restrict__ mark, int64_t vend, uint32_t *__restrict__ cand = nullptr, int npm = 0)
+    uint64_t *__restrict__ mark, int64_t vend, uint32_t *__restrict__ cand = nullptr, int npm = 0,
+    bool rows_ok = false)
 {
     __shared__ int64_t s_lo, s_hi;
+    // the minimum pass of a pure push level whose new vertices have fewer
+    // edges than its frontier runs by rows (k_min_rows)
+    if (kMin && rows_ok && push_next_edges(cnt) < cnt[kFqEdges]) return;
     __shared__ int32_t s_off[kPushTile];
     __shared__ uint32_t s_u[kPushTile], s_uo[kPushTile];
     __shared__ int64_t s_rb[kPushTile];
@@ -1011,6 +1024,45 @@ __global__ void k_zero_listed(uint64_t *__restrict__ next, const uint32_t *__res
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
         next[list[i]] = 0;
+}
+
+// The minimum pass of a pure push level by rows, when its newly reached
+// vertices have fewer edges than its frontier (levels past the peak: few new
+// vertices of low degree): each new vertex scans its row in ascending
+// original-id order, and the first frontier neighbour of each new lane is
+// its parent.  The same minimum as the edge pass (k_push_edges<true>), which
+// runs instead otherwise; each of the two exits at once when not chosen.
+__global__ __launch_bounds__(kBlock) void k_min_rows(const uint32_t *__restrict__ nq,
+                                                     const int64_t *__restrict__ rowptr,
+                                                     const uint32_t *__restrict__ cols,
+                                                     const uint32_t *__restrict__ cols_orig,
+                                                     const uint32_t *__restrict__ new2old,
+                                                     const uint64_t *__restrict__ frontier,
+                                                     const uint64_t *__restrict__ next,
+                                                     int32_t *__restrict__ parent, const int64_t *__restrict__ cnt)
+{
+    if (push_next_edges(cnt) >= cnt[kFqEdges]) return;
+    const int64_t n = cnt[kNq];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const uint32_t v = nq[i];
+        const uint64_t w = next[v];
+        const int64_t b = rowptr[v], e = rowptr[v + 1];
+        int32_t *rec = &parent[(int64_t)new2old[v] * 64];
+        uint64_t found = 0;
+        for (int64_t j = b; j < e && found != w; j += 4) {
+            uint64_t fr[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fr[k] = j + k < e ? frontier[cols[j + k]] : 0ull;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint64_t f = fr[k] & w & ~found;
+                if (!f) continue;
+                found |= f;
+                const int32_t uo = (int32_t)cols_orig[j + k];
+                for (; f; f &= f - 1) rec[__ffsll((long long)f) - 1] = uo;
+            }
+        }
+    }
 }
 
 // A pure push level's next frontier from its queue nq (cnt[kNq] entries)
@@ -1708,7 +1760,12 @@ int push_lanes(g5_msbfs *m, hipStream_t s, int p, const uint64_t *act_front, int
                        (const int64_t *)m->d_qoff, (const int64_t *)R.rowptr, (const uint32_t *)R.cols,
                        (const uint32_t *)R.cols_orig, (const uint32_t *)R.new2old, (const uint64_t *)m->d_visited,
                        front, next, m->d_parent,
-                       m->d_level, m->d_cnt, lm, (uint64_t *)nullptr, vend);
+                       m->d_level, m->d_cnt, lm, (uint64_t *)nullptr, vend, (uint32_t *)nullptr, 0, whole);
+    if (whole)
+        hipLaunchKernelGGL(k_min_rows, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
+                           (const int64_t *)R.rowptr, (const uint32_t *)R.cols, (const uint32_t *)R.cols_orig,
+                           (const uint32_t *)R.new2old, front, (const uint64_t *)next, m->d_parent,
+                           (const int64_t *)m->d_cnt);
     hipLaunchKernelGGL(k_push_commit, kPushVertGrid, kBlock, 0, s, (const uint32_t *)m->d_nq,
                        (const uint64_t *)next, m->d_visited, m->d_cnt, lm);
     // the block and wave bins' part of the next frontier: active bitmap,
