@@ -1814,7 +1814,6 @@ int split_level(g5_msbfs *m, hipStream_t s, int p, uint64_t pm, bool act_check, 
 {
     const uint64_t *front = p == 0 ? m->d_fa : m->d_fb;
     const int64_t nw = m->n / 64;
-    uint64_t *act_next = m->d_act + (p == 0 ? nw : 0);
     uint64_t *act_cur = m->d_act + (p == 0 ? 0 : nw);
     hipLaunchKernelGGL(k_level_begin, 1, kBlock, 0, s, m->d_cnt, m->d_lane);
     pull_bins(m, s, p, act_check, prof, m->rl.split_first);
