@@ -106,9 +106,10 @@ struct Tune {
 // the next frontier's vertices and edges; then the OR of the next frontier's
 // words (the lanes still searching).  Then set-once slots: the level number,
 // the newly reached queue length (push), the frontier queue length and its
-// edge total (push), the lanes the block and wave kernels work on and those
-// the thread kernel works on (all written by the host before each pull
-// level), the lanes in use and whether the level flags hold.
+// edge total (push), the lanes the block and wave kernels work on, the lanes
+// in use, the first word of level flags the level before wrote, the lanes
+// the thread kernel works on and the wave kernel's first step (the last five
+// written before each pull level, k_set_pull_lanes).
 enum MsCnt {
     kNewBits = 0, kEdgesRead = 1, kVisitedVerts = 2, kGathers = 3, kNextVerts = 4, kNextEdges = 5,
     kUnitSlots = 6,
