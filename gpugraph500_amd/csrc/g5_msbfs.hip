@@ -33,7 +33,12 @@
 // the end at such a level anyway, so it pulls every lane, the pushed ones
 // included, and the push skips its vertices.  Each (vertex, root) pair is
 // found by one kernel, so it gets the same level and minimum-id parent as
-// before.
+// before.  As every row is scanned to its end there, split levels bound the
+// thread bin lower (thread_max >> split_shift; the wave kernel's coalesced
+// steps serve the mid-degree rows better then), start the wave kernel with
+// a whole step, and, when the block and wave bins' vertices times the pushed
+// lanes fit in n words, push through compact per-pair candidates (one
+// atomicMin pass and a commit) instead of the two passes of a push level.
 //
 // Relabelling: the search runs on a copy of the block whose vertices are
 // renumbered by descending degree class (powers of two; ties by id).
