@@ -337,7 +337,7 @@ __device__ __forceinline__ uint64_t lanes_excl_or(uint64_t f)
 // same, except on a split level: there it takes every lane, the pushed ones
 // too, since its rows are scanned to the end anyway): a vertex is done when
 // every unvisited lane among them is found.
-constexpr int kHits = 4;
+constexpr int kHits = 6;  // 4: +0.5 ms at level 3 (spilled hits' scattered stores), 8: 3 waves per SIMD, slower
 constexpr int kThreadUnroll = 8;  // 4: +0.6 ms at level 3, 16: +0.7 ms at level 4; 12 after the short first step: the same
 constexpr int kThreadFirst = 2;  // 1: same time, more neighbours read
 
