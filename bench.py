@@ -67,6 +67,7 @@ sys.path.insert(0, ROOT)
 METRIC = "GTEPS hmean, batched-64 vs single-root, scale 26, 1 MI355X; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 COPY_WORDS = 1 << 26          # copy-kernel calibration: 64 Mi uint64 words read and written
+TUNE_PASSES = 2               # coordinate-descent rounds of the batched search's autotune
 PULL_KERNELS = ("k_pull_block", "k_pull_wave", "k_pull_thread")
 DEFAULT_SCALE = {"c4": 26, "c2": 22, "c3": 26, "c1": 16}
 
@@ -196,7 +197,7 @@ def tune_probe(args):
     g = g5.Graph.kronecker(args.scale)
     timed, tune = root_sets(g, args.scale, args.roots)
     key = shapes.graph_key(g, args.scale)
-    saved = shapes.load(key)
+    saved = shapes.load(key, g5.MSBFS.TUNE_KEYS)
     if saved is not None:
         print(json.dumps({"params": saved["params"], "candidates_run": 0, "best_ms": saved.get("best_ms"),
                           "source": f"persisted ({shapes.PATH}, key {key})", "tuned_on": saved.get("tuned_on")}),
@@ -204,7 +205,10 @@ def tune_probe(args):
         return 0
     m = g5.MSBFS(g)
     m.run(tune)
-    best_ms, trials, params = m.autotune(tune)
+    # tune in the fast record placement, the one the timed searches run in
+    # (the timed run places its records the same way before it times)
+    m.place(tune, 4)
+    best_ms, trials, params = m.autotune(tune, passes=TUNE_PASSES)
     shapes.save(key, {"params": params, "best_ms": best_ms, "tuned_on": "roots 65-128 of the benchmark's draw"})
     print(json.dumps({"params": params, "candidates_run": trials, "best_ms": best_ms,
                       "source": f"autotuned now, persisted under key {key}",
@@ -408,7 +412,9 @@ def bench_c4(args, g5, traffic, traffic_info, probe_tuned):
         log(f"shapes from the tune probe: {probe_tuned['params']} ({probe_tuned.get('source')})")
     elif args.autotune == "on":
         t0 = time.perf_counter()
-        best_ms, trials, params = m.autotune(tune_roots)
+        if args.place > 0:
+            m.place(tune_roots, args.place)
+        best_ms, trials, params = m.autotune(tune_roots, passes=TUNE_PASSES)
         tuned = {"params": params, "candidates_run": trials, "best_ms": best_ms,
                  "seconds": time.perf_counter() - t0, "source": "this process",
                  "tuned_on": "roots 65-128 of the benchmark's draw"}

@@ -475,7 +475,7 @@ class MSBFS:
 
     # ---- shape parameters ----
     TUNE_KEYS = {"thread_max": 0, "wave_max": 1, "push_ratio": 2, "active_ratio": 3, "lane_ratio": 4,
-                 "split_shift": 5}
+                 "split_max": 5}
 
     def tune(self, **params):
         """Set shape parameters; the bin bounds are applied in the order that

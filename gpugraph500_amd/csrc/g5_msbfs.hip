@@ -34,8 +34,10 @@
 // included, and the push skips its vertices.  Each (vertex, root) pair is
 // found by one kernel, so it gets the same level and minimum-id parent as
 // before.  As every row is scanned to its end there, split levels bound the
-// thread bin lower (thread_max >> split_shift; the wave kernel's coalesced
-// steps serve the mid-degree rows better then), start the wave kernel with
+// thread bin by a degree of their own (split_max, at most thread_max; the
+// wave kernel's coalesced steps serve the mid-degree rows better then, while
+// on pull levels, where most scans end early, the thread bin pays up to
+// higher degrees), start the wave kernel with
 // a whole step, and, when the block and wave bins' vertices times the pushed
 // lanes fit in n words, push through compact per-pair candidates (one
 // atomicMin pass and a commit) instead of the two passes of a push level.
@@ -101,8 +103,8 @@ struct Tune {
                                // frontier's edges < nnz / active_ratio
     int64_t lane_ratio = 64;   // a pull level pushes the lanes whose frontier
                                // edges < nnz / lane_ratio (0: never)
-    int64_t split_shift = 1;   // split levels bound the thread bin at
-                               // thread_max >> split_shift
+    int64_t split_max = 16;    // split levels bound the thread bin at
+                               // min(split_max, thread_max)
 };
 
 // Level counters.  Per accounting unit u (the kBins pull bins, then
@@ -158,7 +160,7 @@ struct Relabeled {
     uint64_t *head_orig = nullptr;
     uint32_t *gdeg = nullptr;
     int64_t bin_first[kBins + 1] = {0};  // bin b holds new ids [bin_first[b], bin_first[b+1])
-    int64_t split_first[kBins + 1] = {0};  // the same on split levels (thread_max >> split_shift)
+    int64_t split_first[kBins + 1] = {0};  // the same on split levels (min(split_max, thread_max))
 };
 
 }  // namespace
@@ -1939,7 +1941,7 @@ int bins_for(g5_msbfs *m, int64_t thread_max, int64_t *bf)
 int compute_bins(g5_msbfs *m)
 {
     int rc = bins_for(m, m->tune.thread_max, m->rl.bin_first);
-    if (!rc) rc = bins_for(m, m->tune.thread_max >> m->tune.split_shift, m->rl.split_first);
+    if (!rc) rc = bins_for(m, std::min(m->tune.split_max, m->tune.thread_max), m->rl.split_first);
     return rc;
 }
 
@@ -2639,7 +2641,7 @@ int64_t *tune_slot(g5_msbfs *m, int key)
     case G5_TUNE_PUSH_RATIO: return &m->tune.push_ratio;
     case G5_TUNE_ACTIVE_RATIO: return &m->tune.active_ratio;
     case G5_TUNE_LANE_RATIO: return &m->tune.lane_ratio;
-    case G5_TUNE_SPLIT_SHIFT: return &m->tune.split_shift;
+    case G5_TUNE_SPLIT_MAX: return &m->tune.split_max;
     default: return nullptr;
     }
 }
@@ -2648,7 +2650,7 @@ int64_t *tune_slot(g5_msbfs *m, int key)
 int apply_tune(g5_msbfs *m, const Tune &t)
 {
     const bool rebin = t.thread_max != m->tune.thread_max || t.wave_max != m->tune.wave_max ||
-                       t.split_shift != m->tune.split_shift;
+                       t.split_max != m->tune.split_max;
     m->tune = t;
     if (!rebin) return G5_OK;
     G5_HIP(hipStreamSynchronize(m->s));
@@ -2672,14 +2674,15 @@ extern "C" int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value)
     case G5_TUNE_PUSH_RATIO: t.push_ratio = value; break;
     case G5_TUNE_ACTIVE_RATIO: t.active_ratio = value; break;
     case G5_TUNE_LANE_RATIO: t.lane_ratio = value; break;
-    default: t.split_shift = value; break;
+    default: t.split_max = value; break;
     }
     G5_REQUIRE(t.thread_max >= 0 && t.wave_max >= t.thread_max && t.wave_max <= (int64_t)1 << 40, G5_E_ARG,
                "g5_msbfs_tune: need 0 <= thread_max <= wave_max");
     G5_REQUIRE(t.push_ratio >= 1 && t.active_ratio >= 1, G5_E_ARG, "g5_msbfs_tune: ratios must be >= 1");
     G5_REQUIRE(t.lane_ratio >= 0 && t.lane_ratio <= (int64_t)1 << 40, G5_E_ARG,
                "g5_msbfs_tune: lane_ratio must be >= 0");
-    G5_REQUIRE(t.split_shift >= 0 && t.split_shift <= 40, G5_E_ARG, "g5_msbfs_tune: split_shift must be in [0, 40]");
+    G5_REQUIRE(t.split_max >= 0 && t.split_max <= (int64_t)1 << 40, G5_E_ARG,
+               "g5_msbfs_tune: split_max must be in [0, 2^40]");
     return apply_tune(m, t);
 }
 
@@ -2698,14 +2701,16 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     G5_REQUIRE(m && roots && nroots >= 1 && nroots <= 64 && passes >= 1 && passes <= 4, G5_E_ARG,
                "g5_msbfs_autotune: bad arguments");
     G5_REQUIRE(!(m->cont && m->busy), G5_E_STATE, "g5_msbfs_autotune: lanes in use");
-    // candidates per key; -1 stands for 0 (lane_ratio: never split;
-    // split_shift: the same thread bin on split levels)
-    static const int64_t cand[6][6] = {{8, 16, 32, 64, 128, 0},
-                                       {512, 1024, 2048, 4096, 8192, 16384},
-                                       {4, 8, 16, 32, 64, 0},
-                                       {1, 2, 4, 8, 0, 0},
-                                       {-1, 16, 32, 64, 128, 256},
-                                       {-1, 1, 2, 3, 0, 0}};
+    // candidates per key (0 ends a list); -1 stands for 0 (lane_ratio: never
+    // split; split_max: no thread bin on split levels).  The two thread-bin
+    // bounds are separate keys, so that the descent can raise the pull
+    // levels' bound while the split levels keep theirs.
+    static const int64_t cand[6][7] = {{8, 16, 32, 64, 128, 256, 512},
+                                       {512, 1024, 2048, 4096, 8192, 16384, 0},
+                                       {4, 8, 16, 32, 64, 0, 0},
+                                       {1, 2, 4, 8, 0, 0, 0},
+                                       {-1, 16, 32, 64, 128, 256, 0},
+                                       {-1, 8, 16, 32, 64, 128, 0}};
     const bool prof = m->prof;
     m->prof = false;
     int n_trials = 0;
@@ -2730,8 +2735,8 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
     int rc = measure(best, &best_t);
     for (int pass = 0; pass < passes && !rc; ++pass)
         for (int key = 0; key < 6 && !rc; ++key)
-            for (int i = 0; i < 6 && !rc; ++i) {
-                if (cand[key][i] == 0) continue;
+            for (int i = 0; i < 7 && !rc; ++i) {
+                if (cand[key][i] == 0) break;
                 const int64_t v = cand[key][i] < 0 ? 0 : cand[key][i];
                 Tune t = best;
                 int64_t *f = key == 0   ? &t.thread_max
@@ -2739,7 +2744,7 @@ extern "C" int g5_msbfs_autotune(g5_msbfs *m, const int64_t *roots, int nroots, 
                              : key == 2 ? &t.push_ratio
                              : key == 3 ? &t.active_ratio
                              : key == 4 ? &t.lane_ratio
-                                        : &t.split_shift;
+                                        : &t.split_max;
                 if (*f == v) continue;
                 *f = v;
                 if (t.wave_max < t.thread_max) continue;

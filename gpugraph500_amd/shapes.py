@@ -27,9 +27,14 @@ def _read():
         return {}
 
 
-def load(key):
-    """The persisted record for key ({"params": {...}, ...}) or None."""
-    return _read().get(key)
+def load(key, keys=()):
+    """The persisted record for key ({"params": {...}, ...}) or None; None
+    too when its params lack one of keys (a record tuned before a shape
+    parameter existed is tuned again)."""
+    rec = _read().get(key)
+    if rec is None or any(k not in rec.get("params", {}) for k in keys):
+        return None
+    return rec
 
 
 def save(key, record):

@@ -301,8 +301,8 @@ int g5_msbfs_info(g5_msbfs *m, int64_t *bins, int cap);
  * G5_TUNE_LANE_RATIO: a pull level that follows a push level pushes the
  * lanes whose frontier edges are below nnz / ratio (64; 0 = never), which
  * stops one lagging root from making every vertex scan its whole list;
- * G5_TUNE_SPLIT_SHIFT: such split levels bound the thread-per-vertex bin at
- * thread_max >> shift (1), as every row is scanned to its end there.
+ * G5_TUNE_SPLIT_MAX: such split levels bound the thread-per-vertex bin at
+ * min(split_max, thread_max) (16), as every row is scanned to its end there.
  * g5_msbfs_autotune searches these by coordinate descent over candidate
  * values (passes rounds, two timed batch searches of roots per candidate),
  * keeps the fastest and reports its time and the number of candidates run. */
@@ -312,7 +312,7 @@ enum {
     G5_TUNE_PUSH_RATIO = 2,
     G5_TUNE_ACTIVE_RATIO = 3,
     G5_TUNE_LANE_RATIO = 4,
-    G5_TUNE_SPLIT_SHIFT = 5
+    G5_TUNE_SPLIT_MAX = 5
 };
 int g5_msbfs_tune(g5_msbfs *m, int key, int64_t value);
 int g5_msbfs_get_tune(g5_msbfs *m, int key, int64_t *value);

@@ -91,7 +91,7 @@ def test_c2_codec_in_the_loop_scale22():
 @pytest.mark.parametrize("scale,shape", [
     (24, None),
     (26, None),
-    (26, dict(thread_max=64, wave_max=2048, lane_ratio=128)),
+    (26, dict(thread_max=256, wave_max=2048, lane_ratio=256, split_max=32)),
 ])
 def test_c4_batched_equals_single_root(scale, shape):
     t0 = time.time()

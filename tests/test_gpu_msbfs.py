@@ -184,14 +184,16 @@ def test_shape_parameters_do_not_change_results(shape):
     (12, {}), (14, {}),
     # every vertex in the thread kernel: a split level pushes into no bin,
     # the thread kernel pulls every lane
-    (14, dict(thread_max=1 << 20, wave_max=1 << 20)),
+    (14, dict(thread_max=1 << 20, wave_max=1 << 20, split_max=1 << 20)),
     # no thread bin: the split level's push reaches every target
     (14, dict(thread_max=0, wave_max=64)),
-    # split levels with the pull levels' thread bin, and with a quarter of it
-    # (the level after a split then trusts the level flags only from the
-    # split level's thread bin on)
-    (14, dict(split_shift=0)),
-    (14, dict(thread_max=64, split_shift=2)),
+    # split levels with the pull levels' thread bin, with a quarter and an
+    # eighth of it (the level after a split then trusts the level flags only
+    # from the split level's thread bin on), and with none
+    (14, dict(split_max=32)),
+    (14, dict(thread_max=64, split_max=16)),
+    (14, dict(thread_max=256, split_max=32)),
+    (14, dict(split_max=0)),
 ])
 def test_split_levels_match_oracle(scale, shape):
     # a pull level after a push level pushes the lanes with few frontier
@@ -253,7 +255,7 @@ def test_tune_rejects_bad_values():
     e, ref, g = kron(10)
     m = g5.MSBFS(g)
     for bad in (dict(thread_max=-1), dict(thread_max=100, wave_max=50), dict(push_ratio=0), dict(active_ratio=0),
-                dict(lane_ratio=-1)):
+                dict(lane_ratio=-1), dict(split_max=-1)):
         m2 = g5.MSBFS(g)
         with pytest.raises(g5.G5Error):
             m2.tune(**bad)

@@ -99,5 +99,8 @@ def test_persisted_shapes_round_trip(tmp_path, monkeypatch):
     shapes.save("s21-h4.5", {"params": {"thread_max": 32}})
     assert shapes.load("s20-h1.2.3") == rec
     assert shapes.load("s21-h4.5")["params"] == {"thread_max": 32}
+    # a record without one of the current keys is tuned again
+    assert shapes.load("s20-h1.2.3", ("thread_max", "wave_max")) == rec
+    assert shapes.load("s21-h4.5", ("thread_max", "split_max")) is None
     (tmp_path / "tuned.json").write_text("not json")
     assert shapes.load("s20-h1.2.3") is None
