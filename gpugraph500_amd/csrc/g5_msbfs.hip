@@ -1963,25 +1963,34 @@ uint64_t used_lanes(const g5_msbfs *m)
 }
 
 // The lanes a pull level pushes instead: those of the frontier with fewer
-// than nnz / lane_ratio frontier edges, provided they are not all of the
-// frontier's lanes and their edges together are few enough for a push level.
-// Only right after a push level or a join: after a pull level every lane has
-// had a dense level, and the unvisited vertices left are few (a split there
-// has measured slower).  0: no split.
+// than nnz / lane_ratio frontier edges, fewest edges first, as many as keep
+// their edges together few enough for a push level (nnz / push_ratio), and
+// not all of the frontier's lanes.  (Taking all such lanes or none left a
+// level whose small lanes add up past the bound a full pull, 26 against 15
+// ms of pull kernels at scale 27.)  Only right after a push level or a join:
+// after a pull level every lane has had a dense level, and the unvisited
+// vertices left are few (a split there has measured slower).  0: no split.
 uint64_t split_lanes(const g5_msbfs *m)
 {
     const int64_t nnz = m->g->nnz;
     if (m->mode != 0 || m->tune.lane_ratio <= 0 || !m->split_ok) return 0;
+    const uint64_t cand = m->front_lanes & m->lane_known;  // lanes of unknown size are pulled
+    int order[64];
+    int nc = 0;
+    for (int r = 0; r < 64; ++r)
+        if ((cand >> r & 1) && m->lane_edges[r] * m->tune.lane_ratio < nnz) order[nc++] = r;
+    std::sort(order, order + nc, [&](int a, int b) {
+        return m->lane_edges[a] != m->lane_edges[b] ? m->lane_edges[a] < m->lane_edges[b] : a < b;
+    });
     uint64_t pm = 0;
     int64_t pe = 0;
-    const uint64_t cand = m->front_lanes & m->lane_known;  // lanes of unknown size are pulled
-    for (int r = 0; r < 64; ++r)
-        if ((cand >> r & 1) && m->lane_edges[r] * m->tune.lane_ratio < nnz) {
-            pm |= 1ull << r;
-            pe += m->lane_edges[r];
-        }
-    if (pm == m->front_lanes || pe * m->tune.push_ratio >= nnz) return 0;
-    return pm;
+    for (int i = 0; i < nc; ++i) {
+        const int64_t e = m->lane_edges[order[i]];
+        if ((pe + e) * m->tune.push_ratio >= nnz) break;
+        pm |= 1ull << order[i];
+        pe += e;
+    }
+    return pm == m->front_lanes ? 0 : pm;
 }
 
 // One level at parity m->p: push or pull by the frontier's edges (a pull

@@ -194,6 +194,9 @@ def test_shape_parameters_do_not_change_results(shape):
     (14, dict(thread_max=64, split_max=16)),
     (14, dict(thread_max=256, split_max=32)),
     (14, dict(split_max=0)),
+    # a small push budget: the split takes the lanes with the fewest edges
+    # that fit in it, not all candidates or none
+    (14, dict(push_ratio=64)),
 ])
 def test_split_levels_match_oracle(scale, shape):
     # a pull level after a push level pushes the lanes with few frontier
