@@ -1965,9 +1965,9 @@ uint64_t used_lanes(const g5_msbfs *m)
 // The lanes a pull level pushes instead: those of the frontier with fewer
 // than nnz / lane_ratio frontier edges, fewest edges first, as many as keep
 // their edges together few enough for a push level (nnz / push_ratio), and
-// not all of the frontier's lanes.  (Taking all such lanes or none left a
-// level whose small lanes add up past the bound a full pull, 26 against 15
-// ms of pull kernels at scale 27.)  Only right after a push level or a join:
+// not all of the frontier's lanes.  (Taking all such lanes or none made a
+// level whose small lanes add up past the bound a full pull: 43 ms against
+// 30 ms split, at scale 27.)  Only right after a push level or a join:
 // after a pull level every lane has had a dense level, and the unvisited
 // vertices left are few (a split there has measured slower).  0: no split.
 uint64_t split_lanes(const g5_msbfs *m)
